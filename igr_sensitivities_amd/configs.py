@@ -1,0 +1,240 @@
+"""Seeded synthetic workloads (host-side set-up, numpy FP64).
+
+The reference defines its initial conditions in
+IGR_adjoints_2d/initial_conditions.jl and its paper scripts; the workloads
+here are the ones this library is measured on:
+
+  C1  1D forward: periodic [0,1], N_e=128, p=3, gamma=1.4, rho=1, u=0,
+      pressure 1000 / 0.01 / 100 on [0,0.1) / [0.1,0.9) / [0.9,1] times
+      (1 + 1e-3 eta(x)) with eta a seeded sum of 8 periodic sine modes
+      (max|eta| = 1), alpha = 10 (h/4)^2, CFL 0.1 to t = 0.038, Jacobi-50.
+  C2  2D forward: [0,1]^2, 128x128, p=3, rho=1, u=v=0, p_bg=1e-5 plus three
+      Gaussian energy deposits (sigma=0.02, total E = 1, seeded centres),
+      alpha = 10 (h/4)^2, PCG-60, 200 SSP-RK3 steps, CFL 0.4.
+  C3  C2 with J = sum w w Jx Jy rho(T) exp(-|x - c_w|^2 / 0.1^2), c_w seeded.
+  C5  2D blast + Lamb-Oseen vortex, 256x256, PCG-60, 400 steps.
+
+The time step follows the reference scripts' formula
+dt = CFL * dx / ((2p+1) * max(|u| + c)) (IGR_adjoints_1d/enzyme_adjoint.jl:272-273,
+IGR_adjoints_2d/tests/run_quasi1d_test.jl:70).
+"""
+import math
+
+import numpy as np
+
+
+def lgl(p):
+    """LGL nodes and weights (host copy for set-up; same Newton iteration as the library)."""
+    n = p + 1
+    xi = np.empty(n)
+    for i in range(n):
+        x = -math.cos(math.pi * i / p)
+        for _ in range(100):
+            P0, P1 = 1.0, x
+            for k in range(2, p + 1):
+                P0, P1 = P1, ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k
+            dx = (x * P1 - P0) / (n * P1)
+            x -= dx
+            if abs(dx) < 1e-17:
+                break
+        xi[i] = x
+    xi[0], xi[-1] = -1.0, 1.0
+    for i in range(n // 2):
+        h = 0.5 * (xi[n - 1 - i] - xi[i])
+        xi[i], xi[n - 1 - i] = -h, h
+    if n % 2:
+        xi[n // 2] = 0.0
+    w = np.empty(n)
+    for i in range(n):
+        x = xi[i]
+        P0, P1 = 1.0, x
+        for k in range(2, p + 1):
+            P0, P1 = P1, ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k
+        w[i] = 2.0 / (p * (p + 1.0) * P1 * P1)
+    return xi, w
+
+
+def coords2(p, Nex, Ney, Lx=1.0, Ly=1.0):
+    """Node coordinates, shape (Ney, Nex, np, np) (CartesianMesh2D, mesh.jl:116-123)."""
+    xi, _ = lgl(p)
+    dx, dy = Lx / Nex, Ly / Ney
+    ex = np.arange(Nex)[None, :, None, None]
+    ey = np.arange(Ney)[:, None, None, None]
+    x = ex * dx + (dx / 2.0) * (1.0 + xi[None, None, None, :])
+    y = ey * dy + (dy / 2.0) * (1.0 + xi[None, None, :, None])
+    shape = (Ney, Nex, p + 1, p + 1)
+    return np.broadcast_to(x, shape).copy(), np.broadcast_to(y, shape).copy()
+
+
+def quad2(p, Nex, Ney, Lx=1.0, Ly=1.0):
+    _, w = lgl(p)
+    Jx, Jy = Lx / Nex / 2.0, Ly / Ney / 2.0
+    wij = w[None, :] * w[:, None]
+    return np.broadcast_to(wij * Jx * Jy, (Ney, Nex, p + 1, p + 1)).copy()
+
+
+def coords1(p, Ne, L=1.0):
+    xi, _ = lgl(p)
+    dx = L / Ne
+    return np.arange(Ne)[:, None] * dx + (dx / 2.0) * (1.0 + xi[None, :])
+
+
+def max_wavespeed2(q, gamma):
+    mx, my, r, E = q
+    P = (gamma - 1.0) * (E - (mx * mx + my * my) / (2.0 * r))
+    c = np.sqrt(gamma * P / r)
+    return float(np.max(np.sqrt(mx * mx + my * my) / r + c))
+
+
+def max_wavespeed1(q, gamma):
+    m, r, E = q
+    P = (gamma - 1.0) * (E - m * m / (2.0 * r))
+    return float(np.max(np.abs(m / r) + np.sqrt(gamma * P / r)))
+
+
+def alpha_of(h, p):
+    """alpha = 10 (h/(p+1))^2."""
+    return 10.0 * (h / (p + 1)) ** 2
+
+
+def periodic_d2(x, y, xc, yc, Lx=1.0, Ly=1.0):
+    dx = np.abs(x - xc)
+    dx = np.minimum(dx, Lx - dx)
+    dy = np.abs(y - yc)
+    dy = np.minimum(dy, Ly - dy)
+    return dx * dx + dy * dy
+
+
+def config1(Ne=128, p=3, seed=1, gamma=1.4):
+    """C1 1D step data with seeded relative noise."""
+    rng = np.random.default_rng(seed)
+    x = coords1(p, Ne)
+    k = np.arange(1, 9)
+    amp = rng.standard_normal(8)
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    eta = np.sum(amp[:, None, None] * np.sin(2 * np.pi * k[:, None, None] * x[None] + ph[:, None, None]), axis=0)
+    eta /= np.max(np.abs(eta))
+    pr = np.where(x < 0.1, 1000.0, np.where(x < 0.9, 0.01, 100.0)) * (1.0 + 1e-3 * eta)
+    q = np.zeros((3, Ne, p + 1))
+    q[1] = 1.0
+    q[2] = pr / (gamma - 1.0)
+    h = 1.0 / Ne
+    lam = max_wavespeed1(q, gamma)
+    dt = 0.1 * h / ((2 * p + 1) * lam)
+    return dict(q=q, gamma=gamma, alpha=alpha_of(h, p), dt=dt, T=0.038, n_iter=50, solver="jacobi",
+                p=p, Ne=Ne)
+
+
+def config2(N=128, p=3, seed=2, gamma=1.4, n_steps=200, cfl=0.4, sigma=0.02, p_bg=1e-5):
+    """C2 2D triple energy deposit."""
+    rng = np.random.default_rng(seed)
+    x, y = coords2(p, N, N)
+    wq = quad2(p, N, N)
+    E = np.full(x.shape, p_bg / (gamma - 1.0))
+    centres = rng.uniform(0.15, 0.85, size=(3, 2))
+    for xc, yc in centres:
+        g = np.exp(-periodic_d2(x, y, xc, yc) / sigma ** 2)
+        E += (1.0 / 3.0) * g / np.sum(wq * g)
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0
+    q[3] = E
+    h = 1.0 / N
+    lam = max_wavespeed2(q, gamma)
+    dt = cfl * h / ((2 * p + 1) * lam)
+    return dict(q=q, gamma=gamma, alpha=alpha_of(h, p), dt=dt, n_steps=n_steps, n_iter=60, solver="pcg",
+                p=p, N=N, centres=centres)
+
+
+def config3_objective(seed=3):
+    """C3 window centre and width for J = int rho(T) exp(-|x-c|^2/0.1^2)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.3, 0.7, size=2)
+    return (float(c[0]), float(c[1]), 0.1, 0.0)
+
+
+def config5(N=256, p=3, seed=5, gamma=1.4, n_steps=400, cfl=0.4):
+    """C5 Gaussian pressure blast (peak 100, sigma 0.03) beside a Lamb-Oseen vortex
+    (circulation 1, core radius 0.05) on rho=1, p=1.  The vortex pressure comes
+    from radial equilibrium dp/dr = rho v^2/r integrated on a fine radial grid."""
+    rng = np.random.default_rng(seed)
+    x, y = coords2(p, N, N)
+    bx, by = rng.uniform(0.25, 0.35), rng.uniform(0.45, 0.55)
+    vx, vy = rng.uniform(0.65, 0.75), rng.uniform(0.45, 0.55)
+    Gam, a = 1.0, 0.05
+    r2 = periodic_d2(x, y, vx, vy)
+    r = np.sqrt(r2)
+    rs = np.maximum(r, 1e-12)
+    vth = Gam / (2 * np.pi * rs) * (1.0 - np.exp(-r2 / a ** 2))
+    dxv = (x - vx + 0.5) % 1.0 - 0.5
+    dyv = (y - vy + 0.5) % 1.0 - 0.5
+    u = -vth * dyv / rs
+    v = vth * dxv / rs
+    rr = np.linspace(0.0, 1.0, 20001)
+    vr = Gam / (2 * np.pi * np.maximum(rr, 1e-12)) * (1.0 - np.exp(-rr ** 2 / a ** 2))
+    integrand = np.where(rr > 0, vr ** 2 / np.maximum(rr, 1e-12), 0.0)
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (integrand[1:] + integrand[:-1]) * np.diff(rr))])
+    dp = np.interp(r, rr, cum) - cum[-1]
+    pr = 1.0 + dp + 100.0 * np.exp(-periodic_d2(x, y, bx, by) / 0.03 ** 2)
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0
+    q[0] = u
+    q[1] = v
+    q[3] = pr / (gamma - 1.0) + 0.5 * (u * u + v * v)
+    h = 1.0 / N
+    lam = max_wavespeed2(q, gamma)
+    dt = cfl * h / ((2 * p + 1) * lam)
+    disc = (float(vx), float(vy), 0.15, 0.0)
+    return dict(q=q, gamma=gamma, alpha=alpha_of(h, p), dt=dt, n_steps=n_steps, n_iter=60, solver="pcg",
+                p=p, N=N, disc=disc)
+
+
+def taylor_green2(p, Nex, Ney, A=0.1, gamma=1.4, Lx=1.0, Ly=1.0):
+    """build_taylor_green_ic! (IGR_adjoints_2d/initial_conditions.jl:105-121)."""
+    x, y = coords2(p, Nex, Ney, Lx, Ly)
+    cx, cy = 2 * np.pi * x / Lx, 2 * np.pi * y / Ly
+    ux = A * np.sin(cx) * np.cos(cy)
+    uy = -A * np.cos(cx) * np.sin(cy)
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0
+    q[0], q[1] = ux, uy
+    q[3] = 1.0 / (gamma - 1.0) + 0.5 * (ux ** 2 + uy ** 2)
+    return q
+
+
+def smooth_random2(p, Nex, Ney, seed, gamma=1.4, amp=0.1):
+    """Smooth positive random state (few low modes) for parity tests."""
+    rng = np.random.default_rng(seed)
+    x, y = coords2(p, Nex, Ney)
+
+    def fld():
+        f = np.zeros_like(x)
+        for kx in range(3):
+            for ky in range(3):
+                f += rng.standard_normal() * np.cos(2 * np.pi * (kx * x + ky * y) + rng.uniform(0, 2 * np.pi))
+        return f / 9.0
+
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0 + 0.5 * amp * fld()
+    q[0] = amp * fld() * q[2]
+    q[1] = amp * fld() * q[2]
+    pr = 1.0 + 0.5 * amp * fld()
+    q[3] = pr / (gamma - 1.0) + 0.5 * (q[0] ** 2 + q[1] ** 2) / q[2]
+    return q
+
+
+def smooth_random1(p, Ne, seed, gamma=1.4, amp=0.1):
+    rng = np.random.default_rng(seed)
+    x = coords1(p, Ne)
+
+    def fld():
+        f = np.zeros_like(x)
+        for k in range(4):
+            f += rng.standard_normal() * np.cos(2 * np.pi * k * x + rng.uniform(0, 2 * np.pi))
+        return f / 4.0
+
+    q = np.zeros((3,) + x.shape)
+    q[1] = 1.0 + 0.5 * amp * fld()
+    q[0] = amp * fld() * q[1]
+    pr = 1.0 + 0.5 * amp * fld()
+    q[2] = pr / (gamma - 1.0) + 0.5 * q[0] ** 2 / q[1]
+    return q

@@ -1,0 +1,274 @@
+"""Host-side mirror of the reference's Julia operator API over the C ABI.
+
+The reference (f-t-s/IGR_sensitivities) exposes positional, mutating Julia
+functions — run_forward!(q..., Sigma, dt, T, basis, mesh, gamma, alpha,
+n_iter, solver) and friends (IGR_adjoints_2d/forward.jl:6,
+elliptic.jl:8-10, IGR_adjoints_1d/forward.jl:5, elliptic.jl:7-8).  This
+module keeps their names, argument meaning and error behaviour:
+
+* arrays are numpy float64 arrays in the reference layout (2D field shape
+  (Ney, Nex, np, np) in C order == Julia (np, np, Nex, Ney); 1D (Ne, np));
+* `solver` is "pcg" | "jacobi" | "chebyshev" (Julia Symbols); an unknown
+  solver raises ValueError like Julia's ErrorException;
+* a negative pressure under sqrt raises DomainError, as Julia's sqrt does;
+* the in-place functions (suffix `_`) overwrite their array arguments.
+
+All arithmetic runs in libigr_hip.so on the GPU; nothing here computes.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+SOLVERS = {"pcg": 0, "jacobi": 1, "chebyshev": 2}
+
+
+class DomainError(ArithmeticError):
+    """Negative pressure under sqrt (Julia DomainError)."""
+
+
+class IGRError(RuntimeError):
+    pass
+
+
+def _solver_code(solver):
+    if isinstance(solver, str):
+        key = solver.lstrip(":").lower()
+        if key not in SOLVERS:
+            raise ValueError(f"Unknown solver: {solver} (use :pcg, :jacobi, or :chebyshev)")
+        return SOLVERS[key]
+    return int(solver)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DGBasis:
+    """Polynomial degree and LGL node count (DGBasis, IGR_adjoints_2d/basis.jl:7-12)."""
+
+    def __init__(self, p):
+        if not 1 <= p <= 4:
+            raise ValueError("p must be in 1..4")
+        self.p = int(p)
+        self.np = self.p + 1
+
+
+class CartesianMesh2D:
+    """Uniform doubly periodic mesh (IGR_adjoints_2d/mesh.jl:89-125)."""
+
+    dim = 2
+
+    def __init__(self, N_ex, N_ey, Lx, Ly, basis):
+        self.N_ex, self.N_ey, self.Lx, self.Ly = int(N_ex), int(N_ey), float(Lx), float(Ly)
+        self.basis = basis
+        self.dx, self.dy = self.Lx / self.N_ex, self.Ly / self.N_ey
+        self.Jx, self.Jy = self.dx / 2.0, self.dy / 2.0
+        self.shape = (self.N_ey, self.N_ex, basis.np, basis.np)
+        self.nv = 4
+
+
+class PeriodicMesh1D:
+    """Uniform periodic 1D mesh (IGR_adjoints_1d/mesh.jl:7-33)."""
+
+    dim = 1
+
+    def __init__(self, N_e, L, basis):
+        self.N_e, self.L = int(N_e), float(L)
+        self.basis = basis
+        self.dx = self.L / self.N_e
+        self.J = self.dx / 2.0
+        self.shape = (self.N_e, basis.np)
+        self.nv = 3
+
+
+class Context:
+    """One device context (mesh + HBM-resident state + HIP stream)."""
+
+    def __init__(self, mesh):
+        L = _lib.lib()
+        self._L = L
+        self.mesh = mesh
+        h = C.c_void_p()
+        if mesh.dim == 2:
+            rc = L.igr_create(C.byref(h), 2, mesh.basis.p, mesh.N_ex, mesh.N_ey, mesh.Lx, mesh.Ly)
+        else:
+            rc = L.igr_create(C.byref(h), 1, mesh.basis.p, mesh.N_e, 1, mesh.L, 1.0)
+        if rc != 0:
+            raise IGRError(f"igr_create failed with status {rc}")
+        self.h = h
+        self.shape = mesh.shape
+        self.nv = mesh.nv
+
+    def close(self):
+        if self.h:
+            self._L.igr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc == 0:
+            return
+        msg = self._L.igr_last_error(self.h)
+        msg = msg.decode() if msg else ""
+        if rc == _lib.IGR_EDOMAIN:
+            raise DomainError(msg)
+        if rc == _lib.IGR_EINVAL:
+            raise ValueError(msg or "invalid argument")
+        raise IGRError(f"status {rc}: {msg}")
+
+    def set_option(self, option, value):
+        self.check(self._L.igr_set_option(self.h, int(option), int(value)))
+
+    # state transfer
+    def set_state(self, q, sigma=None):
+        q = _f64(q)
+        s = None if sigma is None else _f64(sigma)
+        self.check(self._L.igr_set_state(self.h, _p(q), _p(s)))
+
+    def get_state(self):
+        q = np.empty((self.nv,) + self.shape)
+        s = np.empty(self.shape)
+        self.check(self._L.igr_get_state(self.h, _p(q), _p(s)))
+        return q, s
+
+    # operators on host arrays
+    def hyperbolic_rhs(self, gamma, q, sigma):
+        q, sigma = _f64(q), _f64(sigma)
+        dq = np.empty_like(q)
+        self.check(self._L.igr_hyperbolic_rhs(self.h, float(gamma), _p(q), _p(sigma), _p(dq)))
+        return dq
+
+    def apply_sip(self, alpha, x, rho):
+        x, rho = _f64(x), _f64(rho)
+        y = np.empty_like(x)
+        self.check(self._L.igr_apply_sip(self.h, float(alpha), _p(x), _p(rho), _p(y)))
+        return y
+
+    def igr_rhs(self, alpha, q):
+        q = _f64(q)
+        b = np.empty(self.shape)
+        self.check(self._L.igr_igr_rhs(self.h, float(alpha), _p(q), _p(b)))
+        return b
+
+    def sip_diagonal(self, alpha, rho):
+        rho = _f64(rho)
+        d = np.empty(self.shape)
+        self.check(self._L.igr_sip_diagonal(self.h, float(alpha), _p(rho), _p(d)))
+        return d
+
+    # operators on the context state
+    def solve_elliptic(self, alpha, n_iter, solver):
+        self.check(self._L.igr_solve_elliptic(self.h, float(alpha), int(n_iter), _solver_code(solver)))
+
+    def solver_trace(self, n_iter):
+        a, b = np.empty(n_iter), np.empty(n_iter)
+        self.check(self._L.igr_solver_trace(self.h, int(n_iter), _p(a), _p(b)))
+        return a, b
+
+    def euler_rhs(self, gamma, alpha, n_iter, solver):
+        dq = np.empty((self.nv,) + self.shape)
+        self.check(self._L.igr_euler_rhs(self.h, float(gamma), float(alpha), int(n_iter), _solver_code(solver),
+                                         _p(dq)))
+        return dq
+
+    def run_forward(self, dt, T, gamma, alpha, n_iter, solver):
+        ns = C.c_int(0)
+        self.check(self._L.igr_run_forward(self.h, float(dt), float(T), float(gamma), float(alpha), int(n_iter),
+                                           _solver_code(solver), C.byref(ns)))
+        return ns.value
+
+    def run_steps(self, n_steps, dt, gamma, alpha, n_iter, solver):
+        self.check(self._L.igr_run_steps(self.h, int(n_steps), float(dt), float(gamma), float(alpha), int(n_iter),
+                                         _solver_code(solver)))
+
+    def time_kernel(self, kernel, gamma, alpha, reps):
+        ms = C.c_double(0)
+        self.check(self._L.igr_time_kernel(self.h, int(kernel), float(gamma), float(alpha), int(reps),
+                                           C.byref(ms)))
+        return ms.value
+
+
+_CTX = {}
+
+
+def _ctx(mesh):
+    key = id(mesh)
+    c = _CTX.get(key)
+    if c is None or c.mesh is not mesh:
+        c = Context(mesh)
+        _CTX[key] = (c)
+    return c
+
+
+def _stack(*fields):
+    return np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.float64) for f in fields]))
+
+
+# ---------------------------------------------------------------- mirror API
+def compute_hyperbolic_rhs(q, Sigma, basis, mesh, gamma):
+    """compute_hyperbolic_rhs! (IGR_adjoints_2d/hyperbolic.jl:27; 1D :23); returns dq."""
+    return _ctx(mesh).hyperbolic_rhs(gamma, q, Sigma)
+
+
+def apply_sip_(y, x, rho, basis, mesh, alpha):
+    """apply_sip!(y, x, rho, basis, mesh, alpha) (IGR_adjoints_2d/elliptic.jl:30)."""
+    y[...] = _ctx(mesh).apply_sip(alpha, x, rho)
+
+
+def compute_igr_rhs_(b, q, basis, mesh, alpha):
+    """compute_igr_rhs! (IGR_adjoints_2d/elliptic.jl:159; 1D :126)."""
+    b[...] = _ctx(mesh).igr_rhs(alpha, q)
+
+
+def compute_sip_diagonal(rho, basis, mesh, alpha):
+    """compute_sip_diagonal (IGR_adjoints_2d/elliptic.jl:205; 1D :206)."""
+    return _ctx(mesh).sip_diagonal(alpha, rho)
+
+
+def solve_elliptic_(Sigma, q, basis, mesh, alpha, n_iter=10, solver="pcg"):
+    """solve_elliptic! (IGR_adjoints_2d/elliptic.jl:413): Sigma is the warm start, overwritten."""
+    c = _ctx(mesh)
+    code = _solver_code(solver)
+    c.set_state(q, Sigma)
+    c.solve_elliptic(alpha, n_iter, code)
+    Sigma[...] = c.get_state()[1]
+    return n_iter
+
+
+def compute_igr_euler_rhs_(dq, q, Sigma, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
+    """compute_igr_euler_rhs! (IGR_adjoints_2d/forward.jl:141; 1D :23)."""
+    c = _ctx(mesh)
+    code = _solver_code(solver)
+    c.set_state(q, Sigma)
+    dq[...] = c.euler_rhs(gamma, alpha, n_iter, code)
+    Sigma[...] = c.get_state()[1]
+
+
+def run_forward_(q, Sigma, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
+    """run_forward! (IGR_adjoints_2d/forward.jl:160; 1D :50): q and Sigma overwritten."""
+    c = _ctx(mesh)
+    code = _solver_code(solver)
+    c.set_state(q)
+    c.run_forward(dt, T, gamma, alpha, n_iter, code)
+    qo, so = c.get_state()
+    q[...] = qo
+    Sigma[...] = so
+
+
+def run_forward(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
+    """run_forward (IGR_adjoints_2d/forward.jl:203): returns (q, Sigma) at T."""
+    q = np.array(q0, dtype=np.float64, copy=True)
+    S = np.zeros(mesh.shape)
+    run_forward_(q, S, dt, T, basis, mesh, gamma, alpha, n_iter, solver)
+    return q, S

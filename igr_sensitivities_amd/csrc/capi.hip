@@ -1,0 +1,519 @@
+// C ABI (include/igr.h): context management and host orchestration of the
+// 1D and 2D forward paths.  Every entry point returns an IGR_* status and
+// keeps a message for igr_last_error().
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <tuple>
+
+#include "igr_ctx.hpp"
+#include "k1_common.hpp"
+#include "k2_common.hpp"
+#include "host_common.hpp"
+
+namespace igr {
+
+// launchers defined in the kernel translation units
+void launch2_sip(const Ctx& c, int mode, const SipArgs& a);
+void launch2_hyp(const Ctx& c, const HypArgs& a);
+int grid2_blocks(const Ctx& c);
+void launch_fill(const Ctx& c, double* x, size_t n, double v);
+void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
+void launch1_run(const Ctx& c, const K1Args* dargs, int n_inst, int block);
+
+double* Ctx::alloc(size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
+  owned.push_back(p);
+  return static_cast<double*>(p);
+}
+
+// ---------------------------------------------------------------- basis --
+// Legendre-Gauss-Lobatto nodes/weights (Newton on (1-x^2)P_p'), barycentric
+// weights and the differentiation matrix with diagonal = -(row sum of the
+// off-diagonal), as DGBasis builds them (IGR_adjoints_2d/basis.jl:19-68).
+void make_basis(int p, double* xi, double* w, double* Dm) {
+  const int n = p + 1;
+  for (int i = 0; i < n; ++i) {
+    double x = -std::cos(M_PI * i / p);
+    for (int it = 0; it < 100; ++it) {
+      double P0 = 1.0, P1 = x;
+      for (int k = 2; k <= p; ++k) { double P2 = ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k; P0 = P1; P1 = P2; }
+      const double dx = (x * P1 - P0) / (n * P1);
+      x -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    xi[i] = x;
+  }
+  xi[0] = -1.0; xi[n - 1] = 1.0;
+  for (int i = 0; i < n / 2; ++i) { const double h = 0.5 * (xi[n - 1 - i] - xi[i]); xi[i] = -h; xi[n - 1 - i] = h; }
+  if (n % 2 == 1) xi[n / 2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double x = xi[i], P0 = 1.0, P1 = x;
+    for (int k = 2; k <= p; ++k) { double P2 = ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k; P0 = P1; P1 = P2; }
+    w[i] = 2.0 / (p * (p + 1.0) * P1 * P1);
+  }
+  double lam[MAXNP];
+  for (int j = 0; j < n; ++j) {
+    lam[j] = 1.0;
+    for (int i = 0; i < n; ++i) if (i != j) lam[j] *= (xi[j] - xi[i]);
+    lam[j] = 1.0 / lam[j];
+  }
+  for (int q = 0; q < n * n; ++q) Dm[q] = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      if (i == j) continue;
+      Dm[i * n + j] = (lam[j] / lam[i]) / (xi[i] - xi[j]);
+      Dm[i * n + i] -= Dm[i * n + j];
+    }
+}
+
+Geo2 geo2(const Ctx& c) {
+  Geo2 g;
+  g.Nex = c.Nex; g.Ney = c.Ney;
+  g.dx = c.Lx / c.Nex; g.dy = c.Ly / c.Ney;
+  g.Jx = g.dx / 2.0; g.Jy = g.dy / 2.0;
+  g.invJx = 1.0 / g.Jx; g.invJy = 1.0 / g.Jy;
+  g.eta = (double)((c.p + 1) * (c.p + 1));
+  return g;
+}
+
+double cheb_lmax(int p) {
+  static const double tab[4] = {1.5, 2.0, 2.1, 2.2};   // elliptic.jl:375
+  return tab[std::min(p, 4) - 1];
+}
+
+int ensure_scal(Ctx& c, int n_iter) {
+  const size_t need = 4 * (size_t)std::max(n_iter, 1) + 1;
+  if (c.scal_n >= need) return IGR_OK;
+  c.scal = c.alloc(need);
+  if (!c.scal) return IGR_ENOMEM;
+  c.scal_n = need;
+  return IGR_OK;
+}
+
+// ------------------------------------------------------------- 2D solve --
+SipArgs sip_base(const Ctx& c, double alpha) {
+  SipArgs a;
+  a.g = geo2(c); a.B = c.basis; a.alpha = alpha;
+  a.partials = c.partials; a.counter = c.counter; a.flag = c.flag;
+  return a;
+}
+
+// Sigma solve on stage state qs (device, 4N); warm start and result c.S.
+void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc) {
+  const size_t N = c.N;
+  SipArgs a = sip_base(c, alpha);
+  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = c.S;
+  a.invR_out = c.invR; a.invM_out = c.invM; a.sc = sc; a.n_iter = n_iter;
+  if (solver == IGR_SOLVER_PCG) {
+    a.rout = c.r;
+    launch2_sip(c, M_INIT_PCG, a);
+    double* P[2] = {c.pA, c.pB};
+    for (int it = 0; it < n_iter; ++it) {
+      SipArgs k = sip_base(c, alpha);
+      k.sc = sc; k.n_iter = n_iter; k.it = it; k.first = (it == 0);
+      k.r = c.r; k.invM = c.invM; k.invR = c.invR;
+      k.pold = P[(it + 1) % 2]; k.pnew = P[it % 2]; k.y = c.Ap;
+      launch2_sip(c, M_PCG_A, k);
+      SipArgs m = sip_base(c, alpha);
+      m.sc = sc; m.n_iter = n_iter; m.it = it;
+      m.xout = c.S; m.pold = P[it % 2]; m.y = c.Ap; m.rout = c.r; m.invM = c.invM;
+      launch2_sip(c, M_PCG_B, m);
+    }
+  } else if (solver == IGR_SOLVER_JACOBI) {
+    a.b_out = c.b;
+    launch2_sip(c, M_INIT_JAC, a);
+    double* X[2] = {c.S, c.xA};
+    for (int it = 0; it < n_iter; ++it) {
+      SipArgs k = sip_base(c, alpha);
+      k.x = X[it % 2]; k.xout = X[(it + 1) % 2]; k.b = c.b; k.invM = c.invM; k.invR = c.invR;
+      k.omega = 2.0 / 3.0;
+      launch2_sip(c, M_JACOBI, k);
+    }
+    if (n_iter % 2) launch_copy(c, c.S, c.xA, N);
+  } else {
+    a.rout = c.r; a.b_out = c.b;
+    launch2_sip(c, M_INIT_CHEB, a);
+    const double lmin = 1.0, lmax = cheb_lmax(c.p);
+    const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
+    const double sigma = theta / std::max(delta, 1e-10);
+    double rc = 1.0 / sigma;
+    double* X[2] = {c.S, c.xA};
+    double* Dd[2] = {c.dA, c.dB};
+    double* R[2] = {c.r, c.r2};
+    for (int k = 1; k <= n_iter; ++k) {
+      SipArgs m = sip_base(c, alpha);
+      m.first = (k == 1); m.theta = theta;
+      if (k > 1) { const double rn = 1.0 / (2.0 * sigma - rc); m.c1 = rn * rc; m.c2 = 2.0 * rn / delta; rc = rn; }
+      m.x = X[(k - 1) % 2]; m.d = Dd[(k - 1) % 2]; m.r = R[(k - 1) % 2];
+      m.xout = X[k % 2]; m.dout = Dd[k % 2]; m.rout = R[k % 2];
+      m.b = c.b; m.invM = c.invM; m.invR = c.invR;
+      launch2_sip(c, M_CHEB, m);
+    }
+    if (n_iter % 2) launch_copy(c, c.S, c.xA, N);
+  }
+}
+
+HypArgs hyp_base(const Ctx& c, double gamma) {
+  HypArgs h;
+  h.g = geo2(c); h.B = c.basis; h.gamma = gamma; h.flag = c.flag;
+  h.q = nullptr; h.S = c.S; h.qn = c.q; h.out = nullptr; h.mode = 0; h.cdt = 0.0;
+  return h;
+}
+
+// One stage: Sigma solve (or Sigma = 0), then hyperbolic RHS + stage update.
+void stage2(Ctx& c, const double* qs, double gamma, double alpha, int n_iter, int solver, int mode, double cdt,
+            double* out) {
+  if (alpha > 0.0) solve2(c, qs, alpha, n_iter, solver, c.scal);
+  else launch_fill(c, c.S, c.N, 0.0);
+  HypArgs h = hyp_base(c, gamma);
+  h.q = qs; h.mode = mode; h.cdt = cdt; h.out = out;
+  launch2_hyp(c, h);
+}
+
+// One SSP-RK3 step (forward.jl:170-188).
+void step2(Ctx& c, double dt, double gamma, double alpha, int n_iter, int solver) {
+  stage2(c, c.q, gamma, alpha, n_iter, solver, 1, dt, c.q1);
+  stage2(c, c.q1, gamma, alpha, n_iter, solver, 2, 1.0 / 4.0 * dt, c.q2);
+  stage2(c, c.q2, gamma, alpha, n_iter, solver, 3, 2.0 / 3.0 * dt, c.q);
+}
+
+// ------------------------------------------------------------------- 1D --
+K1Args k1_base(Ctx& c, double gamma, double alpha, int n_iter, int solver) {
+  K1Args a;
+  a.Ne = c.Nex; a.dx = c.Lx / c.Nex; a.J = a.dx / 2.0; a.invJ = 1.0 / a.J;
+  a.eta = (double)((c.p + 1) * (c.p + 1));
+  a.B = c.basis; a.gamma = gamma; a.alpha = alpha; a.n_iter = n_iter; a.solver = solver;
+  a.cheb_lmax = cheb_lmax(c.p);
+  a.q = c.q; a.q1 = c.q1; a.q2 = c.q2; a.S = c.S; a.dq = c.dq;
+  a.invR = c.invR; a.invM = c.invM; a.b = c.b; a.r = c.r; a.p = c.pA; a.z = c.pB; a.Ap = c.Ap;
+  a.u = c.xA; a.g = c.dA; a.F = c.k1F; a.fstar = c.k1face; a.fjump = c.k1face + 3 * c.Nex;
+  a.favg = c.k1face + 4 * c.Nex; a.fsig = c.k1face + 5 * c.Nex; a.flag = c.flag;
+  return a;
+}
+int k1_block(const Ctx& c) {
+  const size_t N = c.N;
+  return N >= 1024 ? 1024 : (int)(((N + 63) / 64) * 64);
+}
+int run1(Ctx& c, const K1Args& a) {
+  HIPCHK(c, hipMemcpyAsync(c.k1args, &a, sizeof(K1Args), hipMemcpyHostToDevice, c.stream));
+  launch1_run(c, c.k1args, 1, k1_block(c));
+  HIPCHK(c, hipGetLastError());
+  return IGR_OK;
+}
+
+}  // namespace igr
+
+using namespace igr;
+
+// =================================================================== ABI ==
+extern "C" {
+
+int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, double ly) {
+  if (!out) return IGR_EINVAL;
+  *out = nullptr;
+  if ((dim != 1 && dim != 2) || p < 1 || p > 4 || nex < 1 || (dim == 2 && ney < 1) || !(lx > 0.0) ||
+      (dim == 2 && !(ly > 0.0)))
+    return IGR_EINVAL;
+  igr_ctx* c = new igr_ctx();
+  c->dim = dim; c->p = p; c->np = p + 1; c->Nex = nex; c->Ney = dim == 2 ? ney : 1;
+  c->Lx = lx; c->Ly = dim == 2 ? ly : 1.0;
+  c->nv = dim == 2 ? 4 : 3;
+  c->N = dim == 2 ? (size_t)c->np * c->np * nex * ney : (size_t)c->np * nex;
+  double xi[MAXNP], w[MAXNP], Dm[MAXNP * MAXNP];
+  make_basis(p, xi, w, Dm);
+  for (int i = 0; i < c->np; ++i) { c->basis.xi[i] = xi[i]; c->basis.w[i] = w[i]; }
+  for (int k = 0; k < c->np * c->np; ++k) c->basis.D[k] = Dm[k];
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return IGR_EHIP; }
+  const size_t N = c->N, nvN = c->nv * N;
+  bool ok = true;
+  auto A = [&](double*& ptr, size_t n) { ptr = c->alloc(n); ok = ok && ptr; };
+  A(c->q, nvN); A(c->S, N); A(c->q1, nvN); A(c->q2, nvN); A(c->dq, nvN);
+  A(c->invR, N); A(c->invM, N); A(c->b, N); A(c->r, N); A(c->r2, N);
+  A(c->pA, N); A(c->pB, N); A(c->Ap, N); A(c->xA, N); A(c->dA, N); A(c->dB, N); A(c->tmp, N);
+  A(c->dscalar, 64);
+  if (dim == 1) { A(c->k1F, 3 * N); A(c->k1face, 6 * (size_t)nex); }
+  // sized for the smallest tile (2x2 elements), so any tile choice fits
+  c->partials_n = 4 * (size_t)(((nex + 1) / 2) * ((c->Ney + 1) / 2)) + 64;
+  A(c->partials, c->partials_n);
+  void* pp = nullptr;
+  if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
+  if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->flag = (int*)pp; } else ok = false;
+  if (ok && hipMalloc(&pp, sizeof(K1Args) * 2) == hipSuccess) { c->owned.push_back(pp); c->k1args = (K1Args*)pp; } else ok = false;
+  if (ok) ok = ensure_scal(*c, 64) == IGR_OK;
+  if (ok) ok = hipMemset(c->counter, 0, 64) == hipSuccess && hipMemset(c->flag, 0, 64) == hipSuccess &&
+               hipMemset(c->S, 0, N * sizeof(double)) == hipSuccess &&
+               hipMemset(c->q, 0, nvN * sizeof(double)) == hipSuccess;
+  if (!ok) { igr_destroy(c); return IGR_ENOMEM; }
+  *out = c;
+  return IGR_OK;
+}
+
+int igr_destroy(igr_ctx* c) {
+  if (!c) return IGR_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_graphs(*c);
+  for (void* p : c->owned) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return IGR_OK;
+}
+
+const char* igr_last_error(const igr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int igr_sizes(const igr_ctx* c, long long* n_nodes, int* n_vars) {
+  if (!c) return IGR_EINVAL;
+  if (n_nodes) *n_nodes = (long long)c->N;
+  if (n_vars) *n_vars = c->nv;
+  return IGR_OK;
+}
+
+int igr_set_option(igr_ctx* c, int option, int value) {
+  if (!c) return IGR_EINVAL;
+  if (option == IGR_OPT_GRAPHS) { c->use_graphs = value ? 1 : 0; return IGR_OK; }
+  if (option == IGR_OPT_TILE) {
+    const int tx = value / 100, ty = value % 100;
+    const bool ok = (tx == 4 && ty == 4) || (tx == 8 && ty == 4) || (tx == 2 && ty == 2);
+    if (!ok) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
+    if (c->np == 4 && (tx != c->tile_x || ty != c->tile_y)) {
+      (void)hipStreamSynchronize(c->stream);
+      destroy_graphs(*c);
+      c->tile_x = tx; c->tile_y = ty;
+    }
+    return IGR_OK;
+  }
+  return seterr(*c, IGR_EINVAL, "unknown option");
+}
+
+int igr_set_state(igr_ctx* c, const double* q, const double* sigma) {
+  if (!c || !q) return IGR_EINVAL;
+  HIPCHK(*c, hipMemcpyAsync(c->q, q, c->nv * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (sigma) HIPCHK(*c, hipMemcpyAsync(c->S, sigma, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(*c, hipStreamSynchronize(c->stream));
+  return IGR_OK;
+}
+
+int igr_get_state(igr_ctx* c, double* q, double* sigma) {
+  if (!c) return IGR_EINVAL;
+  if (q) HIPCHK(*c, hipMemcpyAsync(q, c->q, c->nv * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (sigma) HIPCHK(*c, hipMemcpyAsync(sigma, c->S, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(*c, hipStreamSynchronize(c->stream));
+  return IGR_OK;
+}
+
+int igr_hyperbolic_rhs(igr_ctx* c, double gamma, const double* q, const double* sigma, double* dq) {
+  if (!c || !q || !sigma || !dq) return IGR_EINVAL;
+  const size_t N = c->N, nvN = c->nv * N;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  HIPCHK(*c, hipMemcpyAsync(c->q1, q, nvN * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(*c, hipMemcpyAsync(c->tmp, sigma, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->dim == 2) {
+    HypArgs h = hyp_base(*c, gamma);
+    h.q = c->q1; h.S = c->tmp; h.mode = 0; h.out = c->dq;
+    launch2_hyp(*c, h);
+  } else {
+    K1Args a = k1_base(*c, gamma, 0.0, 0, 0);
+    a.op = OP_HYP; a.q = c->q1; a.S = c->tmp;
+    rc = run1(*c, a);
+    if (rc) return rc;
+  }
+  HIPCHK(*c, hipMemcpyAsync(dq, c->dq, nvN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return end_call(*c);
+}
+
+int igr_apply_sip(igr_ctx* c, double alpha, const double* x, const double* rho, double* y) {
+  if (!c || !x || !rho || !y) return IGR_EINVAL;
+  const size_t N = c->N;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  HIPCHK(*c, hipMemcpyAsync(c->xA, x, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->dim == 2) {
+    HIPCHK(*c, hipMemcpyAsync(c->tmp, rho, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SipArgs a = sip_base(*c, alpha);
+    a.x = c->xA; a.rho = c->tmp; a.y = c->Ap;
+    launch2_sip(*c, M_APPLY, a);
+  } else {
+    // OP_APPLY reads rho from the second field of q1
+    HIPCHK(*c, hipMemcpyAsync(c->q1 + N, rho, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    K1Args a = k1_base(*c, 0.0, alpha, 0, 0);
+    a.op = OP_APPLY; a.q = c->q1; a.x_in = c->xA; a.y_out = c->Ap; a.u = c->tmp;
+    rc = run1(*c, a);
+    if (rc) return rc;
+  }
+  HIPCHK(*c, hipMemcpyAsync(y, c->Ap, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return end_call(*c);
+}
+
+// b and diag(A) on their own: the fused solver-init kernel computes both,
+// so run it with a zero warm start and read them back.
+static int init_fields(igr_ctx* c, double alpha, const double* q_host, const double* rho_host) {
+  const size_t N = c->N;
+  if (q_host) HIPCHK(*c, hipMemcpyAsync(c->q1, q_host, c->nv * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  else {
+    launch_fill(*c, c->q1, c->nv * N, 0.0);
+    HIPCHK(*c, hipMemcpyAsync(c->q1 + (c->dim == 2 ? 2 : 1) * N, rho_host, N * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+  }
+  launch_fill(*c, c->tmp, N, 0.0);
+  if (c->dim == 2) {
+    SipArgs a = sip_base(*c, alpha);
+    a.mx = c->q1; a.my = c->q1 + N; a.rho = c->q1 + 2 * N; a.x = c->tmp;
+    a.invR_out = c->invR; a.invM_out = c->invM; a.b_out = c->b;
+    launch2_sip(*c, M_INIT_JAC, a);
+  } else {
+    K1Args a = k1_base(*c, 0.0, alpha, 0, IGR_SOLVER_JACOBI);
+    a.op = OP_SOLVE; a.q = c->q1; a.S = c->tmp;
+    return run1(*c, a);
+  }
+  return IGR_OK;
+}
+
+int igr_igr_rhs(igr_ctx* c, double alpha, const double* q, double* b) {
+  if (!c || !q || !b) return IGR_EINVAL;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = init_fields(c, alpha, q, nullptr);
+  if (rc) return rc;
+  HIPCHK(*c, hipMemcpyAsync(b, c->b, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return end_call(*c);
+}
+
+int igr_sip_diagonal(igr_ctx* c, double alpha, const double* rho, double* diag) {
+  if (!c || !rho || !diag) return IGR_EINVAL;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = init_fields(c, alpha, nullptr, rho);
+  if (rc) return rc;
+  std::vector<double> im(c->N);
+  HIPCHK(*c, hipMemcpyAsync(im.data(), c->invM, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  rc = end_call(*c);
+  if (rc) return rc;
+  for (size_t n = 0; n < c->N; ++n) diag[n] = 1.0 / im[n];
+  return IGR_OK;
+}
+
+int igr_solve_elliptic(igr_ctx* c, double alpha, int n_iter, int solver) {
+  if (!c || n_iter < 0 || solver < 0 || solver > 2) return c ? seterr(*c, IGR_EINVAL, "unknown solver") : IGR_EINVAL;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = ensure_scal(*c, n_iter);
+  if (rc) return rc;
+  if (c->dim == 2) {
+    solve2(*c, c->q, alpha, n_iter, solver, c->scal);
+  } else {
+    K1Args a = k1_base(*c, 0.0, alpha, n_iter, solver);
+    a.op = OP_SOLVE; a.trace = c->scal;
+    rc = run1(*c, a);
+    if (rc) return rc;
+  }
+  return end_call(*c);
+}
+
+int igr_solver_trace(igr_ctx* c, int n_iter, double* a, double* beta) {
+  if (!c || n_iter < 0 || (size_t)(4 * n_iter + 1) > c->scal_n) return IGR_EINVAL;
+  std::vector<double> h(4 * (size_t)n_iter + 1);
+  HIPCHK(*c, hipMemcpy(h.data(), c->scal, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n_iter; ++k) {
+    if (c->dim == 2) { if (a) a[k] = h[2 * n_iter + 1 + k]; if (beta) beta[k] = h[3 * n_iter + 1 + k]; }
+    else { if (a) a[k] = h[k]; if (beta) beta[k] = h[n_iter + k]; }
+  }
+  return IGR_OK;
+}
+
+int igr_euler_rhs(igr_ctx* c, double gamma, double alpha, int n_iter, int solver, double* dq) {
+  if (!c || n_iter < 0) return IGR_EINVAL;
+  if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = ensure_scal(*c, n_iter);
+  if (rc) return rc;
+  if (c->dim == 2) {
+    if (alpha > 0.0) solve2(*c, c->q, alpha, n_iter, solver, c->scal);
+    else launch_fill(*c, c->S, c->N, 0.0);
+    HypArgs h = hyp_base(*c, gamma);
+    h.q = c->q; h.mode = 0; h.out = c->dq;
+    launch2_hyp(*c, h);
+  } else {
+    K1Args a = k1_base(*c, gamma, alpha, n_iter, solver);
+    a.op = OP_RHS;
+    rc = run1(*c, a);
+    if (rc) return rc;
+  }
+  if (dq) HIPCHK(*c, hipMemcpyAsync(dq, c->dq, c->nv * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return end_call(*c);
+}
+
+int igr_run_steps(igr_ctx* c, int n_steps, double dt, double gamma, double alpha, int n_iter, int solver) {
+  if (!c || n_steps < 0 || n_iter < 0) return IGR_EINVAL;
+  if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = run_steps_impl(*c, n_steps, dt, gamma, alpha, n_iter, solver, 1, 0);
+  if (rc) return rc;
+  return end_call(*c);
+}
+
+int igr_run_forward(igr_ctx* c, double dt, double t_end, double gamma, double alpha, int n_iter, int solver,
+                    int* n_steps_out) {
+  if (!c || !(dt > 0.0) || !(t_end > 0.0) || n_iter < 0) return IGR_EINVAL;
+  if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
+  const int n_steps = (int)std::ceil(t_end / dt);
+  dt = t_end / n_steps;
+  if (n_steps_out) *n_steps_out = n_steps;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = run_steps_impl(*c, n_steps, dt, gamma, alpha, n_iter, solver, 1, 1);
+  if (rc) return rc;
+  return end_call(*c);
+}
+
+int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps, double* ms) {
+  if (!c || !ms || reps < 1 || kernel < 0 || kernel > 2 || c->dim != 2) return IGR_EINVAL;
+  int rc = begin_call(*c);
+  if (rc) return rc;
+  rc = ensure_scal(*c, 4);
+  if (rc) return rc;
+  const size_t N = c->N;
+  // valid solver buffers for the current state, on copies so the state is untouched
+  launch_copy(*c, c->xA, c->S, N);
+  SipArgs a = sip_base(*c, alpha);
+  a.mx = c->q; a.my = c->q + N; a.rho = c->q + 2 * N; a.x = c->xA;
+  a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = 4;
+  launch2_sip(*c, M_INIT_PCG, a);
+  SipArgs k = sip_base(*c, alpha);
+  k.sc = c->scal; k.n_iter = 4; k.it = 0; k.first = 1;
+  k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = c->pB; k.pnew = c->pA; k.y = c->Ap;
+  launch2_sip(*c, M_PCG_A, k);
+  SipArgs m = sip_base(*c, alpha);
+  m.sc = c->scal; m.n_iter = 4; m.it = 0;
+  m.xout = c->xA; m.pold = c->pA; m.y = c->Ap; m.rout = c->r2; m.invM = c->invM;
+  launch_copy(*c, c->r2, c->r, N);
+  k.first = 0; k.it = 1; k.pold = c->pA; k.pnew = c->pB;
+  HypArgs h = hyp_base(*c, gamma);
+  h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;
+  launch_copy(*c, c->q1, c->q, c->nv * N);
+  hipEvent_t e0, e1;
+  HIPCHK(*c, hipEventCreate(&e0));
+  HIPCHK(*c, hipEventCreate(&e1));
+  auto once = [&]() {
+    if (kernel == 0) launch2_sip(*c, M_PCG_A, k);
+    else if (kernel == 1) launch2_sip(*c, M_PCG_B, m);
+    else launch2_hyp(*c, h);
+  };
+  for (int w = 0; w < 3; ++w) once();
+  HIPCHK(*c, hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) once();
+  HIPCHK(*c, hipEventRecord(e1, c->stream));
+  HIPCHK(*c, hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(*c, hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return end_call(*c);
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Host-side context: one mesh, its device-resident state and workspace,
+// one HIP stream.  A context is not thread-safe (the reference is serial).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstdio>
+#include <map>
+#include <utility>
+
+#include "../../include/igr.h"
+#include "igr_device.hpp"
+
+namespace igr {
+
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+};
+
+struct Ctx {
+  int dim = 2, p = 3, np = 4, Nex = 1, Ney = 1;
+  double Lx = 1.0, Ly = 1.0;
+  size_t N = 0;         // nodes per field
+  int nv = 4;           // conserved variables
+  hipStream_t stream = nullptr;
+  BasisK basis{};
+  std::string err;
+
+  // state (device): q[nv*N], Sigma[N]
+  double* q = nullptr;
+  double* S = nullptr;
+  // RK3 stage buffers and RHS
+  double* q1 = nullptr;
+  double* q2 = nullptr;
+  double* dq = nullptr;
+  // elliptic workspace
+  double* invR = nullptr;
+  double* invM = nullptr;
+  double* b = nullptr;
+  double* r = nullptr;
+  double* r2 = nullptr;
+  double* pA = nullptr;
+  double* pB = nullptr;
+  double* Ap = nullptr;
+  double* xA = nullptr;
+  double* dA = nullptr;
+  double* dB = nullptr;
+  double* tmp = nullptr;
+  double* scal = nullptr;     // per-solve scalars
+  size_t scal_n = 0;
+  double* partials = nullptr; // grid reduction partials
+  size_t partials_n = 0;
+  unsigned* counter = nullptr;
+  int* flag = nullptr;        // domain flag
+  double* dscalar = nullptr;  // small device scalar scratch
+  std::vector<void*> owned;   // every hipMalloc, freed by igr_destroy
+
+  // 1D whole-instance kernel workspace
+  double* k1F = nullptr;      // 3N fluxes
+  double* k1face = nullptr;   // 6*Ne face values
+  struct K1Args* k1args = nullptr;  // device copy of the launch arguments
+
+  // tile shape chosen by the autotuner (2D): elements per workgroup
+  int tile_x = 4, tile_y = 4;
+  int use_graphs = 1;
+
+  // captured SSP-RK3 steps, keyed by their scalar parameters
+  struct GraphKey {
+    double dt, gamma, alpha; int n_iter, solver;
+    bool operator<(const GraphKey& o) const {
+      if (dt != o.dt) return dt < o.dt;
+      if (gamma != o.gamma) return gamma < o.gamma;
+      if (alpha != o.alpha) return alpha < o.alpha;
+      if (n_iter != o.n_iter) return n_iter < o.n_iter;
+      return solver < o.solver;
+    }
+  };
+  std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+
+  double* alloc(size_t n_doubles);
+};
+
+}  // namespace igr
+
+struct igr_ctx : igr::Ctx {};

@@ -1,0 +1,101 @@
+// Device-side helpers shared by the 1D and 2D kernels: basis constants
+// passed by value in the kernel-argument segment (scalar loads), wavefront
+// and workgroup reductions, and the deterministic "last workgroup finishes
+// the reduction" pattern used for grid-wide dot products.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace igr {
+
+constexpr int MAXNP = 5;      // p <= 4
+constexpr int WAVE = 64;      // CDNA wavefront width
+
+struct BasisK {
+  double D[MAXNP * MAXNP];    // D[i*NP + k] = D_{ik}, packed with stride NP
+  double w[MAXNP];
+  double xi[MAXNP];
+};
+
+// Wavefront sum (64 lanes) by butterfly shuffles; every lane gets the sum.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+// Workgroup sum of NV values; result valid in thread 0.  `red` is LDS of
+// at least NV * (blockDim/64) doubles.  Fixed summation order.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += red[k * nw + w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid-wide reduction of NV values.  Each workgroup writes its partial sums
+// to partials[k*nblocks + bid]; the last workgroup to arrive (device-scope
+// counter) sums all partials in a fixed order and calls fin(totals) on
+// thread 0, then re-arms the counter.  Deterministic run to run.
+template <int NV, class Fin>
+__device__ __forceinline__ void grid_sum(double (&v)[NV], double* red, double* partials,
+                                         unsigned* counter, int nblocks, int bid, Fin fin) {
+  block_sum<NV>(v, red);
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * nblocks + bid] = v[k];
+    __threadfence();
+    unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (old == (unsigned)(nblocks - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      s += __hip_atomic_load(&partials[(size_t)k * nblocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc[k] = s;
+  }
+  block_sum<NV>(acc, red);
+  if (threadIdx.x == 0) {
+    fin(acc);
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Julia max(x, y): returns x on ties.
+__device__ __forceinline__ double fmax_first(double x, double y) { return (y > x) ? y : x; }
+
+// Device flag set when a pressure under sqrt is negative or a value is not
+// finite; the host turns it into IGR_EDOMAIN.
+__device__ __forceinline__ void flag_domain(int* flag) {
+  __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8
+// XCDs, so workgroup b runs on XCD b % 8.  Giving each XCD a contiguous band
+// of tiles keeps neighbouring tiles (halo reads) in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + (b / 8);
+}
+
+}  // namespace igr

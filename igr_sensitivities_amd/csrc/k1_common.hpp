@@ -1,0 +1,63 @@
+// Shared 1D definitions: kernel argument block and pointwise Euler physics.
+#pragma once
+#include "igr_device.hpp"
+
+namespace igr {
+
+enum K1Op { OP_RUN = 0, OP_SOLVE = 1, OP_RHS = 2, OP_HYP = 3, OP_APPLY = 4 };
+
+// One 1D instance (one workgroup).  All pointers are device memory.
+struct K1Args {
+  int op = OP_RUN;
+  int Ne = 1;
+  double J = 0.5, invJ = 2.0, dx = 1.0, eta = 16.0;
+  BasisK B;
+  double gamma = 1.4, alpha = 0.0;
+  int n_iter = 0, solver = 0;
+  double cheb_lmax = 2.1;
+  int n_steps = 0;
+  double dt = 0.0;
+  int zero_sigma = 1, final_solve = 0;
+  double* q = nullptr;     // state (3N): mu, rho, E
+  double* q1 = nullptr;
+  double* q2 = nullptr;
+  double* S = nullptr;     // Sigma (N), warm start in/out
+  double* dq = nullptr;    // RHS output (3N)
+  double* invR = nullptr;
+  double* invM = nullptr;
+  double* b = nullptr;
+  double* r = nullptr;
+  double* p = nullptr;
+  double* z = nullptr;
+  double* Ap = nullptr;
+  double* u = nullptr;
+  double* g = nullptr;
+  double* F = nullptr;     // 3N
+  double* fstar = nullptr; // 3Ne
+  double* fjump = nullptr; // Ne
+  double* favg = nullptr;
+  double* fsig = nullptr;
+  double* trace = nullptr; // 2*n_iter PCG scalars (OP_SOLVE)
+  const double* x_in = nullptr;   // OP_APPLY
+  double* y_out = nullptr;
+  int* flag = nullptr;
+};
+
+#pragma clang fp contract(off)
+// polytropic_flux, IGR_adjoints_1d/euler.jl:26-35
+__device__ __forceinline__ void flux1(double g, double m, double r, double E, double S, double f[3]) {
+  const double P = (g - 1.0) * (E - m * m / (2.0 * r));
+  const double pb = P + S;
+  f[0] = m * m / r + pb; f[1] = m; f[2] = (E + pb) * m / r;
+}
+// max_wavespeed, euler.jl:52-56 (sound_speed :42-45; negative argument = DomainError)
+__device__ __forceinline__ double wave1(double g, const double q[4], int* flag) {
+  const double u = q[0] / q[1];
+  const double P = (g - 1.0) * (q[2] - q[0] * q[0] / (2.0 * q[1]));
+  const double arg = g * P / q[1];
+  if (!(arg >= 0.0)) flag_domain(flag);
+  return fabs(u) + sqrt(arg);
+}
+#pragma clang fp contract(fast)
+
+}  // namespace igr

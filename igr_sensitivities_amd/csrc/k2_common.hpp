@@ -1,0 +1,150 @@
+// Shared 2D definitions: geometry, kernel argument blocks, tile/halo
+// addressing and the pointwise Euler physics.
+#pragma once
+#include "igr_device.hpp"
+
+namespace igr {
+
+struct Geo2 {
+  int Nex, Ney;
+  double Jx, Jy, invJx, invJy, dx, dy;
+  double eta;   // SIP penalty (p+1)^2, IGR_adjoints_2d/elliptic.jl:40
+};
+
+enum SipMode {
+  M_APPLY = 0, M_INIT_PCG = 1, M_INIT_CHEB = 2, M_INIT_JAC = 3,
+  M_PCG_A = 4, M_JACOBI = 5, M_CHEB = 6, M_PCG_B = 7
+};
+
+struct SipArgs {
+  Geo2 g;
+  BasisK B;
+  double alpha;
+  const double* x = nullptr;     // vector (APPLY/JACOBI/CHEB) or warm start (INIT)
+  const double* rho = nullptr;   // APPLY, INIT
+  const double* mx = nullptr;    // INIT
+  const double* my = nullptr;    // INIT
+  const double* invR = nullptr;
+  const double* r = nullptr;
+  const double* invM = nullptr;
+  const double* pold = nullptr;
+  const double* b = nullptr;
+  const double* d = nullptr;
+  double* sc = nullptr;          // per-solve scalars (rz | pAp | a | beta)
+  int first = 0, it = 0, n_iter = 0;
+  double c1 = 0.0, c2 = 0.0, theta = 1.0, omega = 0.0;
+  double* y = nullptr;
+  double* pnew = nullptr;
+  double* xout = nullptr;
+  double* dout = nullptr;
+  double* rout = nullptr;
+  double* invR_out = nullptr;
+  double* invM_out = nullptr;
+  double* b_out = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  int* flag = nullptr;
+};
+
+struct HypArgs {
+  Geo2 g;
+  BasisK B;
+  double gamma;
+  const double* q;    // stage input state (4 fields)
+  const double* S;    // Sigma
+  const double* qn;   // q^n (stages 2 and 3)
+  double* out;        // dq (mode 0) or the stage output state
+  int mode;           // 0: dq ; 1, 2, 3: SSP-RK3 stage
+  double cdt;         // dt, (1/4)dt or (2/3)dt, as the reference forms it
+  int* flag;
+};
+
+// Tile of TX x TY elements plus a one-element halo ring (no corners).
+// LDS slots: tile s = ly*TX + lx, then left[TY], right[TY], bottom[TX], top[TX].
+template <int NP, int TX, int TY>
+struct TileInfo {
+  static constexpr int T = TX * TY, NP2 = NP * NP;
+  int Nex, Ney, ex0, ey0, nvx, nvy;
+  __device__ TileInfo(const Geo2& g) {
+    const int ntx = (g.Nex + TX - 1) / TX;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % ntx, ty = tile / ntx;
+    Nex = g.Nex; Ney = g.Ney;
+    ex0 = tx * TX; ey0 = ty * TY;
+    nvx = min(TX, Nex - ex0); nvy = min(TY, Ney - ey0);
+  }
+  __device__ size_t gidx(int s, int n) const {
+    const int lx = s % TX, ly = s / TX;
+    return (size_t)n + (size_t)NP2 * ((size_t)(ex0 + lx) + (size_t)Nex * (ey0 + ly));
+  }
+  __device__ int nb_left(int s) const { const int lx = s % TX, ly = s / TX; return lx > 0 ? s - 1 : T + ly; }
+  __device__ int nb_right(int s) const { const int lx = s % TX, ly = s / TX; return lx < nvx - 1 ? s + 1 : T + TY + ly; }
+  __device__ int nb_bottom(int s) const { const int lx = s % TX, ly = s / TX; return ly > 0 ? s - TX : T + 2 * TY + lx; }
+  __device__ int nb_top(int s) const { const int lx = s % TX, ly = s / TX; return ly < nvy - 1 ? s + TX : T + 2 * TY + TX + lx; }
+  // element of LDS slot sl; false when the slot is outside this tile's rows/columns
+  __device__ bool slot_elem(int sl, int& gex, int& gey) const {
+    if (sl < T) {
+      const int lx = sl % TX, ly = sl / TX;
+      if (lx >= nvx || ly >= nvy) return false;
+      gex = ex0 + lx; gey = ey0 + ly; return true;
+    }
+    int h = sl - T;
+    if (h < TY) { if (h >= nvy) return false; gex = (ex0 - 1 + Nex) % Nex; gey = ey0 + h; return true; }
+    h -= TY;
+    if (h < TY) { if (h >= nvy) return false; gex = (ex0 + nvx) % Nex; gey = ey0 + h; return true; }
+    h -= TY;
+    if (h < TX) { if (h >= nvx) return false; gex = ex0 + h; gey = (ey0 - 1 + Ney) % Ney; return true; }
+    h -= TX;
+    if (h >= nvx) return false;
+    gex = ex0 + h; gey = (ey0 + nvy) % Ney; return true;
+  }
+  __device__ bool slot_global(int sl, int nn, size_t& gi) const {
+    int gex, gey;
+    if (!slot_elem(sl, gex, gey)) return false;
+    gi = (size_t)nn + (size_t)NP2 * ((size_t)gex + (size_t)Nex * gey);
+    return true;
+  }
+  // k-th node of the face of halo element h that touches the tile
+  __device__ bool halo_face_global(int h, int k, size_t& gi) const {
+    int gex, gey, nn;
+    if (!slot_elem(T + h, gex, gey)) return false;
+    if (h < TY) nn = (NP - 1) + NP * k;              // left neighbour: its right face
+    else if (h < 2 * TY) nn = NP * k;                // right neighbour: its left face
+    else if (h < 2 * TY + TX) nn = k + NP * (NP - 1); // bottom neighbour: its top face
+    else nn = k;                                      // top neighbour: its bottom face
+    gi = (size_t)nn + (size_t)NP2 * ((size_t)gex + (size_t)Nex * gey);
+    return true;
+  }
+};
+
+// ---- pointwise Euler physics, reference evaluation order, no contraction
+#pragma clang fp contract(off)
+// polytropic_flux_x / _y, IGR_adjoints_2d/euler.jl:156-186.  q = (mx,my,rho,E,Sigma)
+__device__ __forceinline__ void flux_x2(double gam, const double q[5], double f[4]) {
+  const double P = (gam - 1.0) * (q[3] - (q[0] * q[0] + q[1] * q[1]) / (2.0 * q[2]));
+  const double pb = P + q[4];
+  const double ux = q[0] / q[2];
+  f[0] = q[0] * ux + pb; f[1] = q[1] * ux; f[2] = q[0]; f[3] = (q[3] + pb) * ux;
+}
+__device__ __forceinline__ void flux_y2(double gam, const double q[5], double f[4]) {
+  const double P = (gam - 1.0) * (q[3] - (q[0] * q[0] + q[1] * q[1]) / (2.0 * q[2]));
+  const double pb = P + q[4];
+  const double uy = q[1] / q[2];
+  f[0] = q[0] * uy; f[1] = q[1] * uy + pb; f[2] = q[1]; f[3] = (q[3] + pb) * uy;
+}
+// sound_speed, euler.jl:193-196; a negative argument is Julia's DomainError
+__device__ __forceinline__ double sound2(double gam, const double q[5], int* flag) {
+  const double P = (gam - 1.0) * (q[3] - (q[0] * q[0] + q[1] * q[1]) / (2.0 * q[2]));
+  const double arg = gam * P / q[2];
+  if (!(arg >= 0.0)) flag_domain(flag);
+  return sqrt(arg);
+}
+// f* = (fL + fR)/2 + lam (qL - qR)/2, euler.jl:230-233
+__device__ __forceinline__ void llf_combine(const double fL[4], const double fR[4], const double qL[5],
+                                            const double qR[5], double lam, double f[4]) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
+}
+#pragma clang fp contract(fast)
+
+}  // namespace igr

@@ -1,0 +1,544 @@
+// 2D forward kernels for gfx950 (MI355X), FP64.
+//
+// Data layout in HBM: one array per field (SoA), nodes in the reference's
+// column-major order, so one element of p=3 is 16 contiguous doubles
+// (128 B) and a wavefront of 64 lanes reads 4 whole elements per field.
+//
+// Work decomposition: a workgroup owns a TX x TY tile of elements and one
+// thread per node (TX*TY*NP^2 threads; 4x4 tiles of p=3 give 256).  The
+// tile plus a one-element halo ring (no corners) is staged in LDS.  Tiles
+// are assigned XCD-contiguously (xcd_tile) so halo reads of neighbouring
+// tiles hit the same L2.
+//
+// Kernels:
+//   k2_sip<MODE>   matrix-free SIP apply y = A(rho) x  (apply_sip!,
+//                  IGR_adjoints_2d/elliptic.jl:30-145) in gather form, fused
+//                  with what surrounds it in each solver:
+//                    INIT_*  IGR source b (compute_igr_rhs!, :159-197),
+//                            diag(A) (compute_sip_diagonal, :205-264),
+//                            1/rho, 1/M, r0 = b - A x0 and <r0, z0>
+//                    PCG_A   p = z + beta p on the fly (own + halo nodes),
+//                            Ap = A p, <p, Ap>       (pcg_fixed!, :329-347)
+//                    JACOBI  x' = x + w (b - A x)/M  (jacobi_fixed!, :354-361)
+//                    CHEB    d' = c1 d + c2 z, x' = x + d', r' = b - A x'
+//                            (chebyshev_fixed!, :370-397)
+//   k2_pcg_b       x += a p, r -= a Ap, <r, z>   (pcg_fixed!, :339-343)
+//   k2_hyp         hyperbolic RHS (compute_hyperbolic_rhs!,
+//                  IGR_adjoints_2d/hyperbolic.jl:27-119) fused with the
+//                  SSP-RK3 stage combination (forward.jl:170-188), each face
+//                  LLF flux computed once per tile (euler.jl:222-258).
+// Dot products finish with the deterministic last-workgroup reduction.
+#include "igr_ctx.hpp"
+#include "k2_common.hpp"
+
+namespace igr {
+
+// ------------------------------------------------------------------------
+// SIP family
+// ------------------------------------------------------------------------
+template <int NP, int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
+  constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
+  constexpr bool INIT = (MODE == M_INIT_PCG || MODE == M_INIT_CHEB || MODE == M_INIT_JAC);
+  __shared__ double vs[NS * NP2];
+  __shared__ double irs[NS * NP2];
+  __shared__ double gx[T * NP2], gy[T * NP2];
+  __shared__ double sxl[T * NP], sxr[T * NP], syb[T * NP], syt[T * NP];
+  __shared__ double us[INIT ? T * NP2 : 1], ws_[INIT ? T * NP2 : 1];
+  __shared__ double red[4 * (NT / WAVE + 1)];
+
+  const Geo2 g = a.g;
+  const BasisK& B = a.B;
+  TileInfo<NP, TX, TY> t(g);
+  const int tid = threadIdx.x;
+  const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
+  const int lx = s % TX, ly = s / TX;
+  const bool valid = lx < t.nvx && ly < t.nvy;
+  const size_t gme = valid ? t.gidx(s, n) : 0;
+
+  double own_d = 0.0;
+  double beta = 0.0;
+  if constexpr (MODE == M_PCG_A) {
+    if (!a.first) beta = a.sc[3 * a.n_iter + 1 + a.it - 1];
+  }
+  // ---- stage the vector, 1/rho (and velocities for INIT) in LDS
+  for (int idx = tid; idx < NS * NP2; idx += NT) {
+    const int sl = idx / NP2, nn = idx % NP2;
+    double v = 0.0, ir = 0.0;
+    size_t gi;
+    if (t.slot_global(sl, nn, gi)) {
+      if constexpr (MODE == M_APPLY) {
+        v = a.x[gi]; ir = 1.0 / a.rho[gi];
+      } else if constexpr (INIT) {
+        v = a.x[gi];
+        double rh = a.rho[gi];
+        ir = 1.0 / rh;
+        if (sl < T) { us[idx] = a.mx[gi] / rh; ws_[idx] = a.my[gi] / rh; }
+      } else if constexpr (MODE == M_PCG_A) {
+        double z = a.r[gi] * a.invM[gi];
+        v = a.first ? z : z + beta * a.pold[gi];
+        ir = a.invR[gi];
+      } else if constexpr (MODE == M_JACOBI) {
+        v = a.x[gi]; ir = a.invR[gi];
+      } else if constexpr (MODE == M_CHEB) {
+        double z = a.r[gi] * a.invM[gi];
+        double dn = a.first ? z / a.theta : a.c1 * a.d[gi] + a.c2 * z;
+        v = a.x[gi] + dn;
+        ir = a.invR[gi];
+        if (idx == tid) own_d = dn;
+      }
+    }
+    vs[idx] = v;
+    irs[idx] = ir;
+  }
+  __syncthreads();
+
+  const double al = a.alpha;
+  const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
+  const double vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
+  double fc = 0.0;        // this node's own face terms (consistency + penalty)
+  double sigx_l = 0.0, sigx_r = 0.0, sigy_b = 0.0, sigy_t = 0.0;
+  double bval = 0.0;
+  if (valid) {
+    double dxi = 0.0, deta = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      dxi += B.D[i * NP + k] * vs[s * NP2 + k + NP * j];
+      deta += B.D[j * NP + k] * vs[s * NP2 + i + NP * k];
+    }
+    gx[s * NP2 + n] = al * B.w[i] * B.w[j] * g.Jy * g.invJx * ir * dxi;
+    gy[s * NP2 + n] = al * B.w[i] * B.w[j] * g.Jx * g.invJy * ir * deta;
+    if (i == 0) {            // x-face between left neighbour (L) and this element (R)
+      const double xL = vs[sL * NP2 + (NP - 1) + NP * j];
+      const double irL = irs[sL * NP2 + (NP - 1) + NP * j];
+      double dL = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) dL += B.D[(NP - 1) * NP + k] * vs[sL * NP2 + k + NP * j];
+      const double jump = xL - vown, wf = B.w[j] * g.Jy;
+      const double avg = g.invJx * (irL * dL + ir * dxi) / 2.0;
+      sigx_l = g.eta * fmax(irL, ir) / g.dx;
+      fc += al * avg * wf;
+      fc -= al * sigx_l * jump * wf;
+      sxl[s * NP + j] = -al * (ir / 2.0) * g.invJx * jump * wf;
+    }
+    if (i == NP - 1) {       // x-face between this element (L) and right neighbour (R)
+      const double xR = vs[sR * NP2 + NP * j];
+      const double irR = irs[sR * NP2 + NP * j];
+      double dR = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) dR += B.D[k] * vs[sR * NP2 + k + NP * j];
+      const double jump = vown - xR, wf = B.w[j] * g.Jy;
+      const double avg = g.invJx * (ir * dxi + irR * dR) / 2.0;
+      sigx_r = g.eta * fmax(ir, irR) / g.dx;
+      fc += -al * avg * wf;
+      fc += al * sigx_r * jump * wf;
+      sxr[s * NP + j] = -al * (ir / 2.0) * g.invJx * jump * wf;
+    }
+    if (j == 0) {            // y-face between bottom neighbour (L) and this element (R)
+      const double xL = vs[sB * NP2 + i + NP * (NP - 1)];
+      const double irL = irs[sB * NP2 + i + NP * (NP - 1)];
+      double dL = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) dL += B.D[(NP - 1) * NP + k] * vs[sB * NP2 + i + NP * k];
+      const double jump = xL - vown, wf = B.w[i] * g.Jx;
+      const double avg = g.invJy * (irL * dL + ir * deta) / 2.0;
+      sigy_b = g.eta * fmax(irL, ir) / g.dy;
+      fc += al * avg * wf;
+      fc -= al * sigy_b * jump * wf;
+      syb[s * NP + i] = -al * (ir / 2.0) * g.invJy * jump * wf;
+    }
+    if (j == NP - 1) {       // y-face between this element (L) and top neighbour (R)
+      const double xR = vs[sT * NP2 + i];
+      const double irR = irs[sT * NP2 + i];
+      double dR = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) dR += B.D[k] * vs[sT * NP2 + i + NP * k];
+      const double jump = vown - xR, wf = B.w[i] * g.Jx;
+      const double avg = g.invJy * (ir * deta + irR * dR) / 2.0;
+      sigy_t = g.eta * fmax(ir, irR) / g.dy;
+      fc += -al * avg * wf;
+      fc += al * sigy_t * jump * wf;
+      syt[s * NP + i] = -al * (ir / 2.0) * g.invJy * jump * wf;
+    }
+    if constexpr (INIT) {    // IGR source R = alpha[(div u)^2 + tr((Du)^2)]
+      double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        uxx += B.D[i * NP + k] * us[s * NP2 + k + NP * j];
+        uyx += B.D[i * NP + k] * ws_[s * NP2 + k + NP * j];
+        uxy += B.D[j * NP + k] * us[s * NP2 + i + NP * k];
+        uyy += B.D[j * NP + k] * ws_[s * NP2 + i + NP * k];
+      }
+      uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
+      const double dv = uxx + uyy;
+      const double R = al * (dv * dv + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
+      bval = B.w[i] * B.w[j] * g.Jx * g.Jy * R;
+    }
+  }
+  if (!valid) {
+    // keep LDS reads of never-written face factors deterministic
+    if (i == 0) sxl[s * NP + j] = 0.0;
+    if (i == NP - 1) sxr[s * NP + j] = 0.0;
+    if (j == 0) syb[s * NP + i] = 0.0;
+    if (j == NP - 1) syt[s * NP + i] = 0.0;
+  }
+  __syncthreads();
+
+  double y = 0.0;
+  if (valid) {
+    y = B.w[i] * B.w[j] * g.Jx * g.Jy * vown * ir;
+    double st = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      st += B.D[k * NP + i] * gx[s * NP2 + k + NP * j];
+      st += B.D[k * NP + j] * gy[s * NP2 + i + NP * k];
+    }
+    y += st;
+    y += sxl[s * NP + j] * B.D[i] + sxr[s * NP + j] * B.D[(NP - 1) * NP + i];
+    y += syb[s * NP + i] * B.D[j] + syt[s * NP + i] * B.D[(NP - 1) * NP + j];
+    y += fc;
+  }
+
+  if constexpr (MODE == M_APPLY) {
+    if (valid) a.y[gme] = y;
+  } else if constexpr (INIT) {
+    double red_v[1] = {0.0};
+    if (valid) {
+      // diag(A): mass, x/y stiffness self-coupling, face terms
+      double Md = B.w[i] * B.w[j] * g.Jx * g.Jy * ir;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const double D2 = B.D[k * NP + i] * B.D[k * NP + i];
+        Md += al * B.w[k] * B.w[j] * g.Jy * g.invJx * D2 * irs[s * NP2 + k + NP * j];
+      }
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const double D2 = B.D[k * NP + j] * B.D[k * NP + j];
+        Md += al * B.w[i] * B.w[k] * g.Jx * g.invJy * D2 * irs[s * NP2 + i + NP * k];
+      }
+      if (i == NP - 1) Md += (-al * g.invJx * B.D[(NP - 1) * NP + NP - 1] * ir + al * sigx_r) * (B.w[j] * g.Jy);
+      if (i == 0) Md += (al * g.invJx * B.D[0] * ir + al * sigx_l) * (B.w[j] * g.Jy);
+      if (j == NP - 1) Md += (-al * g.invJy * B.D[(NP - 1) * NP + NP - 1] * ir + al * sigy_t) * (B.w[i] * g.Jx);
+      if (j == 0) Md += (al * g.invJy * B.D[0] * ir + al * sigy_b) * (B.w[i] * g.Jx);
+      const double iM = 1.0 / Md;
+      a.invR_out[gme] = ir;
+      a.invM_out[gme] = iM;
+      if (!(ir > 0.0) || !isfinite(ir)) flag_domain(a.flag);
+      if constexpr (MODE == M_INIT_PCG) {
+        const double r0 = bval - y;
+        a.rout[gme] = r0;
+        red_v[0] = r0 * (r0 * iM);
+      } else if constexpr (MODE == M_INIT_CHEB) {
+        a.rout[gme] = bval - y;
+        a.b_out[gme] = bval;
+      } else {
+        a.b_out[gme] = bval;
+      }
+    }
+    if constexpr (MODE == M_INIT_PCG) {
+      double* sc = a.sc;
+      grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x,
+                  [sc](double* tot) { sc[0] = tot[0]; });
+    }
+  } else if constexpr (MODE == M_PCG_A) {
+    double red_v[1] = {0.0};
+    if (valid) {
+      a.pnew[gme] = vown;
+      a.y[gme] = y;
+      red_v[0] = vown * y;
+    }
+    double* sc = a.sc;
+    const int nit = a.n_iter, it = a.it;
+    grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x, [sc, nit, it](double* tot) {
+      sc[nit + 1 + it] = tot[0];
+      sc[2 * nit + 1 + it] = sc[it] / (tot[0] + 1e-30);
+    });
+  } else if constexpr (MODE == M_JACOBI) {
+    if (valid) a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
+  } else if constexpr (MODE == M_CHEB) {
+    if (valid) {
+      a.xout[gme] = vown;
+      a.dout[gme] = own_d;
+      a.rout[gme] = a.b[gme] - y;
+    }
+  }
+}
+
+// x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
+  constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
+  __shared__ double red[NT / WAVE + 1];
+  TileInfo<NP, TX, TY> t(a.g);
+  const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
+  const int lx = s % TX, ly = s / TX;
+  const bool valid = lx < t.nvx && ly < t.nvy;
+  double* sc = a.sc;
+  const int nit = a.n_iter, it = a.it;
+  double red_v[1] = {0.0};
+  if (valid) {
+    const size_t gi = t.gidx(s, n);
+    const double ak = sc[2 * nit + 1 + it];
+    a.xout[gi] = a.xout[gi] + ak * a.pold[gi];
+    const double rn = a.rout[gi] - ak * a.y[gi];
+    a.rout[gi] = rn;
+    red_v[0] = rn * (rn * a.invM[gi]);
+  }
+  grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x, [sc, nit, it](double* tot) {
+    sc[it + 1] = tot[0];
+    sc[3 * nit + 1 + it] = tot[0] / (sc[it] + 1e-30);
+  });
+}
+
+// ------------------------------------------------------------------------
+// Hyperbolic RHS + SSP-RK3 stage update.  Built without FMA contraction and
+// in the reference's evaluation order, so dq equals the CPU restatement
+// bit for bit.
+// ------------------------------------------------------------------------
+#pragma clang fp contract(off)
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp(HypArgs a) {
+  constexpr int NP2 = NP * NP, T = TX * TY, NH = 2 * (TX + TY), NT = T * NP2;
+  __shared__ double qs[5][T * NP2];
+  __shared__ double Fx[4][T * NP2], Fy[4][T * NP2];
+  __shared__ double wsx[T * NP2], wsy[T * NP2];
+  __shared__ double hq[5][NH * NP];          // halo face nodes
+  __shared__ double fxr[4][T * NP], fxl[4][T * NP], fyt[4][T * NP], fyb[4][T * NP];
+
+  const Geo2 g = a.g;
+  const BasisK& B = a.B;
+  const double gam = a.gamma;
+  TileInfo<NP, TX, TY> t(g);
+  const int tid = threadIdx.x;
+  const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
+  const int lx = s % TX, ly = s / TX;
+  const bool valid = lx < t.nvx && ly < t.nvy;
+  const size_t N = (size_t)g.Nex * g.Ney * NP2;
+  const size_t gme = valid ? t.gidx(s, n) : 0;
+
+  double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0};
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qo[v] = a.q[v * N + gme];
+    qo[4] = a.S[gme];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
+  // halo face nodes of the neighbour elements
+  for (int idx = tid; idx < NH * NP; idx += NT) {
+    const int h = idx / NP, k = idx % NP;
+    size_t gi;
+    if (t.halo_face_global(h, k, gi)) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) hq[v][idx] = a.q[v * N + gi];
+      hq[4][idx] = a.S[gi];
+    } else {
+      hq[0][idx] = 0.0; hq[1][idx] = 0.0; hq[2][idx] = 1.0; hq[3][idx] = 1.0; hq[4][idx] = 0.0;
+    }
+  }
+  double fxo[4], fyo[4];
+  flux_x2(gam, qo, fxo);
+  flux_y2(gam, qo, fyo);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
+  const bool face = (i == 0 || i == NP - 1 || j == 0 || j == NP - 1);
+  if (valid && face) {
+    const double c = sound2(gam, qo, a.flag);
+    wsx[tid] = fabs(qo[0] / qo[2]) + c;
+    wsy[tid] = fabs(qo[1] / qo[2]) + c;
+  }
+  __syncthreads();
+
+  // volume term: dq = -(1/Jx) Dx Fx - (1/Jy) Dy Fy, x then y per k (hyperbolic.jl:55-66)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * B.D[i * NP + k] * Fx[v][s * NP2 + k + NP * j];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * B.D[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
+    }
+    // face fluxes, one evaluation per face inside the tile
+    if (i == NP - 1) {
+      double qR[5], wR;
+      const int sR = t.nb_right(s);
+      if (sR < T) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qR[v] = qs[v][sR * NP2 + NP * j];
+        wR = wsx[sR * NP2 + NP * j];
+      } else {
+        const int h = sR - T;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qR[v] = hq[v][h * NP + j];
+        wR = fabs(qR[0] / qR[2]) + sound2(gam, qR, a.flag);
+      }
+      double fR[4], f[4];
+      flux_x2(gam, qR, fR);
+      llf_combine(fxo, fR, qo, qR, fmax_first(wsx[tid], wR), f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fxr[v][s * NP + j] = f[v];
+    }
+    if (i == 0 && lx == 0) {
+      double qL[5];
+      const int h = t.nb_left(s) - T;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) qL[v] = hq[v][h * NP + j];
+      const double wL = fabs(qL[0] / qL[2]) + sound2(gam, qL, a.flag);
+      double fL[4], f[4];
+      flux_x2(gam, qL, fL);
+      llf_combine(fL, fxo, qL, qo, fmax_first(wL, wsx[tid]), f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fxl[v][s * NP + j] = f[v];
+    }
+    if (j == NP - 1) {
+      double qR[5], wR;
+      const int sT = t.nb_top(s);
+      if (sT < T) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qR[v] = qs[v][sT * NP2 + i];
+        wR = wsy[sT * NP2 + i];
+      } else {
+        const int h = sT - T;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qR[v] = hq[v][h * NP + i];
+        wR = fabs(qR[1] / qR[2]) + sound2(gam, qR, a.flag);
+      }
+      double fR[4], f[4];
+      flux_y2(gam, qR, fR);
+      llf_combine(fyo, fR, qo, qR, fmax_first(wsy[tid], wR), f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fyt[v][s * NP + i] = f[v];
+    }
+    if (j == 0 && ly == 0) {
+      double qL[5];
+      const int h = t.nb_bottom(s) - T;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) qL[v] = hq[v][h * NP + i];
+      const double wL = fabs(qL[1] / qL[2]) + sound2(gam, qL, a.flag);
+      double fL[4], f[4];
+      flux_y2(gam, qL, fL);
+      llf_combine(fL, fyo, qL, qo, fmax_first(wL, wsy[tid]), f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fyb[v][s * NP + i] = f[v];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  // face corrections: x faces, then y faces (hyperbolic.jl:75-116)
+  if (i == NP - 1) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * (fxr[v][s * NP + j] - fxo[v]) / B.w[NP - 1];
+  }
+  if (i == 0) {
+    const int sL = t.nb_left(s);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double f = (sL < T) ? fxr[v][sL * NP + j] : fxl[v][s * NP + j];
+      acc[v] += g.invJx * (f - fxo[v]) / B.w[0];
+    }
+  }
+  if (j == NP - 1) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * (fyt[v][s * NP + i] - fyo[v]) / B.w[NP - 1];
+  }
+  if (j == 0) {
+    const int sB = t.nb_bottom(s);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double f = (sB < T) ? fyt[v][sB * NP + i] : fyb[v][s * NP + i];
+      acc[v] += g.invJy * (f - fyo[v]) / B.w[0];
+    }
+  }
+  // SSP-RK3 stage combination (forward.jl:172-187)
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const size_t gi = v * N + gme;
+    double o;
+    if (a.mode == 0) o = acc[v];
+    else if (a.mode == 1) o = qo[v] + a.cdt * acc[v];
+    else if (a.mode == 2) o = 3.0 / 4.0 * a.qn[gi] + 1.0 / 4.0 * qo[v] + a.cdt * acc[v];
+    else o = 1.0 / 3.0 * a.qn[gi] + 2.0 / 3.0 * qo[v] + a.cdt * acc[v];
+    a.out[gi] = o;
+  }
+}
+#pragma clang fp contract(fast)
+
+// Plain fill (used instead of hipMemsetAsync so the step can be captured in
+// a graph).
+__global__ void k_fill(double* x, size_t n, double v) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = v;
+}
+__global__ void k_copy(double* dst, const double* src, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
+// ------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------
+template <int NP, int TX, int TY>
+struct Launch2 {
+  static dim3 grid(const Geo2& g) {
+    int ntx = (g.Nex + TX - 1) / TX, nty = (g.Ney + TY - 1) / TY;
+    return dim3(ntx * nty);
+  }
+  static dim3 block() { return dim3(TX * TY * NP * NP); }
+  static void sip(int mode, const SipArgs& a, hipStream_t st) {
+    dim3 G = grid(a.g), Bk = block();
+    switch (mode) {
+      case M_APPLY: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_APPLY>), G, Bk, 0, st, a); break;
+      case M_INIT_PCG: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_PCG>), G, Bk, 0, st, a); break;
+      case M_INIT_CHEB: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_CHEB>), G, Bk, 0, st, a); break;
+      case M_INIT_JAC: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_JAC>), G, Bk, 0, st, a); break;
+      case M_PCG_A: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_PCG_A>), G, Bk, 0, st, a); break;
+      case M_JACOBI: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_JACOBI>), G, Bk, 0, st, a); break;
+      case M_CHEB: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_CHEB>), G, Bk, 0, st, a); break;
+      case M_PCG_B: hipLaunchKernelGGL((k2_pcg_b<NP, TX, TY>), G, Bk, 0, st, a); break;
+    }
+  }
+  static void hyp(const HypArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k2_hyp<NP, TX, TY>), grid(a.g), block(), 0, st, a);
+  }
+};
+
+void launch2_sip(const Ctx& c, int mode, const SipArgs& a) {
+  switch (c.np) {
+    case 2: Launch2<2, 8, 8>::sip(mode, a, c.stream); break;
+    case 3: Launch2<3, 4, 4>::sip(mode, a, c.stream); break;
+    case 4:
+      if (c.tile_x == 8 && c.tile_y == 4) Launch2<4, 8, 4>::sip(mode, a, c.stream);
+      else if (c.tile_x == 2 && c.tile_y == 2) Launch2<4, 2, 2>::sip(mode, a, c.stream);
+      else Launch2<4, 4, 4>::sip(mode, a, c.stream);
+      break;
+    case 5: Launch2<5, 2, 4>::sip(mode, a, c.stream); break;
+  }
+}
+void launch2_hyp(const Ctx& c, const HypArgs& a) {
+  switch (c.np) {
+    case 2: Launch2<2, 8, 8>::hyp(a, c.stream); break;
+    case 3: Launch2<3, 4, 4>::hyp(a, c.stream); break;
+    case 4:
+      if (c.tile_x == 8 && c.tile_y == 4) Launch2<4, 8, 4>::hyp(a, c.stream);
+      else if (c.tile_x == 2 && c.tile_y == 2) Launch2<4, 2, 2>::hyp(a, c.stream);
+      else Launch2<4, 4, 4>::hyp(a, c.stream);
+      break;
+    case 5: Launch2<5, 2, 4>::hyp(a, c.stream); break;
+  }
+}
+int grid2_blocks(const Ctx& c) {
+  int tx = 4, ty = 4;
+  if (c.np == 2) { tx = 8; ty = 8; }
+  else if (c.np == 5) { tx = 2; ty = 4; }
+  else if (c.np == 4) { tx = c.tile_x; ty = c.tile_y; }
+  return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
+}
+void launch_fill(const Ctx& c, double* x, size_t n, double v) {
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, x, n, v);
+}
+void launch_copy(const Ctx& c, double* dst, const double* src, size_t n) {
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, dst, src, n);
+}
+
+}  // namespace igr

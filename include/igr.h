@@ -1,0 +1,110 @@
+/*
+ * igr.h — C ABI of the MI355X (gfx950) IGR-regularized Euler library.
+ *
+ * Replaces the hot path of the reference Julia packages
+ * f-t-s/IGR_sensitivities (IGR_adjoints_1d/, IGR_adjoints_2d/): the nodal
+ * Gauss–Lobatto DG step with LLF faces, the matrix-free SIP solve for the
+ * entropic pressure Sigma (fixed-iteration PCG / damped Jacobi / Chebyshev),
+ * SSP-RK3, the forward tangent (sensitivity) equations and the discrete
+ * adjoint that the reference obtains from Enzyme.
+ *
+ * Conventions
+ *  - All arrays are host pointers to contiguous FP64 data in the reference's
+ *    column-major layout: 2D node (i,j,ex,ey) at i + np*(j + np*(ex+Nex*ey)),
+ *    1D node (i,e) at i + np*e (np = p+1).  A "state" stores its conserved
+ *    variables back to back: 2D (mx, my, rho, E), 1D (mu, rho, E), each of
+ *    N = nodes entries (IGR_adjoints_2d/euler.jl:5-9).
+ *  - Every function returns an IGR_* status.  Julia's ErrorException for an
+ *    unknown solver/objective symbol maps to IGR_EINVAL; Julia's DomainError
+ *    from sqrt of a negative pressure maps to IGR_EDOMAIN.
+ *  - The library owns device memory inside the context; state lives in HBM
+ *    between calls.  One HIP stream per context; a context is not
+ *    thread-safe (the reference is serial).
+ */
+#ifndef IGR_H
+#define IGR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum igr_status {
+  IGR_OK = 0,
+  IGR_EINVAL = 1,   /* bad argument, unknown solver or objective code */
+  IGR_EDOMAIN = 2,  /* negative pressure under sqrt / non-finite state */
+  IGR_EHIP = 3,     /* HIP runtime failure */
+  IGR_ENOMEM = 4    /* device allocation failed */
+};
+
+/* Replaces the `solver::Symbol` argument (:pcg, :jacobi, :chebyshev) of
+ * solve_elliptic! (IGR_adjoints_2d/elliptic.jl:413-427). */
+enum igr_solver { IGR_SOLVER_PCG = 0, IGR_SOLVER_JACOBI = 1, IGR_SOLVER_CHEBYSHEV = 2 };
+
+typedef struct igr_ctx igr_ctx;
+
+/* Mesh + basis set-up: DGBasis(p) (IGR_adjoints_2d/basis.jl:56-68) and
+ * CartesianMesh2D(nex, ney, lx, ly) (IGR_adjoints_2d/mesh.jl:108-125) for
+ * dim = 2, or PeriodicMesh1D(nex, lx) (IGR_adjoints_1d/mesh.jl:21-33) for
+ * dim = 1 (ney, ly ignored).  1 <= p <= 4. */
+int igr_create(igr_ctx** ctx, int dim, int p, int nex, int ney, double lx, double ly);
+int igr_destroy(igr_ctx* ctx);
+const char* igr_last_error(const igr_ctx* ctx);
+/* Number of nodes per field (N) and conserved variables (nv). */
+int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
+
+/* Options: IGR_OPT_GRAPHS (1 = capture each SSP-RK3 step in a hipGraph and
+ * replay it, 0 = launch kernels directly), IGR_OPT_TILE (2D elements per
+ * workgroup tile, encoded tile_x*100 + tile_y; p = 3 supports 404, 804, 202). */
+enum igr_option { IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1 };
+int igr_set_option(igr_ctx* ctx, int option, int value);
+
+/* Upload / download the state q (nv*N) and Sigma (N; may be NULL). */
+int igr_set_state(igr_ctx* ctx, const double* q, const double* sigma);
+int igr_get_state(igr_ctx* ctx, double* q, double* sigma);
+
+/* compute_hyperbolic_rhs! on host arrays (IGR_adjoints_2d/hyperbolic.jl:27;
+ * IGR_adjoints_1d/hyperbolic.jl:23). */
+int igr_hyperbolic_rhs(igr_ctx* ctx, double gamma, const double* q, const double* sigma, double* dq);
+/* apply_sip!(y, x, rho, basis, mesh, alpha) (IGR_adjoints_2d/elliptic.jl:30;
+ * IGR_adjoints_1d/elliptic.jl:32). */
+int igr_apply_sip(igr_ctx* ctx, double alpha, const double* x, const double* rho, double* y);
+/* compute_igr_rhs! (IGR_adjoints_2d/elliptic.jl:159) on a state q. */
+int igr_igr_rhs(igr_ctx* ctx, double alpha, const double* q, double* b);
+/* compute_sip_diagonal (IGR_adjoints_2d/elliptic.jl:205). */
+int igr_sip_diagonal(igr_ctx* ctx, double alpha, const double* rho, double* diag);
+
+/* solve_elliptic!(Sigma, q..., alpha, n_iter, solver) on the context state:
+ * Sigma (context) is the warm start and is overwritten
+ * (IGR_adjoints_2d/elliptic.jl:413-427; IGR_adjoints_1d/elliptic.jl:451). */
+int igr_solve_elliptic(igr_ctx* ctx, double alpha, int n_iter, int solver);
+/* PCG scalars of the last PCG solve: a_k and beta_k, k < n_iter. */
+int igr_solver_trace(igr_ctx* ctx, int n_iter, double* a, double* beta);
+
+/* compute_igr_euler_rhs! on the context state (IGR_adjoints_2d/forward.jl:141):
+ * Sigma solve when alpha > 0 (else Sigma = 0), then the hyperbolic RHS,
+ * written to dq (host, nv*N; may be NULL). */
+int igr_euler_rhs(igr_ctx* ctx, double gamma, double alpha, int n_iter, int solver, double* dq);
+
+/* run_forward!(q..., Sigma, dt, T, basis, mesh, gamma, alpha, n_iter, solver)
+ * (IGR_adjoints_2d/forward.jl:160-194; IGR_adjoints_1d/forward.jl:50-87):
+ * Sigma <- 0, n_steps = ceil(T/dt), dt = T/n_steps, SSP-RK3, final Sigma
+ * solve when alpha > 0.  Advances the context state in place. */
+int igr_run_forward(igr_ctx* ctx, double dt, double t_end, double gamma, double alpha, int n_iter,
+                    int solver, int* n_steps_out);
+/* The fixed-step loop inlined in the Enzyme objectives
+ * (IGR_adjoints_2d/enzyme_adjoint.jl:50-70): Sigma <- 0, n_steps steps of
+ * size dt, no final solve. */
+int igr_run_steps(igr_ctx* ctx, int n_steps, double dt, double gamma, double alpha, int n_iter,
+                  int solver);
+
+/* Mean device time (ms) per launch of one hot kernel on the current state,
+ * measured with HIP events around `reps` back-to-back launches on the
+ * context stream.  kernel: 0 = fused SIP-apply + PCG direction update,
+ * 1 = PCG x/r update, 2 = hyperbolic RHS + RK3 stage update. */
+int igr_time_kernel(igr_ctx* ctx, int kernel, double gamma, double alpha, int reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IGR_H */

@@ -1,0 +1,1203 @@
+// CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// A plain-loop C++ restatement of the reference Julia packages
+// IGR_adjoints_1d/ and IGR_adjoints_2d/ (f-t-s/IGR_sensitivities).  Every
+// function cites the reference file:line it follows and keeps the
+// reference's loop order and left-to-right expression order, built with
+// -ffp-contract=off so no FMA contraction changes the rounding.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+// load this code; the product path (igr_sensitivities_amd/) never does.
+//
+// Everything is templated on the scalar type T.  T = double reproduces the
+// reference map; T = Dual (value + one tangent) gives the exact discrete
+// Jacobian-vector product of the same map, which is what pins hand-written
+// reverse (adjoint) kernels through the dot-product identity
+// <v, J u> = <J^T v, u>.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace orc {
+
+// ---------------------------------------------------------------- Dual ----
+struct Dual {
+  double v, d;
+  Dual() : v(0.0), d(0.0) {}
+  Dual(double a) : v(a), d(0.0) {}
+  Dual(double a, double b) : v(a), d(b) {}
+};
+inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) {
+  double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+inline Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
+inline Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+inline Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
+inline Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
+inline Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+inline Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
+inline Dual operator/(double a, Dual b) {
+  double q = a / b.v;
+  return Dual(q, -q * b.d / b.v);
+}
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
+inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, double b) { return a.v > b; }
+
+inline double val(double x) { return x; }
+inline double val(const Dual& x) { return x.v; }
+inline double tan_(double) { return 0.0; }
+inline double tan_(const Dual& x) { return x.d; }
+
+// Domain flag: Julia's sqrt throws DomainError on a negative argument
+// (IGR_adjoints_2d/euler.jl:70, IGR_adjoints_1d/euler.jl:44).  The oracle
+// records it and carries on with NaN; the C wrapper reports it.
+extern thread_local int g_domain_error;
+
+inline double sqrt_(double x) {
+  if (x < 0.0) g_domain_error = 1;
+  return std::sqrt(x);
+}
+inline Dual sqrt_(Dual x) {
+  if (x.v < 0.0) g_domain_error = 1;
+  double s = std::sqrt(x.v);
+  return Dual(s, x.d / (2.0 * s));
+}
+// abs: derivative sign follows signbit, as ForwardDiff's abs(::Dual).
+inline double abs_(double x) { return std::fabs(x); }
+inline Dual abs_(Dual x) { return std::signbit(x.v) ? -x : x; }
+// max: Julia's max(x, y) returns x on ties; derivative follows the value.
+inline double max_(double a, double b) { return (b > a) ? b : a; }
+inline Dual max_(Dual a, Dual b) { return (b.v > a.v) ? b : a; }
+inline double exp_(double x) { return std::exp(x); }
+inline Dual exp_(Dual x) { double e = std::exp(x.v); return Dual(e, e * x.d); }
+
+// --------------------------------------------------------------- Basis ----
+// DGBasis: IGR_adjoints_1d/basis.jl:7-68 (2D copy identical).
+constexpr int MAXNP = 8;
+struct Basis {
+  int p = 0, np = 1;
+  double xi[MAXNP] = {0}, w[MAXNP] = {0};
+  double D[MAXNP * MAXNP] = {0};   // D[i*np + k] = D_{ik}
+  double Dik(int i, int k) const { return D[i * np + k]; }
+};
+
+// Legendre-Gauss-Lobatto nodes and weights by Newton iteration on
+// (1-x^2) P_p'(x) (the reference calls FastGaussQuadrature.gausslobatto,
+// basis.jl:63; the values are pinned by the closed form at p=3 in tests).
+inline void lgl_nodes(int p, double* xi, double* w) {
+  int n = p + 1;
+  if (p == 0) { xi[0] = 0.0; w[0] = 2.0; return; }
+  for (int i = 0; i < n; ++i) {
+    double x = -std::cos(M_PI * i / p);
+    for (int it = 0; it < 100; ++it) {
+      double P0 = 1.0, P1 = x;
+      for (int k = 2; k <= p; ++k) {
+        double P2 = ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k;
+        P0 = P1; P1 = P2;
+      }
+      // P1 = P_p, P0 = P_{p-1}
+      double dx = (x * P1 - P0) / (n * P1);
+      x -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    xi[i] = x;
+  }
+  xi[0] = -1.0; xi[n - 1] = 1.0;
+  for (int i = 0; i < n / 2; ++i) {       // exact antisymmetry
+    double a = 0.5 * (xi[n - 1 - i] - xi[i]);
+    xi[i] = -a; xi[n - 1 - i] = a;
+  }
+  if (n % 2 == 1) xi[n / 2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double x = xi[i], P0 = 1.0, P1 = x;
+    for (int k = 2; k <= p; ++k) {
+      double P2 = ((2.0 * k - 1.0) * x * P1 - (k - 1.0) * P0) / k;
+      P0 = P1; P1 = P2;
+    }
+    w[i] = 2.0 / (p * (p + 1.0) * P1 * P1);
+  }
+}
+
+inline Basis make_basis(int p) {
+  Basis B; B.p = p; B.np = p + 1;
+  int n = B.np;
+  lgl_nodes(p, B.xi, B.w);
+  if (p == 0) { B.D[0] = 0.0; return B; }
+  // barycentric_weights, basis.jl:19-30
+  double lam[MAXNP];
+  for (int j = 0; j < n; ++j) {
+    lam[j] = 1.0;
+    for (int i = 0; i < n; ++i) { if (i == j) continue; lam[j] *= (B.xi[j] - B.xi[i]); }
+    lam[j] = 1.0 / lam[j];
+  }
+  // differentiation_matrix, basis.jl:38-50 (diagonal = -sum of off-diagonal row)
+  for (int q = 0; q < n * n; ++q) B.D[q] = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      if (i == j) continue;
+      B.D[i * n + j] = (lam[j] / lam[i]) / (B.xi[i] - B.xi[j]);
+      B.D[i * n + i] -= B.D[i * n + j];
+    }
+  return B;
+}
+
+// ---------------------------------------------------------------- Mesh ----
+// PeriodicMesh1D: IGR_adjoints_1d/mesh.jl:21-33.
+struct Mesh1D {
+  int Ne; double L, dx, J;
+  Mesh1D(int ne, double l) : Ne(ne), L(l), dx(l / ne), J((l / ne) / 2.0) {}
+  double x(const Basis& B, int i, int e) const { return e * dx + J * (1.0 + B.xi[i]); }
+};
+// CartesianMesh2D: IGR_adjoints_2d/mesh.jl:108-125.
+struct Mesh2D {
+  int Nex, Ney; double Lx, Ly, dx, dy, Jx, Jy;
+  Mesh2D(int nx, int ny, double lx, double ly)
+      : Nex(nx), Ney(ny), Lx(lx), Ly(ly), dx(lx / nx), dy(ly / ny),
+        Jx((lx / nx) / 2.0), Jy((ly / ny) / 2.0) {}
+  double x(const Basis& B, int i, int ex) const { return ex * dx + Jx * (1.0 + B.xi[i]); }
+  double y(const Basis& B, int j, int ey) const { return ey * dy + Jy * (1.0 + B.xi[j]); }
+};
+
+enum Solver { PCG = 0, JACOBI = 1, CHEBYSHEV = 2 };
+
+template <class T>
+inline T dot_(const std::vector<T>& a, const std::vector<T>& b) {
+  T s = 0.0;
+  for (size_t n = 0; n < a.size(); ++n) s += a[n] * b[n];
+  return s;
+}
+
+// =====================================================================
+//                              2D
+// =====================================================================
+struct Grid2 {
+  const Basis& B; const Mesh2D& M; int np, N;
+  Grid2(const Basis& b, const Mesh2D& m) : B(b), M(m), np(b.np), N(b.np * b.np * m.Nex * m.Ney) {}
+  size_t at(int i, int j, int ex, int ey) const {
+    return (size_t)i + np * ((size_t)j + np * ((size_t)ex + (size_t)M.Nex * ey));
+  }
+};
+
+// euler.jl:145-147
+template <class T> inline T pressure2(T g, T mx, T my, T r, T E) {
+  return (g - 1.0) * (E - (mx * mx + my * my) / (2.0 * r));
+}
+// euler.jl:156-167
+template <class T> inline void flux_x2(T g, T mx, T my, T r, T E, T S, T f[4]) {
+  T P = pressure2(g, mx, my, r, E);
+  T pb = P + S;
+  T ux = mx / r;
+  f[0] = mx * ux + pb; f[1] = my * ux; f[2] = mx; f[3] = (E + pb) * ux;
+}
+// euler.jl:175-186
+template <class T> inline void flux_y2(T g, T mx, T my, T r, T E, T S, T f[4]) {
+  T P = pressure2(g, mx, my, r, E);
+  T pb = P + S;
+  T uy = my / r;
+  f[0] = mx * uy; f[1] = my * uy + pb; f[2] = my; f[3] = (E + pb) * uy;
+}
+// euler.jl:193-214
+template <class T> inline T sound2(T g, T mx, T my, T r, T E) {
+  T P = pressure2(g, mx, my, r, E);
+  return sqrt_(g * P / r);
+}
+template <class T> inline T wave_x2(T g, T mx, T my, T r, T E) { return abs_(mx / r) + sound2(g, mx, my, r, E); }
+template <class T> inline T wave_y2(T g, T mx, T my, T r, T E) { return abs_(my / r) + sound2(g, mx, my, r, E); }
+// euler.jl:222-258, dir 0 = x, 1 = y
+template <class T>
+inline void llf2(int dir, T g, const T qL[5], const T qR[5], T f[4]) {
+  T lam = dir == 0 ? max_(wave_x2(g, qL[0], qL[1], qL[2], qL[3]), wave_x2(g, qR[0], qR[1], qR[2], qR[3]))
+                   : max_(wave_y2(g, qL[0], qL[1], qL[2], qL[3]), wave_y2(g, qR[0], qR[1], qR[2], qR[3]));
+  T fL[4], fR[4];
+  if (dir == 0) { flux_x2(g, qL[0], qL[1], qL[2], qL[3], qL[4], fL); flux_x2(g, qR[0], qR[1], qR[2], qR[3], qR[4], fR); }
+  else          { flux_y2(g, qL[0], qL[1], qL[2], qL[3], qL[4], fL); flux_y2(g, qR[0], qR[1], qR[2], qR[3], qR[4], fR); }
+  for (int v = 0; v < 4; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
+}
+
+// compute_hyperbolic_rhs!, IGR_adjoints_2d/hyperbolic.jl:27-119.
+// q: 4 fields (mx,my,rho,E) of N each, contiguous; S: N; dq: 4N.
+template <class T>
+void hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, T* dq) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
+  std::vector<T> Fx(4 * (size_t)N), Fy(4 * (size_t)N);
+  for (int n = 0; n < N; ++n) {   // :39-48
+    T f[4];
+    flux_x2(g, q[n], q[N + n], q[2 * N + n], q[3 * N + n], S[n], f);
+    for (int v = 0; v < 4; ++v) Fx[(size_t)v * N + n] = f[v];
+    flux_y2(g, q[n], q[N + n], q[2 * N + n], q[3 * N + n], S[n], f);
+    for (int v = 0; v < 4; ++v) Fy[(size_t)v * N + n] = f[v];
+  }
+  for (int ey = 0; ey < M.Ney; ++ey)     // :51-72
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          T a[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int k = 0; k < np; ++k) {
+            for (int v = 0; v < 4; ++v) a[v] -= invJx * B.Dik(i, k) * Fx[(size_t)v * N + G.at(k, j, ex, ey)];
+            for (int v = 0; v < 4; ++v) a[v] -= invJy * B.Dik(j, k) * Fy[(size_t)v * N + G.at(i, k, ex, ey)];
+          }
+          for (int v = 0; v < 4; ++v) dq[(size_t)v * N + G.at(i, j, ex, ey)] = a[v];
+        }
+  for (int ey = 0; ey < M.Ney; ++ey)     // :75-94
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int exR = ex == M.Nex - 1 ? 0 : ex + 1;
+      for (int j = 0; j < np; ++j) {
+        size_t L = G.at(np - 1, j, ex, ey), R = G.at(0, j, exR, ey);
+        T qL[5] = {q[L], q[N + L], q[2 * N + L], q[3 * N + L], S[L]};
+        T qR[5] = {q[R], q[N + R], q[2 * N + R], q[3 * N + R], S[R]};
+        T f[4]; llf2(0, g, qL, qR, f);
+        for (int v = 0; v < 4; ++v) dq[(size_t)v * N + L] -= invJx * (f[v] - Fx[(size_t)v * N + L]) / B.w[np - 1];
+        for (int v = 0; v < 4; ++v) dq[(size_t)v * N + R] += invJx * (f[v] - Fx[(size_t)v * N + R]) / B.w[0];
+      }
+    }
+  for (int ey = 0; ey < M.Ney; ++ey)     // :97-116
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
+      for (int i = 0; i < np; ++i) {
+        size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
+        T qL[5] = {q[L], q[N + L], q[2 * N + L], q[3 * N + L], S[L]};
+        T qR[5] = {q[R], q[N + R], q[2 * N + R], q[3 * N + R], S[R]};
+        T f[4]; llf2(1, g, qL, qR, f);
+        for (int v = 0; v < 4; ++v) dq[(size_t)v * N + L] -= invJy * (f[v] - Fy[(size_t)v * N + L]) / B.w[np - 1];
+        for (int v = 0; v < 4; ++v) dq[(size_t)v * N + R] += invJy * (f[v] - Fy[(size_t)v * N + R]) / B.w[0];
+      }
+    }
+}
+
+// apply_sip!, IGR_adjoints_2d/elliptic.jl:30-145 (scatter form, reference order).
+template <class T>
+void apply_sip2(const Grid2& G, T alpha, const T* x, const T* rho, T* y) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  const double Jx = M.Jx, Jy = M.Jy, invJx = 1.0 / Jx, invJy = 1.0 / Jy;
+  const double eta = (double)((B.p + 1) * (B.p + 1));
+  const double* w = B.w;
+  for (int n = 0; n < N; ++n) y[n] = 0.0;
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          size_t n = G.at(i, j, ex, ey);
+          y[n] += w[i] * w[j] * Jx * Jy * x[n] / rho[n];
+          T dxi = 0.0, deta = 0.0;
+          for (int k = 0; k < np; ++k) {
+            dxi += B.Dik(i, k) * x[G.at(k, j, ex, ey)];
+            deta += B.Dik(j, k) * x[G.at(i, k, ex, ey)];
+          }
+          T invr = 1.0 / rho[n];
+          T cx = alpha * w[i] * w[j] * Jy * invJx * invr * dxi;
+          T cy = alpha * w[i] * w[j] * Jx * invJy * invr * deta;
+          for (int k = 0; k < np; ++k) {
+            y[G.at(k, j, ex, ey)] += cx * B.Dik(i, k);
+            y[G.at(i, k, ex, ey)] += cy * B.Dik(j, k);
+          }
+        }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int exR = ex == M.Nex - 1 ? 0 : ex + 1;
+      for (int j = 0; j < np; ++j) {
+        double wf = w[j] * Jy;
+        size_t L = G.at(np - 1, j, ex, ey), R = G.at(0, j, exR, ey);
+        T SL = x[L], SR = x[R], rL = rho[L], rR = rho[R];
+        T jump = SL - SR;
+        T dL = 0.0, dR = 0.0;
+        for (int k = 0; k < np; ++k) {
+          dL += B.Dik(np - 1, k) * x[G.at(k, j, ex, ey)];
+          dR += B.Dik(0, k) * x[G.at(k, j, exR, ey)];
+        }
+        T avg = invJx * ((1.0 / rL) * dL + (1.0 / rR) * dR) / 2.0;
+        T sig = eta * max_(1.0 / rL, 1.0 / rR) / M.dx;
+        y[L] += -alpha * avg * wf;
+        y[R] += alpha * avg * wf;
+        for (int k = 0; k < np; ++k) {
+          y[G.at(k, j, ex, ey)] += -alpha * (1.0 / (2.0 * rL)) * invJx * B.Dik(np - 1, k) * jump * wf;
+          y[G.at(k, j, exR, ey)] += -alpha * (1.0 / (2.0 * rR)) * invJx * B.Dik(0, k) * jump * wf;
+        }
+        y[L] += alpha * sig * jump * wf;
+        y[R] -= alpha * sig * jump * wf;
+      }
+    }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
+      for (int i = 0; i < np; ++i) {
+        double wf = w[i] * Jx;
+        size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
+        T SL = x[L], SR = x[R], rL = rho[L], rR = rho[R];
+        T jump = SL - SR;
+        T dL = 0.0, dR = 0.0;
+        for (int k = 0; k < np; ++k) {
+          dL += B.Dik(np - 1, k) * x[G.at(i, k, ex, ey)];
+          dR += B.Dik(0, k) * x[G.at(i, k, ex, eyR)];
+        }
+        T avg = invJy * ((1.0 / rL) * dL + (1.0 / rR) * dR) / 2.0;
+        T sig = eta * max_(1.0 / rL, 1.0 / rR) / M.dy;
+        y[L] += -alpha * avg * wf;
+        y[R] += alpha * avg * wf;
+        for (int k = 0; k < np; ++k) {
+          y[G.at(i, k, ex, ey)] += -alpha * (1.0 / (2.0 * rL)) * invJy * B.Dik(np - 1, k) * jump * wf;
+          y[G.at(i, k, ex, eyR)] += -alpha * (1.0 / (2.0 * rR)) * invJy * B.Dik(0, k) * jump * wf;
+        }
+        y[L] += alpha * sig * jump * wf;
+        y[R] -= alpha * sig * jump * wf;
+      }
+    }
+}
+
+// compute_igr_rhs!, IGR_adjoints_2d/elliptic.jl:159-197.
+template <class T>
+void igr_rhs2(const Grid2& G, T alpha, const T* q, T* b) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  const double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
+  const T* mx = q; const T* my = q + N; const T* r = q + 2 * (size_t)N;
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          T uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
+          for (int k = 0; k < np; ++k) {
+            size_t kj = G.at(k, j, ex, ey), ik = G.at(i, k, ex, ey);
+            T ux_kj = mx[kj] / r[kj], uy_kj = my[kj] / r[kj];
+            T ux_ik = mx[ik] / r[ik], uy_ik = my[ik] / r[ik];
+            uxx += B.Dik(i, k) * ux_kj;
+            uyx += B.Dik(i, k) * uy_kj;
+            uxy += B.Dik(j, k) * ux_ik;
+            uyy += B.Dik(j, k) * uy_ik;
+          }
+          uxx *= invJx; uyx *= invJx; uxy *= invJy; uyy *= invJy;
+          T s = uxx + uyy;
+          T R = alpha * (s * s + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
+          b[G.at(i, j, ex, ey)] = B.w[i] * B.w[j] * M.Jx * M.Jy * R;
+        }
+}
+
+// compute_sip_diagonal, IGR_adjoints_2d/elliptic.jl:205-264.
+template <class T>
+void sip_diag2(const Grid2& G, T alpha, const T* rho, T* dg) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  const double Jx = M.Jx, Jy = M.Jy, invJx = 1.0 / Jx, invJy = 1.0 / Jy;
+  const double eta = (double)(np * np);
+  const double* w = B.w;
+  for (int n = 0; n < N; ++n) dg[n] = 0.0;
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          size_t n = G.at(i, j, ex, ey);
+          dg[n] += w[i] * w[j] * Jx * Jy / rho[n];
+        }
+      for (int l = 0; l < np; ++l)
+        for (int k = 0; k < np; ++k) {
+          size_t n = G.at(k, l, ex, ey);
+          for (int i = 0; i < np; ++i) {
+            double D2 = B.Dik(i, k) * B.Dik(i, k);
+            dg[n] += alpha * w[i] * w[l] * Jy * invJx * D2 / rho[G.at(i, l, ex, ey)];
+          }
+          for (int j = 0; j < np; ++j) {
+            double D2 = B.Dik(j, l) * B.Dik(j, l);
+            dg[n] += alpha * w[k] * w[j] * Jx * invJy * D2 / rho[G.at(k, j, ex, ey)];
+          }
+        }
+    }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int exR = ex == M.Nex - 1 ? 0 : ex + 1;
+      for (int j = 0; j < np; ++j) {
+        double wf = w[j] * Jy;
+        size_t L = G.at(np - 1, j, ex, ey), R = G.at(0, j, exR, ey);
+        T rL = rho[L], rR = rho[R];
+        T sig = eta * max_(1.0 / rL, 1.0 / rR) / M.dx;
+        dg[L] += (-alpha * invJx * B.Dik(np - 1, np - 1) / rL + alpha * sig) * wf;
+        dg[R] += (alpha * invJx * B.Dik(0, 0) / rR + alpha * sig) * wf;
+      }
+    }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
+      for (int i = 0; i < np; ++i) {
+        double wf = w[i] * Jx;
+        size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
+        T rL = rho[L], rR = rho[R];
+        T sig = eta * max_(1.0 / rL, 1.0 / rR) / M.dy;
+        dg[L] += (-alpha * invJy * B.Dik(np - 1, np - 1) / rL + alpha * sig) * wf;
+        dg[R] += (alpha * invJy * B.Dik(0, 0) / rR + alpha * sig) * wf;
+      }
+    }
+}
+
+// Per-iteration record of the solver scalars (for parity with the GPU).
+struct SolverTrace { std::vector<double> a; std::vector<double> b; };
+
+// Fixed-iteration solvers, dimension-agnostic.  `Apply` computes y = A x.
+// pcg_fixed!: IGR_adjoints_2d/elliptic.jl:329-347 (1D: elliptic.jl:307-339).
+template <class T, class Apply>
+void pcg_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const std::vector<T>& Md,
+               int n_iter, SolverTrace* tr) {
+  size_t N = x.size();
+  std::vector<T> r(N), z(N), p(N), Ap(N);
+  A(x.data(), Ap.data());
+  for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ap[n];
+  for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
+  p = z;
+  T rz_old = dot_(r, z);
+  for (int it = 0; it < n_iter; ++it) {
+    A(p.data(), Ap.data());
+    T a = rz_old / (dot_(p, Ap) + 1e-30);
+    for (size_t n = 0; n < N; ++n) x[n] = x[n] + a * p[n];
+    for (size_t n = 0; n < N; ++n) r[n] = r[n] - a * Ap[n];
+    for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
+    T rz_new = dot_(r, z);
+    T beta = rz_new / (rz_old + 1e-30);
+    for (size_t n = 0; n < N; ++n) p[n] = z[n] + beta * p[n];
+    rz_old = rz_new;
+    if (tr) { tr->a.push_back(val(a)); tr->b.push_back(val(beta)); }
+  }
+}
+// jacobi_fixed!: IGR_adjoints_2d/elliptic.jl:354-361 (1D: :359-368).
+template <class T, class Apply>
+void jacobi_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const std::vector<T>& Md,
+                  int n_iter, double omega) {
+  size_t N = x.size();
+  std::vector<T> Ax(N);
+  for (int it = 0; it < n_iter; ++it) {
+    A(x.data(), Ax.data());
+    for (size_t n = 0; n < N; ++n) x[n] = x[n] + omega * (b[n] - Ax[n]) / Md[n];
+  }
+}
+// chebyshev_fixed!: IGR_adjoints_2d/elliptic.jl:370-397 (1D: :389-430).
+template <class T, class Apply>
+void chebyshev_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const std::vector<T>& Md,
+                     int n_iter, int p) {
+  size_t N = x.size();
+  std::vector<T> Ax(N), r(N), z(N), d(N);
+  static const double lmax_tab[4] = {1.5, 2.0, 2.1, 2.2};
+  double lmin = 1.0, lmax = lmax_tab[std::min(p, 4) - 1];
+  double theta = (lmax + lmin) / 2.0;
+  double delta = (lmax - lmin) / 2.0;
+  double sigma = theta / std::max(delta, 1e-10);
+  A(x.data(), Ax.data());
+  for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ax[n];
+  double rc = 1.0 / sigma;
+  for (int k = 1; k <= n_iter; ++k) {
+    for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
+    if (k == 1) {
+      for (size_t n = 0; n < N; ++n) d[n] = z[n] / theta;
+    } else {
+      double rn = 1.0 / (2.0 * sigma - rc);
+      for (size_t n = 0; n < N; ++n) d[n] = rn * rc * d[n] + (2.0 * rn / delta) * z[n];
+      rc = rn;
+    }
+    for (size_t n = 0; n < N; ++n) x[n] = x[n] + d[n];
+    A(x.data(), Ax.data());
+    for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ax[n];
+  }
+}
+
+// Solver dispatch shared by solve_elliptic! (2D elliptic.jl:413-427,
+// 1D elliptic.jl:451-467) and solve_sensitivity_elliptic!.
+template <class T, class Apply>
+int run_solver(int solver, Apply A, std::vector<T>& x, const std::vector<T>& b,
+               const std::vector<T>& Md, int n_iter, int p, SolverTrace* tr) {
+  if (solver == PCG) pcg_fixed(A, x, b, Md, n_iter, tr);
+  else if (solver == JACOBI) jacobi_fixed(A, x, b, Md, n_iter, 2.0 / 3.0);
+  else if (solver == CHEBYSHEV) chebyshev_fixed(A, x, b, Md, n_iter, p);
+  else return -1;   // error("Unknown solver: ...")
+  return 0;
+}
+
+// solve_elliptic!, IGR_adjoints_2d/elliptic.jl:413-427.  S is the warm start.
+template <class T>
+int solve_elliptic2(const Grid2& G, T alpha, int n_iter, int solver, const T* q, T* S,
+                    SolverTrace* tr = nullptr) {
+  const int N = G.N;
+  std::vector<T> b(N), Md(N), x(S, S + N);
+  igr_rhs2(G, alpha, q, b.data());
+  sip_diag2(G, alpha, q + 2 * (size_t)N, Md.data());
+  const T* rho = q + 2 * (size_t)N;
+  auto A = [&](const T* in, T* out) { apply_sip2(G, alpha, in, rho, out); };
+  int rc = run_solver(solver, A, x, b, Md, n_iter, G.B.p, tr);
+  std::copy(x.begin(), x.end(), S);
+  return rc;
+}
+
+// compute_igr_euler_rhs!, IGR_adjoints_2d/forward.jl:141-150.
+template <class T>
+int euler_rhs2(const Grid2& G, T g, T alpha, int n_iter, int solver, const T* q, T* S, T* dq) {
+  if (alpha > 0.0) {
+    int rc = solve_elliptic2(G, alpha, n_iter, solver, q, S);
+    if (rc) return rc;
+  } else {
+    for (int n = 0; n < G.N; ++n) S[n] = 0.0;
+  }
+  hyperbolic_rhs2(G, g, q, S, dq);
+  return 0;
+}
+
+// One SSP-RK3 step, IGR_adjoints_2d/forward.jl:170-188 (nv = 4 in 2D, 3 in 1D).
+template <class T, class Rhs>
+int ssp_rk3_step(Rhs rhs, size_t nvN, T* q, double dt, std::vector<T>& q1, std::vector<T>& q2,
+                 std::vector<T>& dq) {
+  int rc = rhs(q, dq.data());
+  if (rc) return rc;
+  for (size_t n = 0; n < nvN; ++n) q1[n] = q[n] + dt * dq[n];
+  rc = rhs(q1.data(), dq.data());
+  if (rc) return rc;
+  for (size_t n = 0; n < nvN; ++n) q2[n] = 3.0 / 4.0 * q[n] + 1.0 / 4.0 * q1[n] + 1.0 / 4.0 * dt * dq[n];
+  rc = rhs(q2.data(), dq.data());
+  if (rc) return rc;
+  for (size_t n = 0; n < nvN; ++n) q[n] = 1.0 / 3.0 * q[n] + 2.0 / 3.0 * q2[n] + 2.0 / 3.0 * dt * dq[n];
+  return 0;
+}
+
+// Fixed-step loop of the Enzyme objectives, IGR_adjoints_2d/enzyme_adjoint.jl:50-70:
+// Sigma starts at zero, n_steps and dt are used as given, no final solve.
+template <class T>
+int run_steps2(const Grid2& G, T g, T alpha, int n_iter, int solver, int n_steps, double dt,
+               T* q, T* S) {
+  const size_t nvN = 4 * (size_t)G.N;
+  std::vector<T> q1(nvN), q2(nvN), dq(nvN);
+  for (int n = 0; n < G.N; ++n) S[n] = 0.0;
+  auto rhs = [&](const T* qs, T* d) { return euler_rhs2(G, g, alpha, n_iter, solver, qs, S, d); };
+  for (int s = 0; s < n_steps; ++s) {
+    int rc = ssp_rk3_step(rhs, nvN, q, dt, q1, q2, dq);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// run_forward!, IGR_adjoints_2d/forward.jl:160-194: n_steps = ceil(T/dt),
+// dt = T/n_steps, final Sigma solve when alpha > 0.
+template <class T>
+int run_forward2(const Grid2& G, T g, T alpha, int n_iter, int solver, double dt, double Tend,
+                 T* q, T* S, int* n_steps_out) {
+  int n_steps = (int)std::ceil(Tend / dt);
+  dt = Tend / n_steps;
+  if (n_steps_out) *n_steps_out = n_steps;
+  int rc = run_steps2(G, g, alpha, n_iter, solver, n_steps, dt, q, S);
+  if (rc) return rc;
+  if (alpha > 0.0) rc = solve_elliptic2(G, alpha, n_iter, solver, q, S);
+  return rc;
+}
+
+// Objectives.  The first five restate IGR_adjoints_2d/enzyme_adjoint.jl:72-76,
+// 108-113, 145-151, 184-190, 223-228; WINDOWED_DENSITY and DISC_KE are the
+// rho-window and kinetic-energy-in-a-disc objectives the reference does not
+// define (same quadrature sum).
+enum Objective2 { OBJ_L2_DENSITY = 0, OBJ_L2_PRESSURE = 1, OBJ_KINETIC_ENERGY = 2,
+                  OBJ_WINDOWED_KE = 3, OBJ_WEIGHTED_MOMENTUM = 4, OBJ_WINDOWED_DENSITY = 5,
+                  OBJ_DISC_KE = 6 };
+template <class T>
+int objective2(const Grid2& G, int obj, const double* prm, T g, const T* q, T* J) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  const T* mx = q; const T* my = q + N; const T* r = q + 2 * (size_t)N; const T* E = q + 3 * (size_t)N;
+  T acc = 0.0, mass = 0.0;
+  if (obj < 0 || obj > 6) return -1;
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          size_t n = G.at(i, j, ex, ey);
+          double wJ = B.w[i] * B.w[j] * M.Jx * M.Jy;
+          double x = M.x(B, i, ex), y = M.y(B, j, ey);
+          switch (obj) {
+            case OBJ_L2_DENSITY: acc += wJ * (r[n] * r[n]); break;
+            case OBJ_L2_PRESSURE: {
+              T P = (g - 1.0) * (E[n] - (mx[n] * mx[n] + my[n] * my[n]) / (2.0 * r[n]));
+              acc += wJ * (P * P); break;
+            }
+            case OBJ_KINETIC_ENERGY:
+              acc += wJ * (mx[n] * mx[n] + my[n] * my[n]) / (2.0 * r[n]);
+              mass += wJ * r[n]; break;
+            case OBJ_WINDOWED_KE: {
+              double dx_ = x - prm[0], dy_ = y - prm[1];
+              double wt = std::exp(-(dx_ * dx_ + dy_ * dy_) / (prm[2] * prm[2]));
+              acc += wJ * wt * (mx[n] * mx[n] + my[n] * my[n]) / (2.0 * r[n]); break;
+            }
+            case OBJ_WEIGHTED_MOMENTUM: {
+              double gg = std::sin(2.0 * M_PI * x / M.Lx) * std::sin(2.0 * M_PI * y / M.Ly);
+              acc += wJ * mx[n] * gg; break;
+            }
+            case OBJ_WINDOWED_DENSITY: {
+              double dx_ = x - prm[0], dy_ = y - prm[1];
+              double wt = std::exp(-(dx_ * dx_ + dy_ * dy_) / (prm[2] * prm[2]));
+              acc += wJ * r[n] * wt; break;
+            }
+            case OBJ_DISC_KE: {
+              double dx_ = x - prm[0], dy_ = y - prm[1];
+              double wt = (dx_ * dx_ + dy_ * dy_ < prm[2] * prm[2]) ? 1.0 : 0.0;
+              acc += wJ * wt * (mx[n] * mx[n] + my[n] * my[n]) / (2.0 * r[n]); break;
+            }
+          }
+        }
+  *J = (obj == OBJ_KINETIC_ENERGY) ? acc / mass : acc;
+  return 0;
+}
+
+// ------------------------------------------------ 2D tangent (sensitivity) --
+// IGR_adjoints_2d/sensitivity.jl.  s = (zx, zy, sig, eta), vs = varsigma.
+template <class T>
+inline void sflux2(int dir, T g, const T q[5], const T s[5], T f[4]) {
+  // sensitivity_flux_x/_y, sensitivity.jl:43-76 with _pressure_sensitivity :34-36
+  T mx = q[0], my = q[1], r = q[2], E = q[3], S = q[4];
+  T zx = s[0], zy = s[1], sg = s[2], et = s[3], vs = s[4];
+  T ux = mx / r, uy = my / r;
+  T sux = (zx - ux * sg) / r;
+  T suy = (zy - uy * sg) / r;
+  T P = pressure2(g, mx, my, r, E);
+  T Phi = (g - 1.0) * (et - sg * (ux * ux + uy * uy) / 2.0 - r * (ux * sux + uy * suy));
+  T Pt = Phi + vs;
+  if (dir == 0) {
+    f[0] = zx * ux + mx * sux + Pt; f[1] = zx * uy + mx * suy; f[2] = zx;
+    f[3] = (et + Pt) * ux + (E + P + S) * sux;
+  } else {
+    f[0] = zx * uy + mx * suy; f[1] = zy * uy + my * suy + Pt; f[2] = zy;
+    f[3] = (et + Pt) * uy + (E + P + S) * suy;
+  }
+}
+// sensitivity_llf_flux_x/_y, sensitivity.jl:85-118 (primal wavespeed, no delta-lambda).
+template <class T>
+inline void sllf2(int dir, T g, const T qL[5], const T qR[5], const T sL[5], const T sR[5], T f[4]) {
+  T lam = dir == 0 ? max_(wave_x2(g, qL[0], qL[1], qL[2], qL[3]), wave_x2(g, qR[0], qR[1], qR[2], qR[3]))
+                   : max_(wave_y2(g, qL[0], qL[1], qL[2], qL[3]), wave_y2(g, qR[0], qR[1], qR[2], qR[3]));
+  T fL[4], fR[4];
+  sflux2(dir, g, qL, sL, fL); sflux2(dir, g, qR, sR, fR);
+  for (int v = 0; v < 4; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sR[v]) / 2.0;
+}
+// compute_sensitivity_hyperbolic_rhs!, sensitivity.jl:131-215.
+template <class T>
+void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* s, const T* vs, T* ds) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
+  std::vector<T> Fx(4 * (size_t)N), Fy(4 * (size_t)N);
+  auto gq = [&](size_t n, T o[5]) { o[0] = q[n]; o[1] = q[N + n]; o[2] = q[2 * (size_t)N + n]; o[3] = q[3 * (size_t)N + n]; o[4] = S[n]; };
+  auto gs = [&](size_t n, T o[5]) { o[0] = s[n]; o[1] = s[N + n]; o[2] = s[2 * (size_t)N + n]; o[3] = s[3 * (size_t)N + n]; o[4] = vs[n]; };
+  for (int n = 0; n < N; ++n) {
+    T qq[5], ss[5], f[4]; gq(n, qq); gs(n, ss);
+    sflux2(0, g, qq, ss, f); for (int v = 0; v < 4; ++v) Fx[(size_t)v * N + n] = f[v];
+    sflux2(1, g, qq, ss, f); for (int v = 0; v < 4; ++v) Fy[(size_t)v * N + n] = f[v];
+  }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          T a[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int k = 0; k < np; ++k)
+            for (int v = 0; v < 4; ++v)
+              a[v] -= invJx * B.Dik(i, k) * Fx[(size_t)v * N + G.at(k, j, ex, ey)] +
+                      invJy * B.Dik(j, k) * Fy[(size_t)v * N + G.at(i, k, ex, ey)];
+          for (int v = 0; v < 4; ++v) ds[(size_t)v * N + G.at(i, j, ex, ey)] = a[v];
+        }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int exR = ex == M.Nex - 1 ? 0 : ex + 1;
+      for (int j = 0; j < np; ++j) {
+        size_t L = G.at(np - 1, j, ex, ey), R = G.at(0, j, exR, ey);
+        T qL[5], qR[5], sL[5], sR[5], f[4];
+        gq(L, qL); gq(R, qR); gs(L, sL); gs(R, sR);
+        sllf2(0, g, qL, qR, sL, sR, f);
+        for (int v = 0; v < 4; ++v) ds[(size_t)v * N + L] -= invJx * (f[v] - Fx[(size_t)v * N + L]) / B.w[np - 1];
+        for (int v = 0; v < 4; ++v) ds[(size_t)v * N + R] += invJx * (f[v] - Fx[(size_t)v * N + R]) / B.w[0];
+      }
+    }
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex) {
+      int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
+      for (int i = 0; i < np; ++i) {
+        size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
+        T qL[5], qR[5], sL[5], sR[5], f[4];
+        gq(L, qL); gq(R, qR); gs(L, sL); gs(R, sR);
+        sllf2(1, g, qL, qR, sL, sR, f);
+        for (int v = 0; v < 4; ++v) ds[(size_t)v * N + L] -= invJy * (f[v] - Fy[(size_t)v * N + L]) / B.w[np - 1];
+        for (int v = 0; v < 4; ++v) ds[(size_t)v * N + R] += invJy * (f[v] - Fy[(size_t)v * N + R]) / B.w[0];
+      }
+    }
+}
+// compute_sensitivity_elliptic_rhs!, sensitivity.jl:236-292.
+template <class T>
+void sens_elliptic_rhs2(const Grid2& G, T alpha, const T* q, const T* S, const T* s, T* b) {
+  const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
+  const double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
+  const T* mx = q; const T* my = q + N; const T* r = q + 2 * (size_t)N;
+  const T* zx = s; const T* zy = s + N; const T* sg = s + 2 * (size_t)N;
+  for (int ey = 0; ey < M.Ney; ++ey)
+    for (int ex = 0; ex < M.Nex; ++ex)
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i) {
+          T uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
+          T sxx = 0.0, sxy = 0.0, syx = 0.0, syy = 0.0;
+          T Sx = 0.0, Sy = 0.0, dxs = 0.0, dys = 0.0;
+          for (int k = 0; k < np; ++k) {
+            size_t kj = G.at(k, j, ex, ey), ik = G.at(i, k, ex, ey);
+            T uxkj = mx[kj] / r[kj], uykj = my[kj] / r[kj];
+            T uxik = mx[ik] / r[ik], uyik = my[ik] / r[ik];
+            T sxkj = (zx[kj] - uxkj * sg[kj]) / r[kj];
+            T sykj = (zy[kj] - uykj * sg[kj]) / r[kj];
+            T sxik = (zx[ik] - uxik * sg[ik]) / r[ik];
+            T syik = (zy[ik] - uyik * sg[ik]) / r[ik];
+            double Di = B.Dik(i, k), Dj = B.Dik(j, k);
+            uxx += Di * uxkj; uyx += Di * uykj;
+            uxy += Dj * uxik; uyy += Dj * uyik;
+            sxx += Di * sxkj; syx += Di * sykj;
+            sxy += Dj * sxik; syy += Dj * syik;
+            Sx += Di * S[kj]; Sy += Dj * S[ik];
+            dxs += Di * (sg[kj] / r[kj]);
+            dys += Dj * (sg[ik] / r[ik]);
+          }
+          uxx *= invJx; uyx *= invJx; uxy *= invJy; uyy *= invJy;
+          sxx *= invJx; syx *= invJx; sxy *= invJy; syy *= invJy;
+          Sx *= invJx; Sy *= invJy;
+          dxs *= invJx; dys *= invJy;
+          T divu = uxx + uyy, divs = sxx + syy;
+          T Rp = alpha * (divu * divu + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
+          T Rs = alpha * (2.0 * divu * divs + 2.0 * (uxx * sxx + uxy * syx + uyx * sxy + uyy * syy));
+          size_t n = G.at(i, j, ex, ey);
+          T rr = r[n];
+          T sr = sg[n] / rr;
+          T bpt = Rs + sr * Rp - alpha * (dxs * Sx + dys * Sy) / rr;
+          b[n] = B.w[i] * B.w[j] * M.Jx * M.Jy * bpt;
+        }
+}
+// solve_sensitivity_elliptic!, sensitivity.jl:301-316.
+template <class T>
+int solve_sens_elliptic2(const Grid2& G, T alpha, int n_iter, int solver, const T* q, const T* S,
+                         const T* s, T* vs) {
+  const int N = G.N;
+  std::vector<T> b(N), Md(N), x(vs, vs + N);
+  sens_elliptic_rhs2(G, alpha, q, S, s, b.data());
+  const T* rho = q + 2 * (size_t)N;
+  sip_diag2(G, alpha, rho, Md.data());
+  auto A = [&](const T* in, T* out) { apply_sip2(G, alpha, in, rho, out); };
+  int rc = run_solver(solver, A, x, b, Md, n_iter, G.B.p, nullptr);
+  std::copy(x.begin(), x.end(), vs);
+  return rc;
+}
+// run_sensitivity, sensitivity.jl:353-413 (coupled primal + tangent SSP-RK3).
+template <class T>
+int run_sensitivity2(const Grid2& G, T g, T alpha, int n_iter, int solver, double dt, double Tend,
+                     T* q, T* S, T* s, T* vs, int* n_steps_out) {
+  const int N = G.N; const size_t nvN = 4 * (size_t)N;
+  for (int n = 0; n < N; ++n) { S[n] = 0.0; vs[n] = 0.0; }
+  std::vector<T> q1(nvN), q2(nvN), dq(nvN), s1(nvN), s2(nvN), ds(nvN);
+  int n_steps = (int)std::ceil(Tend / dt);
+  dt = Tend / n_steps;
+  if (n_steps_out) *n_steps_out = n_steps;
+  auto rhs = [&](const T* qs, const T* ss, T* dqo, T* dso) -> int {
+    int rc = euler_rhs2(G, g, alpha, n_iter, solver, qs, S, dqo);
+    if (rc) return rc;
+    if (alpha > 0.0) { rc = solve_sens_elliptic2(G, alpha, n_iter, solver, qs, S, ss, vs); if (rc) return rc; }
+    else for (int n = 0; n < N; ++n) vs[n] = 0.0;
+    sens_hyperbolic_rhs2(G, g, qs, S, ss, vs, dso);
+    return 0;
+  };
+  for (int st = 0; st < n_steps; ++st) {
+    int rc = rhs(q, s, dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) { q1[n] = q[n] + dt * dq[n]; s1[n] = s[n] + dt * ds[n]; }
+    rc = rhs(q1.data(), s1.data(), dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) {
+      q2[n] = 3.0 / 4.0 * q[n] + 1.0 / 4.0 * q1[n] + 1.0 / 4.0 * dt * dq[n];
+      s2[n] = 3.0 / 4.0 * s[n] + 1.0 / 4.0 * s1[n] + 1.0 / 4.0 * dt * ds[n];
+    }
+    rc = rhs(q2.data(), s2.data(), dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) {
+      q[n] = 1.0 / 3.0 * q[n] + 2.0 / 3.0 * q2[n] + 2.0 / 3.0 * dt * dq[n];
+      s[n] = 1.0 / 3.0 * s[n] + 2.0 / 3.0 * s2[n] + 2.0 / 3.0 * dt * ds[n];
+    }
+  }
+  if (alpha > 0.0) {
+    int rc = solve_elliptic2(G, alpha, n_iter, solver, q, S); if (rc) return rc;
+    rc = solve_sens_elliptic2(G, alpha, n_iter, solver, q, S, s, vs); if (rc) return rc;
+  }
+  return 0;
+}
+
+// =====================================================================
+//                              1D
+// =====================================================================
+struct Grid1 {
+  const Basis& B; const Mesh1D& M; int np, N;
+  Grid1(const Basis& b, const Mesh1D& m) : B(b), M(m), np(b.np), N(b.np * m.Ne) {}
+  size_t at(int i, int e) const { return (size_t)i + (size_t)np * e; }
+};
+// IGR_adjoints_1d/euler.jl:13-78
+template <class T> inline T pressure1(T g, T m, T r, T E) { return (g - 1.0) * (E - m * m / (2.0 * r)); }
+template <class T> inline void flux1(T g, T m, T r, T E, T S, T f[3]) {
+  T P = pressure1(g, m, r, E);
+  T pb = P + S;
+  f[0] = m * m / r + pb; f[1] = m; f[2] = (E + pb) * m / r;
+}
+template <class T> inline T wave1(T g, T m, T r, T E) {
+  T u = m / r;
+  T P = pressure1(g, m, r, E);
+  T c = sqrt_(g * P / r);
+  return abs_(u) + c;
+}
+template <class T> inline void llf1(T g, const T qL[4], const T qR[4], T f[3]) {
+  T lam = max_(wave1(g, qL[0], qL[1], qL[2]), wave1(g, qR[0], qR[1], qR[2]));
+  T fL[3], fR[3];
+  flux1(g, qL[0], qL[1], qL[2], qL[3], fL); flux1(g, qR[0], qR[1], qR[2], qR[3], fR);
+  for (int v = 0; v < 3; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
+}
+// compute_hyperbolic_rhs!, IGR_adjoints_1d/hyperbolic.jl:23-83.  q = (mu, rho, E).
+template <class T>
+void hyperbolic_rhs1(const Grid1& G, T g, const T* q, const T* S, T* dq) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  std::vector<T> F(3 * (size_t)N);
+  for (int n = 0; n < N; ++n) {
+    T f[3]; flux1(g, q[n], q[N + n], q[2 * (size_t)N + n], S[n], f);
+    for (int v = 0; v < 3; ++v) F[(size_t)v * N + n] = f[v];
+  }
+  double invJ = 1.0 / G.M.J;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      for (int v = 0; v < 3; ++v) dq[(size_t)v * N + G.at(i, e)] = 0.0;
+      for (int j = 0; j < np; ++j)
+        for (int v = 0; v < 3; ++v)
+          dq[(size_t)v * N + G.at(i, e)] -= invJ * B.Dik(i, j) * F[(size_t)v * N + G.at(j, e)];
+    }
+  auto gq = [&](size_t n, T o[4]) { o[0] = q[n]; o[1] = q[N + n]; o[2] = q[2 * (size_t)N + n]; o[3] = S[n]; };
+  for (int e = 0; e < Ne; ++e) {
+    int eL = e == 0 ? Ne - 1 : e - 1, eR = e == Ne - 1 ? 0 : e + 1;
+    T a[4], b[4], fR[3], fL[3];
+    gq(G.at(np - 1, e), a); gq(G.at(0, eR), b); llf1(g, a, b, fR);
+    gq(G.at(np - 1, eL), a); gq(G.at(0, e), b); llf1(g, a, b, fL);
+    size_t nr = G.at(np - 1, e), nl = G.at(0, e);
+    for (int v = 0; v < 3; ++v) dq[(size_t)v * N + nr] -= invJ * (fR[v] - F[(size_t)v * N + nr]) / B.w[np - 1];
+    for (int v = 0; v < 3; ++v) dq[(size_t)v * N + nl] += invJ * (fL[v] - F[(size_t)v * N + nl]) / B.w[0];
+  }
+}
+// apply_sip!, IGR_adjoints_1d/elliptic.jl:32-116.
+template <class T>
+void apply_sip1(const Grid1& G, T alpha, const T* x, const T* rho, T* y) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const double J = G.M.J, invJ = 1.0 / J, eta = (double)((B.p + 1) * (B.p + 1));
+  for (int n = 0; n < N; ++n) y[n] = 0.0;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      size_t n = G.at(i, e);
+      y[n] += B.w[i] * J * x[n] / rho[n];
+      T dxi = 0.0;
+      for (int j = 0; j < np; ++j) dxi += B.Dik(i, j) * x[G.at(j, e)];
+      for (int k = 0; k < np; ++k) y[G.at(k, e)] += alpha * B.w[i] * invJ * dxi / rho[n] * B.Dik(i, k);
+    }
+  for (int e = 0; e < Ne; ++e) {
+    int eR = e == Ne - 1 ? 0 : e + 1;
+    size_t L = G.at(np - 1, e), R = G.at(0, eR);
+    T SL = x[L], SR = x[R], rL = rho[L], rR = rho[R];
+    T jump = SL - SR;
+    T dL = 0.0, dR = 0.0;
+    for (int j = 0; j < np; ++j) { dL += B.Dik(np - 1, j) * x[G.at(j, e)]; dR += B.Dik(0, j) * x[G.at(j, eR)]; }
+    T avg = invJ * ((1.0 / rL) * dL + (1.0 / rR) * dR) / 2.0;
+    T kap = max_(1.0 / rL, 1.0 / rR);
+    T sig = eta * kap / G.M.dx;
+    y[L] += -alpha * avg * 1.0;
+    y[R] += -alpha * avg * -1.0;
+    for (int k = 0; k < np; ++k) {
+      y[G.at(k, e)] += -alpha * (1.0 / (2.0 * rL)) * invJ * B.Dik(np - 1, k) * 1.0 * jump;
+      y[G.at(k, eR)] += -alpha * (1.0 / (2.0 * rR)) * invJ * B.Dik(0, k) * 1.0 * jump;
+    }
+    y[L] += alpha * sig * jump;
+    y[R] -= alpha * sig * jump;
+  }
+}
+// compute_igr_rhs!, IGR_adjoints_1d/elliptic.jl:126-154.
+template <class T>
+void igr_rhs1(const Grid1& G, T alpha, const T* q, T* b) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const double J = G.M.J, invJ = 1.0 / J;
+  const T* m = q; const T* r = q + N;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      T du = 0.0;
+      for (int j = 0; j < np; ++j) { T uj = m[G.at(j, e)] / r[G.at(j, e)]; du += B.Dik(i, j) * uj; }
+      du *= invJ;
+      T R = 2.0 * alpha * (du * du);
+      b[G.at(i, e)] = B.w[i] * J * R;
+    }
+}
+// compute_sip_diagonal, IGR_adjoints_1d/elliptic.jl:206-253.
+template <class T>
+void sip_diag1(const Grid1& G, T alpha, const T* rho, T* dg) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const double J = G.M.J, invJ = 1.0 / J, eta = (double)(np * np);
+  for (int n = 0; n < N; ++n) dg[n] = 0.0;
+  for (int e = 0; e < Ne; ++e) for (int i = 0; i < np; ++i) dg[G.at(i, e)] += B.w[i] * J / rho[G.at(i, e)];
+  for (int e = 0; e < Ne; ++e)
+    for (int k = 0; k < np; ++k)
+      for (int i = 0; i < np; ++i) {
+        double D2 = B.Dik(i, k) * B.Dik(i, k);
+        dg[G.at(k, e)] += alpha * B.w[i] * invJ * D2 / rho[G.at(i, e)];
+      }
+  for (int e = 0; e < Ne; ++e) {
+    int eR = e == Ne - 1 ? 0 : e + 1;
+    size_t L = G.at(np - 1, e), R = G.at(0, eR);
+    T rL = rho[L], rR = rho[R];
+    T kap = max_(1.0 / rL, 1.0 / rR);
+    T sig = eta * kap / G.M.dx;
+    dg[L] += -alpha * invJ * B.Dik(np - 1, np - 1) / rL;
+    dg[L] += alpha * sig;
+    dg[R] += alpha * invJ * B.Dik(0, 0) / rR;
+    dg[R] += alpha * sig;
+  }
+}
+// solve_elliptic!, IGR_adjoints_1d/elliptic.jl:451-467.
+template <class T>
+int solve_elliptic1(const Grid1& G, T alpha, int n_iter, int solver, const T* q, T* S,
+                    SolverTrace* tr = nullptr) {
+  const int N = G.N;
+  std::vector<T> b(N), Md(N), x(S, S + N);
+  igr_rhs1(G, alpha, q, b.data());
+  const T* rho = q + N;
+  sip_diag1(G, alpha, rho, Md.data());
+  auto A = [&](const T* in, T* out) { apply_sip1(G, alpha, in, rho, out); };
+  int rc = run_solver(solver, A, x, b, Md, n_iter, G.B.p, tr);
+  std::copy(x.begin(), x.end(), S);
+  return rc;
+}
+// compute_igr_euler_rhs!, IGR_adjoints_1d/forward.jl:23-33.
+template <class T>
+int euler_rhs1(const Grid1& G, T g, T alpha, int n_iter, int solver, const T* q, T* S, T* dq) {
+  if (alpha > 0.0) { int rc = solve_elliptic1(G, alpha, n_iter, solver, q, S); if (rc) return rc; }
+  else for (int n = 0; n < G.N; ++n) S[n] = 0.0;
+  hyperbolic_rhs1(G, g, q, S, dq);
+  return 0;
+}
+// IGR_adjoints_1d/enzyme_adjoint.jl:34-63 (fixed n_steps, dt; Sigma zeroed).
+template <class T>
+int run_steps1(const Grid1& G, T g, T alpha, int n_iter, int solver, int n_steps, double dt, T* q, T* S) {
+  const size_t nvN = 3 * (size_t)G.N;
+  std::vector<T> q1(nvN), q2(nvN), dq(nvN);
+  for (int n = 0; n < G.N; ++n) S[n] = 0.0;
+  auto rhs = [&](const T* qs, T* d) { return euler_rhs1(G, g, alpha, n_iter, solver, qs, S, d); };
+  for (int s = 0; s < n_steps; ++s) { int rc = ssp_rk3_step(rhs, nvN, q, dt, q1, q2, dq); if (rc) return rc; }
+  return 0;
+}
+// run_forward!, IGR_adjoints_1d/forward.jl:50-87.
+template <class T>
+int run_forward1(const Grid1& G, T g, T alpha, int n_iter, int solver, double dt, double Tend, T* q, T* S,
+                 int* n_steps_out) {
+  int n_steps = (int)std::ceil(Tend / dt);
+  dt = Tend / n_steps;
+  if (n_steps_out) *n_steps_out = n_steps;
+  int rc = run_steps1(G, g, alpha, n_iter, solver, n_steps, dt, q, S);
+  if (rc) return rc;
+  if (alpha > 0.0) rc = solve_elliptic1(G, alpha, n_iter, solver, q, S);
+  return rc;
+}
+// 1D objectives, IGR_adjoints_1d/enzyme_adjoint.jl:65-71 (l2 density),
+// l2 pressure, weighted momentum, kinetic energy / mass.
+template <class T>
+int objective1(const Grid1& G, int obj, T g, const T* q, T* J) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const T* m = q; const T* r = q + N; const T* E = q + 2 * (size_t)N;
+  T acc = 0.0, mass = 0.0;
+  double Ld = Ne * G.M.dx;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      size_t n = G.at(i, e);
+      double wJ = B.w[i] * G.M.J;
+      if (obj == OBJ_L2_DENSITY) acc += wJ * (r[n] * r[n]);
+      else if (obj == OBJ_L2_PRESSURE) { T P = (g - 1.0) * (E[n] - m[n] * m[n] / (2.0 * r[n])); acc += wJ * (P * P); }
+      else if (obj == OBJ_KINETIC_ENERGY) { acc += wJ * (m[n] * m[n]) / (2.0 * r[n]); mass += wJ * r[n]; }
+      else if (obj == OBJ_WEIGHTED_MOMENTUM) acc += wJ * m[n] * std::sin(2.0 * M_PI * G.M.x(B, i, e) / Ld);
+      else return -1;
+    }
+  *J = (obj == OBJ_KINETIC_ENERGY) ? acc / mass : acc;
+  return 0;
+}
+
+// ------------------------------------------------ 1D tangent (sensitivity) --
+// IGR_adjoints_1d/sensitivity.jl:36-50 (sensitivity_flux, with s_gamma source).
+template <class T>
+inline void sflux1(T g, const T q[4], const T s[4], T sgam, T f[3]) {
+  T m = q[0], r = q[1], E = q[2], S = q[3];
+  T sm = s[0], sr = s[1], sE = s[2], sS = s[3];
+  T u = m / r;
+  T su = (sm - u * sr) / r;
+  T P = pressure1(g, m, r, E);
+  T e = E / r - u * u / 2.0;
+  T sP = (g - 1.0) * (sE - sr * (u * u) / 2.0 - r * u * su) + sgam * r * e;
+  f[0] = sm * u + m * su + sP + sS;
+  f[1] = sm;
+  f[2] = su * (E + P + S) + u * (sE + sP + sS);
+}
+// sensitivity_llf_flux, sensitivity.jl:58-78.
+template <class T>
+inline void sllf1(T g, const T qL[4], const T qR[4], const T sL[4], const T sR[4], T sgam, T f[3]) {
+  T lam = max_(wave1(g, qL[0], qL[1], qL[2]), wave1(g, qR[0], qR[1], qR[2]));
+  T fL[3], fR[3];
+  sflux1(g, qL, sL, sgam, fL); sflux1(g, qR, sR, sgam, fR);
+  for (int v = 0; v < 3; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sR[v]) / 2.0;
+}
+// compute_sensitivity_hyperbolic_rhs!, sensitivity.jl:91-150.
+template <class T>
+void sens_hyperbolic_rhs1(const Grid1& G, T g, const T* q, const T* S, const T* s, const T* sS, T sgam, T* ds) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  auto gq = [&](size_t n, T o[4]) { o[0] = q[n]; o[1] = q[N + n]; o[2] = q[2 * (size_t)N + n]; o[3] = S[n]; };
+  auto gs = [&](size_t n, T o[4]) { o[0] = s[n]; o[1] = s[N + n]; o[2] = s[2 * (size_t)N + n]; o[3] = sS[n]; };
+  std::vector<T> F(3 * (size_t)N);
+  for (int n = 0; n < N; ++n) {
+    T a[4], b[4], f[3]; gq(n, a); gs(n, b); sflux1(g, a, b, sgam, f);
+    for (int v = 0; v < 3; ++v) F[(size_t)v * N + n] = f[v];
+  }
+  double invJ = 1.0 / G.M.J;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      for (int v = 0; v < 3; ++v) ds[(size_t)v * N + G.at(i, e)] = 0.0;
+      for (int j = 0; j < np; ++j)
+        for (int v = 0; v < 3; ++v)
+          ds[(size_t)v * N + G.at(i, e)] -= invJ * B.Dik(i, j) * F[(size_t)v * N + G.at(j, e)];
+    }
+  for (int e = 0; e < Ne; ++e) {
+    int eL = e == 0 ? Ne - 1 : e - 1, eR = e == Ne - 1 ? 0 : e + 1;
+    T qa[4], qb[4], sa[4], sb[4], fR[3], fL[3];
+    gq(G.at(np - 1, e), qa); gq(G.at(0, eR), qb); gs(G.at(np - 1, e), sa); gs(G.at(0, eR), sb);
+    sllf1(g, qa, qb, sa, sb, sgam, fR);
+    gq(G.at(np - 1, eL), qa); gq(G.at(0, e), qb); gs(G.at(np - 1, eL), sa); gs(G.at(0, e), sb);
+    sllf1(g, qa, qb, sa, sb, sgam, fL);
+    size_t nr = G.at(np - 1, e), nl = G.at(0, e);
+    for (int v = 0; v < 3; ++v) ds[(size_t)v * N + nr] -= invJ * (fR[v] - F[(size_t)v * N + nr]) / B.w[np - 1];
+    for (int v = 0; v < 3; ++v) ds[(size_t)v * N + nl] += invJ * (fL[v] - F[(size_t)v * N + nl]) / B.w[0];
+  }
+}
+// compute_sensitivity_elliptic_rhs_form1!, sensitivity.jl:168-256.
+template <class T>
+void sens_elliptic_rhs1_form1(const Grid1& G, T alpha, const T* q, const T* S, const T* s, T* b) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const double J = G.M.J, invJ = 1.0 / J, etap = (double)(np * np);
+  const T* m = q; const T* r = q + N; const T* sm = s; const T* sr = s + N;
+  for (int n = 0; n < N; ++n) b[n] = 0.0;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      size_t n = G.at(i, e);
+      T ux = 0.0, sux = 0.0;
+      for (int j = 0; j < np; ++j) {
+        size_t nj = G.at(j, e);
+        T uj = m[nj] / r[nj];
+        T suj = (sm[nj] - uj * sr[nj]) / r[nj];
+        ux += B.Dik(i, j) * uj; sux += B.Dik(i, j) * suj;
+      }
+      ux *= invJ; sux *= invJ;
+      T src = 4.0 * alpha * ux * sux + sr[n] * S[n] / (r[n] * r[n]);
+      b[n] += B.w[i] * J * src;
+      T kap = sr[n] / (r[n] * r[n]);
+      T dS = 0.0;
+      for (int j = 0; j < np; ++j) dS += B.Dik(i, j) * S[G.at(j, e)];
+      for (int k = 0; k < np; ++k) b[G.at(k, e)] += alpha * B.w[i] * invJ * kap * dS * B.Dik(i, k);
+    }
+  for (int e = 0; e < Ne; ++e) {
+    int eR = e == Ne - 1 ? 0 : e + 1;
+    size_t L = G.at(np - 1, e), R = G.at(0, eR);
+    T SL = S[L], SR = S[R];
+    T kL = sr[L] / (r[L] * r[L]), kR = sr[R] / (r[R] * r[R]);
+    T jump = SL - SR;
+    T dL = 0.0, dR = 0.0;
+    for (int j = 0; j < np; ++j) { dL += B.Dik(np - 1, j) * S[G.at(j, e)]; dR += B.Dik(0, j) * S[G.at(j, eR)]; }
+    T avg = invJ * (kL * dL + kR * dR) / 2.0;
+    T kmax = max_(kL, kR);
+    T sig = etap * kmax / G.M.dx;
+    b[L] += -alpha * avg * 1.0;
+    b[R] += -alpha * avg * -1.0;
+    for (int k = 0; k < np; ++k) {
+      b[G.at(k, e)] += -alpha * (kL / 2.0) * invJ * B.Dik(np - 1, k) * 1.0 * jump;
+      b[G.at(k, eR)] += -alpha * (kR / 2.0) * invJ * B.Dik(0, k) * 1.0 * jump;
+    }
+    b[L] += alpha * sig * jump;
+    b[R] -= alpha * sig * jump;
+  }
+}
+// compute_sensitivity_elliptic_rhs_form2!, sensitivity.jl:272-314.
+template <class T>
+void sens_elliptic_rhs1_form2(const Grid1& G, T alpha, const T* q, const T* S, const T* s, T* b) {
+  const int np = G.np, N = G.N, Ne = G.M.Ne; const Basis& B = G.B;
+  const double J = G.M.J, invJ = 1.0 / J;
+  const T* m = q; const T* r = q + N; const T* sm = s; const T* sr = s + N;
+  for (int e = 0; e < Ne; ++e)
+    for (int i = 0; i < np; ++i) {
+      T ux = 0.0, sux = 0.0, dso = 0.0, dS = 0.0;
+      for (int j = 0; j < np; ++j) {
+        size_t nj = G.at(j, e);
+        T uj = m[nj] / r[nj];
+        T suj = (sm[nj] - uj * sr[nj]) / r[nj];
+        ux += B.Dik(i, j) * uj; sux += B.Dik(i, j) * suj;
+        dso += B.Dik(i, j) * (sr[nj] / r[nj]);
+        dS += B.Dik(i, j) * S[nj];
+      }
+      ux *= invJ; sux *= invJ; dso *= invJ; dS *= invJ;
+      size_t n = G.at(i, e);
+      T so = sr[n] / r[n];
+      T R = alpha * (2.0 * ux * (2.0 * sux + so * ux) - dso * dS / r[n]);
+      b[n] = B.w[i] * J * R;
+    }
+}
+// solve_sensitivity_elliptic!, sensitivity.jl:334-357.
+template <class T>
+int solve_sens_elliptic1(const Grid1& G, T alpha, int n_iter, int solver, int form, const T* q, const T* S,
+                         const T* s, T* sS) {
+  const int N = G.N;
+  std::vector<T> b(N), Md(N), x(sS, sS + N);
+  if (form == 1) sens_elliptic_rhs1_form1(G, alpha, q, S, s, b.data());
+  else if (form == 2) sens_elliptic_rhs1_form2(G, alpha, q, S, s, b.data());
+  else return -2;
+  const T* rho = q + N;
+  sip_diag1(G, alpha, rho, Md.data());
+  auto A = [&](const T* in, T* out) { apply_sip1(G, alpha, in, rho, out); };
+  int rc = run_solver(solver, A, x, b, Md, n_iter, G.B.p, nullptr);
+  std::copy(x.begin(), x.end(), sS);
+  return rc;
+}
+// run_sensitivity + ssp_rk3_coupled, sensitivity.jl:384-515: the step loop is
+// `while t < T - 1e-14; dt = min(dt, T - t)` without pre-adjusting dt.
+template <class T>
+int run_sensitivity1(const Grid1& G, T g, T alpha, int n_iter, int solver, int form, double sgam_d, double Dt,
+                     double Tend, T* q, T* S, T* s, T* sS, int* n_steps_out) {
+  const int N = G.N; const size_t nvN = 3 * (size_t)N;
+  T sgam = sgam_d;
+  for (int n = 0; n < N; ++n) { S[n] = 0.0; sS[n] = 0.0; }
+  std::vector<T> q1(nvN), q2(nvN), dq(nvN), s1(nvN), s2(nvN), ds(nvN);
+  auto rhs = [&](const T* qs, const T* ss, T* dqo, T* dso) -> int {
+    int rc = euler_rhs1(G, g, alpha, n_iter, solver, qs, S, dqo);
+    if (rc) return rc;
+    if (alpha > 0.0) { rc = solve_sens_elliptic1(G, alpha, n_iter, solver, form, qs, S, ss, sS); if (rc) return rc; }
+    else for (int n = 0; n < N; ++n) sS[n] = 0.0;
+    sens_hyperbolic_rhs1(G, g, qs, S, ss, sS, sgam, dso);
+    return 0;
+  };
+  double t = 0.0; int n_steps = 0;
+  while (t < Tend - 1e-14) {
+    double dt = std::min(Dt, Tend - t);
+    int rc = rhs(q, s, dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) { q1[n] = q[n] + dt * dq[n]; s1[n] = s[n] + dt * ds[n]; }
+    rc = rhs(q1.data(), s1.data(), dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) {
+      q2[n] = 3.0 / 4.0 * q[n] + 1.0 / 4.0 * q1[n] + 1.0 / 4.0 * dt * dq[n];
+      s2[n] = 3.0 / 4.0 * s[n] + 1.0 / 4.0 * s1[n] + 1.0 / 4.0 * dt * ds[n];
+    }
+    rc = rhs(q2.data(), s2.data(), dq.data(), ds.data()); if (rc) return rc;
+    for (size_t n = 0; n < nvN; ++n) {
+      q[n] = 1.0 / 3.0 * q[n] + 2.0 / 3.0 * q2[n] + 2.0 / 3.0 * dt * dq[n];
+      s[n] = 1.0 / 3.0 * s[n] + 2.0 / 3.0 * s2[n] + 2.0 / 3.0 * dt * ds[n];
+    }
+    t += dt; ++n_steps;
+  }
+  if (n_steps_out) *n_steps_out = n_steps;
+  if (alpha > 0.0) {
+    int rc = solve_elliptic1(G, alpha, n_iter, solver, q, S); if (rc) return rc;
+    rc = solve_sens_elliptic1(G, alpha, n_iter, solver, form, q, S, s, sS); if (rc) return rc;
+  }
+  return 0;
+}
+
+}  // namespace orc
