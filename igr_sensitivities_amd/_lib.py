@@ -36,6 +36,11 @@ PROTOTYPES = {
     "igr_run_forward": (_i, [_vp, _d, _d, _d, _d, _i, _i, _pi]),
     "igr_run_steps": (_i, [_vp, _i, _d, _d, _d, _i, _i]),
     "igr_time_kernel": (_i, [_vp, _i, _d, _d, _i, _pd]),
+    "igr_objective": (_i, [_vp, _i, _pd, _d, _pd]),
+    "igr_gradient": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd]),
+    "igr_hyperbolic_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "igr_apply_sip_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "igr_solve_elliptic_vjp": (_i, [_vp, _d, _i, _i, _pd, _pd, _pd, _pd, _pd]),
 }
 
 _lib = None

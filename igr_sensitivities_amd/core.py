@@ -192,6 +192,44 @@ class Context:
         self.check(self._L.igr_run_steps(self.h, int(n_steps), float(dt), float(gamma), float(alpha), int(n_iter),
                                          _solver_code(solver)))
 
+    # discrete adjoint
+    def objective(self, obj, gamma, prm=None):
+        J = C.c_double(0)
+        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        self.check(self._L.igr_objective(self.h, int(obj), _p(p), float(gamma), C.byref(J)))
+        return J.value
+
+    def gradient(self, obj, n_steps, dt, gamma, alpha, n_iter, solver="pcg", prm=None, ckpt_slots=0):
+        """(J, dJ/dq0, dJ/dgamma, dJ/dalpha) of the objective after n_steps fixed steps from the state."""
+        J, gb, ab = C.c_double(0), C.c_double(0), C.c_double(0)
+        qb = np.empty((self.nv,) + self.shape)
+        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        self.check(self._L.igr_gradient(self.h, int(obj), _p(p), int(n_steps), float(dt), float(gamma),
+                                        float(alpha), int(n_iter), _solver_code(solver), int(ckpt_slots),
+                                        C.byref(J), _p(qb), C.byref(gb), C.byref(ab)))
+        return J.value, qb, gb.value, ab.value
+
+    def hyperbolic_vjp(self, gamma, q, sigma, dqbar):
+        q, sigma, dqbar = _f64(q), _f64(sigma), _f64(dqbar)
+        qb, sb, gb = np.empty_like(q), np.empty(self.shape), C.c_double(0)
+        self.check(self._L.igr_hyperbolic_vjp(self.h, float(gamma), _p(q), _p(sigma), _p(dqbar), _p(qb), _p(sb),
+                                              C.byref(gb)))
+        return qb, sb, gb.value
+
+    def apply_sip_vjp(self, alpha, x, rho, ybar):
+        x, rho, ybar = _f64(x), _f64(rho), _f64(ybar)
+        xb, rb, ab = np.empty(self.shape), np.empty(self.shape), C.c_double(0)
+        self.check(self._L.igr_apply_sip_vjp(self.h, float(alpha), _p(x), _p(rho), _p(ybar), _p(xb), _p(rb),
+                                             C.byref(ab)))
+        return xb, rb, ab.value
+
+    def solve_elliptic_vjp(self, alpha, n_iter, solver, sigma0, sigmabar):
+        s0, sb = _f64(sigma0), _f64(sigmabar)
+        qb, s0b, ab = np.empty((self.nv,) + self.shape), np.empty(self.shape), C.c_double(0)
+        self.check(self._L.igr_solve_elliptic_vjp(self.h, float(alpha), int(n_iter), _solver_code(solver), _p(s0),
+                                                  _p(sb), _p(qb), _p(s0b), C.byref(ab)))
+        return qb, s0b, ab.value
+
     def time_kernel(self, kernel, gamma, alpha, reps):
         ms = C.c_double(0)
         self.check(self._L.igr_time_kernel(self.h, int(kernel), float(gamma), float(alpha), int(reps),
@@ -264,6 +302,27 @@ def run_forward_(q, Sigma, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="
     qo, so = c.get_state()
     q[...] = qo
     Sigma[...] = so
+
+
+OBJECTIVES = {"l2_density": 0, "l2_pressure": 1, "kinetic_energy": 2, "windowed_kinetic_energy": 3,
+              "weighted_momentum": 4, "windowed_density": 5, "disc_kinetic_energy": 6}
+
+
+def enzyme_adjoint_ic(q0, n_steps, dt, basis, mesh, gamma, alpha, n_iter=10, solver="pcg",
+                      objective="l2_density", prm=None, ckpt_slots=0):
+    """enzyme_adjoint_ic (IGR_adjoints_2d/enzyme_adjoint.jl:270-296) on the GPU
+    discrete adjoint: returns (J, dq0) with dq0 = dJ/dq0 stacked (mx, my, rho, E).
+    Like the reference, n_iter is clamped to >= 1 (enzyme_adjoint.jl:292)."""
+    key = objective.lstrip(":") if isinstance(objective, str) else objective
+    if isinstance(key, str):
+        if key not in OBJECTIVES:
+            raise ValueError(f"Unknown objective: {objective}")
+        key = OBJECTIVES[key]
+    c = _ctx(mesh)
+    code = _solver_code(solver)
+    c.set_state(q0)
+    J, qb, _, _ = c.gradient(key, n_steps, dt, gamma, alpha, max(int(n_iter), 1), code, prm, ckpt_slots)
+    return J, qb
 
 
 def run_forward(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):

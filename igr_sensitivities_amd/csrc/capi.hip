@@ -20,7 +20,8 @@ void launch2_hyp(const Ctx& c, const HypArgs& a);
 int grid2_blocks(const Ctx& c);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
-void launch1_run(const Ctx& c, const K1Args* dargs, int n_inst, int block);
+void launch1_run(const Ctx& c, const K1Args& a, int block);
+void free_grad_work(const Ctx* c);
 
 double* Ctx::alloc(size_t n) {
   void* p = nullptr;
@@ -198,8 +199,7 @@ int k1_block(const Ctx& c) {
   return N >= 1024 ? 1024 : (int)(((N + 63) / 64) * 64);
 }
 int run1(Ctx& c, const K1Args& a) {
-  HIPCHK(c, hipMemcpyAsync(c.k1args, &a, sizeof(K1Args), hipMemcpyHostToDevice, c.stream));
-  launch1_run(c, c.k1args, 1, k1_block(c));
+  launch1_run(c, a, k1_block(c));
   HIPCHK(c, hipGetLastError());
   return IGR_OK;
 }
@@ -241,7 +241,6 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   void* pp = nullptr;
   if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
   if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->flag = (int*)pp; } else ok = false;
-  if (ok && hipMalloc(&pp, sizeof(K1Args) * 2) == hipSuccess) { c->owned.push_back(pp); c->k1args = (K1Args*)pp; } else ok = false;
   if (ok) ok = ensure_scal(*c, 64) == IGR_OK;
   if (ok) ok = hipMemset(c->counter, 0, 64) == hipSuccess && hipMemset(c->flag, 0, 64) == hipSuccess &&
                hipMemset(c->S, 0, N * sizeof(double)) == hipSuccess &&
@@ -255,6 +254,7 @@ int igr_destroy(igr_ctx* c) {
   if (!c) return IGR_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_graphs(*c);
+  free_grad_work(c);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -491,6 +491,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
   m.sc = c->scal; m.n_iter = 4; m.it = 0;
   m.xout = c->xA; m.pold = c->pA; m.y = c->Ap; m.rout = c->r2; m.invM = c->invM;
   launch_copy(*c, c->r2, c->r, N);
+  launch2_sip(*c, M_PCG_B, m);   // fills rz_1 and beta_0 for the timed PCG_A launches
   k.first = 0; k.it = 1; k.pold = c->pA; k.pnew = c->pB;
   HypArgs h = hyp_base(*c, gamma);
   h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;

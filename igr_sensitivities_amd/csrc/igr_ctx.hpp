@@ -59,7 +59,6 @@ struct Ctx {
   // 1D whole-instance kernel workspace
   double* k1F = nullptr;      // 3N fluxes
   double* k1face = nullptr;   // 6*Ne face values
-  struct K1Args* k1args = nullptr;  // device copy of the launch arguments
 
   // tile shape chosen by the autotuner (2D): elements per workgroup
   int tile_x = 4, tile_y = 4;

@@ -236,8 +236,7 @@ __device__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double
 //  OP_HYP       hyperbolic RHS only (Sigma given) into a.dq
 //  OP_APPLY     y = A(rho) x with rho = state rho
 template <int NP>
-__global__ __launch_bounds__(1024) void k1_run(const K1Args* __restrict__ args) {
-  const K1Args& a = args[blockIdx.x];
+__device__ void k1_body(const K1Args& a) {
   K1<NP> k(a);
   const int N = k.N;
   if (a.op == OP_APPLY) {
@@ -271,12 +270,27 @@ __global__ __launch_bounds__(1024) void k1_run(const K1Args* __restrict__ args) 
   if (a.final_solve && a.alpha > 0.0) k.solve(a.q, 0);
 }
 
-void launch1_run(const Ctx& c, const K1Args* dargs, int n_inst, int block) {
+// Single instance: arguments by value in the kernel-argument segment.
+template <int NP>
+__global__ __launch_bounds__(1024) void k1_run(K1Args a) { k1_body<NP>(a); }
+// Batch: one workgroup per instance, arguments from a device array.
+template <int NP>
+__global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ args) { k1_body<NP>(args[blockIdx.x]); }
+
+void launch1_run(const Ctx& c, const K1Args& a, int block) {
   switch (c.np) {
-    case 2: hipLaunchKernelGGL(k1_run<2>, dim3(n_inst), dim3(block), 0, c.stream, dargs); break;
-    case 3: hipLaunchKernelGGL(k1_run<3>, dim3(n_inst), dim3(block), 0, c.stream, dargs); break;
-    case 4: hipLaunchKernelGGL(k1_run<4>, dim3(n_inst), dim3(block), 0, c.stream, dargs); break;
-    case 5: hipLaunchKernelGGL(k1_run<5>, dim3(n_inst), dim3(block), 0, c.stream, dargs); break;
+    case 2: hipLaunchKernelGGL(k1_run<2>, dim3(1), dim3(block), 0, c.stream, a); break;
+    case 3: hipLaunchKernelGGL(k1_run<3>, dim3(1), dim3(block), 0, c.stream, a); break;
+    case 4: hipLaunchKernelGGL(k1_run<4>, dim3(1), dim3(block), 0, c.stream, a); break;
+    case 5: hipLaunchKernelGGL(k1_run<5>, dim3(1), dim3(block), 0, c.stream, a); break;
+  }
+}
+void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int block) {
+  switch (np) {
+    case 2: hipLaunchKernelGGL(k1_run_batch<2>, dim3(n_inst), dim3(block), 0, st, dargs); break;
+    case 3: hipLaunchKernelGGL(k1_run_batch<3>, dim3(n_inst), dim3(block), 0, st, dargs); break;
+    case 4: hipLaunchKernelGGL(k1_run_batch<4>, dim3(n_inst), dim3(block), 0, st, dargs); break;
+    case 5: hipLaunchKernelGGL(k1_run_batch<5>, dim3(n_inst), dim3(block), 0, st, dargs); break;
   }
 }
 

@@ -1,0 +1,628 @@
+// 2D discrete adjoint kernels (gfx950, FP64): hand-written reverse mode of
+// the forward map, replacing the Enzyme reverse sweep the reference uses
+// (IGR_adjoints_2d/enzyme_adjoint.jl:270-296).
+//
+//   k2_hyp_rev      reverse of the hyperbolic RHS + SSP-RK3 stage update
+//                   (hyperbolic.jl:27-119, euler.jl:145-258, forward.jl:170-188)
+//   k2_sip_rev      reverse of the Sigma solve, one launch per PCG iteration
+//                   (pcg_fixed!, elliptic.jl:329-347): rebuilds the adjoint
+//                   of A p_k on the fly for own and halo nodes, applies A
+//                   (A is symmetric, so A^T = A), accumulates d/d(1/rho) of
+//                   <w, A(rho) v>, updates the adjoint direction and computes
+//                   the four dot products the next reverse iteration needs in
+//                   one grid reduction.  FIN reverses r0 = b - A x0, the
+//                   diagonal (compute_sip_diagonal, :205-264) and the IGR
+//                   source (compute_igr_rhs!, :159-197).
+//   k2_objective_*  objective value (grid reduction) and its gradient
+//                   (enzyme_adjoint.jl:72-76, 108-113, 145-151, 184-190, 223-228)
+// Work decomposition, tiles, halo and XCD-contiguous order as in k2_forward.
+#include "igr_ctx.hpp"
+#include "k2_adjoint_common.hpp"
+
+namespace igr {
+
+// ------------------------------------------------------------------------
+// hyperbolic + stage reverse
+// ------------------------------------------------------------------------
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
+  constexpr int NP2 = NP * NP, T = TX * TY, NH = 2 * (TX + TY), NT = T * NP2;
+  __shared__ double qs[5][T * NP2];
+  __shared__ double db[4][T * NP2];
+  __shared__ double hq[4][NH * NP];
+  __shared__ double hdb[4][NH * NP];
+  __shared__ double red[NT / WAVE + 1];
+
+  const Geo2 g = a.g;
+  const BasisK& B = a.B;
+  const double gam = a.gamma;
+  TileInfo<NP, TX, TY> t(g);
+  const int tid = threadIdx.x;
+  const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
+  const int lx = s % TX, ly = s / TX;
+  const bool valid = lx < t.nvx && ly < t.nvy;
+  const size_t N = (size_t)g.Nex * g.Ney * NP2;
+  const size_t gme = valid ? t.gidx(s, n) : 0;
+
+  double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0}, dbo[4] = {0.0, 0.0, 0.0, 0.0}, qbo[4] = {0.0, 0.0, 0.0, 0.0};
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qo[v] = a.q[v * N + gme];
+      qbo[v] = a.qbar_out[v * N + gme];
+      dbo[v] = a.c_dt * qbo[v];
+    }
+    qo[4] = a.S[gme];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) db[v][tid] = dbo[v];
+  for (int idx = tid; idx < NH * NP; idx += NT) {
+    const int h = idx / NP, k = idx % NP;
+    size_t gi;
+    if (t.halo_face_global(h, k, gi)) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { hq[v][idx] = a.q[v * N + gi]; hdb[v][idx] = a.c_dt * a.qbar_out[v * N + gi]; }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { hq[v][idx] = (v >= 2) ? 1.0 : 0.0; hdb[v][idx] = 0.0; }
+    }
+  }
+  __syncthreads();
+
+  double gb = 0.0;
+  if (valid) {
+    double Fxb[4], Fyb[4], qb[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double sx = 0.0, sy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        sx += B.D[k * NP + i] * db[v][s * NP2 + k + NP * j];
+        sy += B.D[k * NP + j] * db[v][s * NP2 + i + NP * k];
+      }
+      Fxb[v] = -g.invJx * sx;
+      Fyb[v] = -g.invJy * sy;
+    }
+    double lxb = 0.0, lyb = 0.0;
+    const double ir = 1.0 / qo[2];
+    const double Po = (gam - 1.0) * (qo[3] - (qo[0] * qo[0] + qo[1] * qo[1]) / (2.0 * qo[2]));
+    const double co = sqrt(gam * Po * ir);
+    const double wl = 1.0 / B.w[NP - 1], w0 = 1.0 / B.w[0];
+    // neighbour face node (tile LDS or halo) -> q and D-bar
+    auto nb = [&](int sl, int node, int hk, double qn[4], double dn[4]) {
+      if (sl < T) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { qn[v] = qs[v][sl * NP2 + node]; dn[v] = db[v][sl * NP2 + node]; }
+      } else {
+        const int h = sl - T;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { qn[v] = hq[v][h * NP + hk]; dn[v] = hdb[v][h * NP + hk]; }
+      }
+    };
+    auto wave = [&](const double qn[4], int dir) {
+      const double Pn = (gam - 1.0) * (qn[3] - (qn[0] * qn[0] + qn[1] * qn[1]) / (2.0 * qn[2]));
+      return fabs(qn[dir] / qn[2]) + sqrt(gam * Pn / qn[2]);
+    };
+    // x faces
+    if (i == NP - 1 || i == 0) {
+      const bool own_is_L = (i == NP - 1);
+      double qn[4], dn[4];
+      if (own_is_L) nb(t.nb_right(s), NP * j, j, qn, dn);
+      else nb(t.nb_left(s), (NP - 1) + NP * j, j, qn, dn);
+      const double lo = fabs(qo[0] * ir) + co, ln = wave(qn, 0);
+      const double lamL = own_is_L ? lo : ln, lamR = own_is_L ? ln : lo;
+      const double lam = fmax_first(lamL, lamR);
+      double lb = 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double dL = own_is_L ? dbo[v] : dn[v], dR = own_is_L ? dn[v] : dbo[v];
+        const double fb = -g.invJx * wl * dL + g.invJx * w0 * dR;
+        const double qL = own_is_L ? qo[v] : qn[v], qR = own_is_L ? qn[v] : qo[v];
+        lb += fb * (qL - qR) / 2.0;
+        if (own_is_L) { Fxb[v] += g.invJx * wl * dbo[v] + fb / 2.0; qb[v] += lam * fb / 2.0; }
+        else { Fxb[v] += -g.invJx * w0 * dbo[v] + fb / 2.0; qb[v] -= lam * fb / 2.0; }
+      }
+      const bool L_chosen = !(lamR > lamL);
+      if (own_is_L == L_chosen) lxb += lb;
+    }
+    // y faces
+    if (j == NP - 1 || j == 0) {
+      const bool own_is_L = (j == NP - 1);
+      double qn[4], dn[4];
+      if (own_is_L) nb(t.nb_top(s), i, i, qn, dn);
+      else nb(t.nb_bottom(s), i + NP * (NP - 1), i, qn, dn);
+      const double lo = fabs(qo[1] * ir) + co, ln = wave(qn, 1);
+      const double lamL = own_is_L ? lo : ln, lamR = own_is_L ? ln : lo;
+      const double lam = fmax_first(lamL, lamR);
+      double lb = 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double dL = own_is_L ? dbo[v] : dn[v], dR = own_is_L ? dn[v] : dbo[v];
+        const double fb = -g.invJy * wl * dL + g.invJy * w0 * dR;
+        const double qL = own_is_L ? qo[v] : qn[v], qR = own_is_L ? qn[v] : qo[v];
+        lb += fb * (qL - qR) / 2.0;
+        if (own_is_L) { Fyb[v] += g.invJy * wl * dbo[v] + fb / 2.0; qb[v] += lam * fb / 2.0; }
+        else { Fyb[v] += -g.invJy * w0 * dbo[v] + fb / 2.0; qb[v] -= lam * fb / 2.0; }
+      }
+      const bool L_chosen = !(lamR > lamL);
+      if (own_is_L == L_chosen) lyb += lb;
+    }
+    double sb = 0.0;
+    euler_node_rev(gam, qo, Fxb, Fyb, lxb, lyb, qb, sb, gb);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const size_t gi = v * N + gme;
+      const double val = a.c_prev * qbo[v] + qb[v];
+      if (a.accumulate) a.qbar_stage[gi] += val;
+      else a.qbar_stage[gi] = val;
+      if (a.qbar_n) a.qbar_n[gi] += a.c_n * qbo[v];
+    }
+    a.sbar_out[gme] = (a.sbar_in ? a.sbar_in[gme] : 0.0) + sb;
+  }
+  double rv[1] = {gb};
+  block_sum<1>(rv, red);
+  if (tid == 0) a.gbar_blk[blockIdx.x] += rv[0];
+}
+
+// ------------------------------------------------------------------------
+// Solve reverse (PCG) and the standalone SIP VJP
+// ------------------------------------------------------------------------
+template <int NP, int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
+  constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
+  constexpr bool FIN = (MODE == RV_PCG_FIN);
+  __shared__ double ws[NS * NP2];   // adjoint vector w (A is applied to it)
+  __shared__ double vs[NS * NP2];   // vector v of the forward product A v
+  __shared__ double irs[NS * NP2];
+  __shared__ double gx[T * NP2], gy[T * NP2];
+  __shared__ double sxl[T * NP], sxr[T * NP], syb[T * NP], syt[T * NP];
+  __shared__ double mbs[FIN ? NS * NP2 : 1];                 // M-bar
+  __shared__ double us[FIN ? T * NP2 : 1], vvs[FIN ? T * NP2 : 1];
+  __shared__ double Gxx[FIN ? T * NP2 : 1], Gxy[FIN ? T * NP2 : 1], Gyx[FIN ? T * NP2 : 1], Gyy[FIN ? T * NP2 : 1];
+  __shared__ double red[4 * (NT / WAVE + 1)];
+
+  const Geo2 g = a.g;
+  const BasisK& B = a.B;
+  const double al = a.alpha;
+  TileInfo<NP, TX, TY> t(g);
+  const int tid = threadIdx.x;
+  const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
+  const int lx = s % TX, ly = s / TX;
+  const bool valid = lx < t.nvx && ly < t.nvy;
+  const size_t N = (size_t)g.Nex * g.Ney * NP2;
+  const size_t gme = valid ? t.gidx(s, n) : 0;
+  const FS fs(a.n_iter);
+  const int k = a.k;
+
+  // reverse scalars of this iteration
+  double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
+  if constexpr (MODE == RV_PCG_ITER) {
+    ak = a.sc[fs.a(k)];
+    bk = a.sc[fs.beta(k)];
+    rzb = a.rs[RS::rzbar_next(k)];
+    papb = a.rs[RS::papbar(k)];
+  } else if constexpr (FIN) {
+    rzb = a.n_iter > 0 ? a.rs[RS::rzbar0(a.n_iter)] : 0.0;
+  }
+  const bool have_carry = (MODE == RV_PCG_ITER) ? !a.first : (a.n_iter > 0);
+
+  // own-node values kept for the outputs
+  double o_rb = 0.0, o_zb = 0.0, o_R = 0.0, o_pb = 0.0;
+  for (int idx = tid; idx < NS * NP2; idx += NT) {
+    const int sl = idx / NP2, nn = idx % NP2;
+    double w = 0.0, v = 0.0, ir = 0.0, mb = 0.0;
+    size_t gi;
+    if (t.slot_global(sl, nn, gi)) {
+      if constexpr (MODE == RV_VJP) {
+        w = a.wbar[gi]; v = a.v[gi]; ir = a.invR[gi];
+      } else {
+        ir = a.invR[gi];
+        const double iM = a.invM[gi];
+        const double rc = have_carry ? a.rbar_in[gi] : 0.0;
+        const double pb = have_carry ? a.pbar_in[gi] : 0.0;
+        const int kr = (MODE == RV_PCG_ITER) ? k + 1 : 0;
+        const double Rk = a.R[(size_t)kr * N + gi];
+        const double z = Rk * iM;
+        const double zb = pb + rzb * Rk;
+        const double rb = rc + rzb * z + zb * iM;
+        if constexpr (MODE == RV_PCG_ITER) {
+          const double pk = a.P[(size_t)k * N + gi];
+          w = -ak * rb + papb * pk;
+          v = pk;
+        } else {
+          w = -rb;
+          v = a.x0[gi];
+          mb = -(a.imbar[gi] + zb * Rk) * iM * iM;
+          if (sl < T) { us[idx] = a.q[gi] / a.q[2 * N + gi]; vvs[idx] = a.q[N + gi] / a.q[2 * N + gi]; }
+        }
+        if (idx == tid) { o_rb = rb; o_zb = zb; o_R = Rk; o_pb = pb; }
+      }
+    }
+    ws[idx] = w; vs[idx] = v; irs[idx] = ir;
+    if constexpr (FIN) mbs[idx] = mb;
+  }
+  __syncthreads();
+
+  const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
+  const double wown = ws[s * NP2 + n], vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
+  double fc = 0.0, irb = 0.0;
+  double bown = 0.0;
+  if (valid) {
+    double dxw = 0.0, dyw = 0.0, dxv = 0.0, dyv = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < NP; ++kk) {
+      dxw += B.D[i * NP + kk] * ws[s * NP2 + kk + NP * j];
+      dyw += B.D[j * NP + kk] * ws[s * NP2 + i + NP * kk];
+      dxv += B.D[i * NP + kk] * vs[s * NP2 + kk + NP * j];
+      dyv += B.D[j * NP + kk] * vs[s * NP2 + i + NP * kk];
+    }
+    const double kx = al * B.w[i] * B.w[j] * g.Jy * g.invJx;
+    const double ky = al * B.w[i] * B.w[j] * g.Jx * g.invJy;
+    const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+    gx[s * NP2 + n] = kx * ir * dxw;
+    gy[s * NP2 + n] = ky * ir * dyw;
+    irb = cm * vown * wown + kx * dxv * dxw + ky * dyv * dyw;
+    // one face: own node is L (own_L) or R; neighbour slot sn, neighbour node nn,
+    // normal-derivative row of the neighbour along direction dir
+    auto face = [&](bool own_L, int sn, int dir, double invJ, double h, double wf, double* symf) {
+      const int nnode = (dir == 0) ? (own_L ? NP * j : (NP - 1) + NP * j)
+                                   : (own_L ? i : i + NP * (NP - 1));
+      const int drow = own_L ? 0 : NP - 1;        // derivative row of the neighbour's face node
+      double dnw = 0.0, dnv = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) {
+        const int ln = (dir == 0) ? kk + NP * j : i + NP * kk;
+        dnw += B.D[drow * NP + kk] * ws[sn * NP2 + ln];
+        dnv += B.D[drow * NP + kk] * vs[sn * NP2 + ln];
+      }
+      const double irn = irs[sn * NP2 + nnode];
+      const double wn = ws[sn * NP2 + nnode], vn = vs[sn * NP2 + nnode];
+      const double dow = (dir == 0) ? dxw : dyw, dov = (dir == 0) ? dxv : dyv;
+      const double irL = own_L ? ir : irn, irR = own_L ? irn : ir;
+      const double jw = own_L ? (wown - wn) : (wn - wown);
+      const double jv = own_L ? (vown - vn) : (vn - vown);
+      const double dLw = own_L ? dow : dnw, dRw = own_L ? dnw : dow;
+      const double dLv = own_L ? dov : dnv, dRv = own_L ? dnv : dov;
+      const double sig = g.eta * fmax(irL, irR) / h;
+      // apply A to w: this node's consistency + penalty, and the symmetry factor
+      const double avg = invJ * (irL * dLw + irR * dRw) / 2.0;
+      if (own_L) { fc += -al * avg * wf; fc += al * sig * jw * wf; }
+      else { fc += al * avg * wf; fc -= al * sig * jw * wf; }
+      *symf = -al * (ir / 2.0) * invJ * jw * wf;
+      // d/d(1/rho_own) of the face's bilinear form in (w, v)
+      const bool L_chosen = !(irR > irL);
+      const double dv_own = own_L ? dLv : dRv, dw_own = own_L ? dLw : dRw;
+      irb += -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
+      if (own_L == L_chosen) irb += al * g.eta / h * jv * wf * jw;
+    };
+    if (i == 0) face(false, sL, 0, g.invJx, g.dx, B.w[j] * g.Jy, &sxl[s * NP + j]);
+    if (i == NP - 1) face(true, sR, 0, g.invJx, g.dx, B.w[j] * g.Jy, &sxr[s * NP + j]);
+    if (j == 0) face(false, sB, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syb[s * NP + i]);
+    if (j == NP - 1) face(true, sT, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syt[s * NP + i]);
+    if constexpr (FIN) {
+      // IGR source b = c alpha [(uxx+uyy)^2 + uxx^2 + 2 uxy uyx + uyy^2], b-bar = r0-bar = -w
+      bown = -wown;
+      double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) {
+        uxx += B.D[i * NP + kk] * us[s * NP2 + kk + NP * j];
+        uyx += B.D[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
+        uxy += B.D[j * NP + kk] * us[s * NP2 + i + NP * kk];
+        uyy += B.D[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
+      }
+      uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
+      const double f = bown * cm * al;
+      Gxx[s * NP2 + n] = f * (2.0 * (uxx + uyy) + 2.0 * uxx);
+      Gyy[s * NP2 + n] = f * (2.0 * (uxx + uyy) + 2.0 * uyy);
+      Gxy[s * NP2 + n] = f * 2.0 * uyx;
+      Gyx[s * NP2 + n] = f * 2.0 * uxy;
+    }
+  }
+  if (!valid) {
+    if (i == 0) sxl[s * NP + j] = 0.0;
+    if (i == NP - 1) sxr[s * NP + j] = 0.0;
+    if (j == 0) syb[s * NP + i] = 0.0;
+    if (j == NP - 1) syt[s * NP + i] = 0.0;
+  }
+  __syncthreads();
+
+  double y = 0.0, ab = 0.0;
+  if (valid) {
+    const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+    y = cm * wown * ir;
+    double st = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < NP; ++kk) {
+      st += B.D[kk * NP + i] * gx[s * NP2 + kk + NP * j];
+      st += B.D[kk * NP + j] * gy[s * NP2 + i + NP * kk];
+    }
+    y += st;
+    y += sxl[s * NP + j] * B.D[i] + sxr[s * NP + j] * B.D[(NP - 1) * NP + i];
+    y += syb[s * NP + i] * B.D[j] + syt[s * NP + i] * B.D[(NP - 1) * NP + j];
+    y += fc;
+    // d/d alpha of <w, A v> = <v, K w>, K = (A - mass)/alpha
+    if (al != 0.0) ab = vown * (y - cm * wown * ir) / al;
+  }
+
+  if constexpr (MODE == RV_VJP) {
+    if (valid) {
+      a.y[gme] = y;
+      a.irbar[gme] = -ir * ir * irb;     // adjoint of rho
+    }
+  } else if constexpr (MODE == RV_PCG_ITER) {
+    double dv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+      a.irbar[gme] += irb;
+      a.imbar[gme] += o_zb * o_R;
+      a.rbar_out[gme] = o_rb;
+      const double xb = a.xbar[gme];
+      const double pbk = bk * o_pb + ak * xb + papb * a.AP[(size_t)k * N + gme] + y;
+      a.pbar_out[gme] = pbk;
+      if (k >= 1) {
+        const double pm = a.P[(size_t)(k - 1) * N + gme], apm = a.AP[(size_t)(k - 1) * N + gme];
+        const double iM = a.invM[gme];
+        dv[0] = pbk * pm;
+        dv[1] = xb * pm;
+        dv[2] = (o_rb + pbk * iM) * apm;
+        dv[3] = (o_R * iM) * apm;
+      }
+    }
+    double rv[1] = {ab};
+    block_sum<1>(rv, red);
+    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
+    double* rs = a.rs;
+    const double* sc = a.sc;
+    const int nit = a.n_iter;
+    grid_sum<4>(dv, red, a.partials, a.counter, gridDim.x, blockIdx.x, [rs, sc, nit, k](double* tot) {
+      const FS f(nit);
+      const double eps = 1e-30;
+      const double rzk = sc[f.rz(k)], rzk1 = sc[f.rz(k + 1)], papk = sc[f.pap(k)];
+      const double carry = -rs[RS::bbar(k)] * rzk1 / ((rzk + eps) * (rzk + eps)) + rs[RS::abar(k)] / (papk + eps);
+      if (k == 0) {
+        rs[RS::rzbar0(nit)] = carry;
+      } else {
+        const int km = k - 1;
+        const double rzm = sc[f.rz(km)], papm = sc[f.pap(km)];
+        const double bb = tot[0];
+        const double rzt = carry + bb / (rzm + eps);
+        const double abm = tot[1] - tot[2] - 2.0 * rzt * tot[3];
+        rs[RS::bbar(km)] = bb;
+        rs[RS::rzbar_next(km)] = rzt;
+        rs[RS::abar(km)] = abm;
+        rs[RS::papbar(km)] = -abm * rzm / ((papm + eps) * (papm + eps));
+      }
+    });
+  } else if constexpr (FIN) {
+    double ab_extra = 0.0;
+    if (valid) {
+      a.x0bar[gme] = a.xbar[gme] + y;    // x0-bar = x-bar - A r0-bar, w = -r0-bar
+      const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+      // diag(A) reverse: M = cm/rho + x/y stiffness self-coupling + face terms
+      const double iM = a.invM[gme];
+      const double Mown = 1.0 / iM;
+      const double mbo = mbs[s * NP2 + n];
+      ab_extra += (al != 0.0) ? mbo * (Mown - cm * ir) / al : 0.0;
+      double irbd = cm * mbo;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) {
+        // x-stiffness of node (kk, j) couples to this node (i, j): alpha w_i w_j Jy/Jx D[i][kk]^2
+        const double Dx = B.D[i * NP + kk];
+        irbd += al * B.w[i] * B.w[j] * g.Jy * g.invJx * Dx * Dx * mbs[s * NP2 + kk + NP * j];
+        const double Dy = B.D[j * NP + kk];
+        irbd += al * B.w[i] * B.w[j] * g.Jx * g.invJy * Dy * Dy * mbs[s * NP2 + i + NP * kk];
+      }
+      auto dface = [&](bool own_L, int sn, int nnode, double invJ, double h, double wf, double Dself) {
+        const double irn = irs[sn * NP2 + nnode], mbn = mbs[sn * NP2 + nnode];
+        const double irL = own_L ? ir : irn, irR = own_L ? irn : ir;
+        irbd += (own_L ? -1.0 : 1.0) * al * invJ * Dself * wf * mbo;
+        const bool L_chosen = !(irR > irL);
+        if (own_L == L_chosen) irbd += (mbo + mbn) * al * wf * g.eta / h;
+      };
+      if (i == NP - 1) dface(true, sR, NP * j, g.invJx, g.dx, B.w[j] * g.Jy, B.D[(NP - 1) * NP + NP - 1]);
+      if (i == 0) dface(false, sL, (NP - 1) + NP * j, g.invJx, g.dx, B.w[j] * g.Jy, B.D[0]);
+      if (j == NP - 1) dface(true, sT, i, g.invJy, g.dy, B.w[i] * g.Jx, B.D[(NP - 1) * NP + NP - 1]);
+      if (j == 0) dface(false, sB, i + NP * (NP - 1), g.invJy, g.dy, B.w[i] * g.Jx, B.D[0]);
+      // IGR source reverse: u-bar, v-bar gathered from G
+      double ub = 0.0, vb = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) {
+        ub += g.invJx * B.D[kk * NP + i] * Gxx[s * NP2 + kk + NP * j] + g.invJy * B.D[kk * NP + j] * Gxy[s * NP2 + i + NP * kk];
+        vb += g.invJx * B.D[kk * NP + i] * Gyx[s * NP2 + kk + NP * j] + g.invJy * B.D[kk * NP + j] * Gyy[s * NP2 + i + NP * kk];
+      }
+      const double rho = a.q[2 * N + gme];
+      const double u = us[s * NP2 + n], v = vvs[s * NP2 + n];
+      // b = cm alpha F(u): d/d alpha = b/alpha; recompute b from the G terms: b-bar * b / alpha
+      {
+        double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < NP; ++kk) {
+          uxx += B.D[i * NP + kk] * us[s * NP2 + kk + NP * j];
+          uyx += B.D[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
+          uxy += B.D[j * NP + kk] * us[s * NP2 + i + NP * kk];
+          uyy += B.D[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
+        }
+        uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
+        const double dvg = uxx + uyy;
+        ab_extra += bown * cm * (dvg * dvg + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
+      }
+      const double irt = a.irbar[gme] + irb + irbd;
+      a.qbar[gme] += ub / rho;
+      a.qbar[N + gme] += vb / rho;
+      a.qbar[2 * N + gme] += -ir * ir * irt - (ub * u + vb * v) / rho;
+    }
+    double rv[1] = {ab + ab_extra};
+    block_sum<1>(rv, red);
+    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
+  }
+}
+
+// First reverse PCG step: a-bar_{n-1} = <x-bar, p_{n-1}> (the later adjoints are 0)
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_rev_start(RevArgs a) {
+  constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
+  __shared__ double red[NT / WAVE + 1];
+  TileInfo<NP, TX, TY> t(a.g);
+  const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
+  const int lx = s % TX, ly = s / TX;
+  const size_t N = (size_t)a.g.Nex * a.g.Ney * NP2;
+  const int nit = a.n_iter;
+  double dv[1] = {0.0};
+  if (lx < t.nvx && ly < t.nvy) {
+    const size_t gi = t.gidx(s, n);
+    dv[0] = a.xbar[gi] * a.P[(size_t)(nit - 1) * N + gi];
+  }
+  double* rs = a.rs;
+  const double* sc = a.sc;
+  grid_sum<1>(dv, red, a.partials, a.counter, gridDim.x, blockIdx.x, [rs, sc, nit](double* tot) {
+    const FS f(nit);
+    const int k = nit - 1;
+    const double eps = 1e-30, papk = sc[f.pap(k)];
+    rs[RS::bbar(k)] = 0.0;
+    rs[RS::rzbar_next(k)] = 0.0;
+    rs[RS::abar(k)] = tot[0];
+    rs[RS::papbar(k)] = -tot[0] * sc[f.rz(k)] / ((papk + eps) * (papk + eps));
+  });
+}
+
+// ------------------------------------------------------------------------
+// objectives: value (NV partial sums) and gradient
+// ------------------------------------------------------------------------
+
+template <int NP, int TX, int TY, bool GRAD>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_objective(ObjArgs a) {
+  constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
+  __shared__ double red[2 * (NT / WAVE + 1)];
+  TileInfo<NP, TX, TY> t(a.g);
+  const Geo2& g = a.g;
+  const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
+  const int lx = s % TX, ly = s / TX;
+  const size_t N = (size_t)g.Nex * g.Ney * NP2;
+  double acc[2] = {0.0, 0.0};
+  if (lx < t.nvx && ly < t.nvy) {
+    const size_t gi = t.gidx(s, n);
+    const double wJ = a.B.w[i] * a.B.w[j] * g.Jx * g.Jy;
+    const double x = (t.ex0 + lx) * g.dx + g.Jx * (1.0 + a.B.xi[i]);
+    const double y = (t.ey0 + ly) * g.dy + g.Jy * (1.0 + a.B.xi[j]);
+    const double mx = a.q[gi], my = a.q[N + gi], r = a.q[2 * N + gi], E = a.q[3 * N + gi];
+    const double ke = (mx * mx + my * my) / (2.0 * r);
+    double wt = 1.0;
+    const double dx_ = x - a.prm[0], dy_ = y - a.prm[1];
+    if (a.obj == 3 || a.obj == 5) wt = exp(-(dx_ * dx_ + dy_ * dy_) / (a.prm[2] * a.prm[2]));
+    if (a.obj == 6) wt = (dx_ * dx_ + dy_ * dy_ < a.prm[2] * a.prm[2]) ? 1.0 : 0.0;
+    const double gg = sin(2.0 * M_PI * x / a.Lx) * sin(2.0 * M_PI * y / a.Ly);
+    const double P = (a.gamma - 1.0) * (E - ke);
+    if (!GRAD) {
+      switch (a.obj) {
+        case 0: acc[0] = wJ * (r * r); break;
+        case 1: acc[0] = wJ * (P * P); break;
+        case 2: acc[0] = wJ * ke; acc[1] = wJ * r; break;
+        case 3: case 6: acc[0] = wJ * wt * ke; break;
+        case 4: acc[0] = wJ * mx * gg; break;
+        case 5: acc[0] = wJ * r * wt; break;
+      }
+    } else {
+      double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+      switch (a.obj) {
+        case 0: b2 = 2.0 * wJ * r; break;
+        case 1: {
+          const double Pb = 2.0 * wJ * P;
+          b3 = (a.gamma - 1.0) * Pb;
+          b0 = -(a.gamma - 1.0) * Pb * mx / r;
+          b1 = -(a.gamma - 1.0) * Pb * my / r;
+          b2 = (a.gamma - 1.0) * Pb * ke / r;
+          acc[0] = Pb * (E - ke);
+          break;
+        }
+        case 2: {
+          const double KE = a.out[1], M = a.out[2];
+          b0 = wJ * mx / r / M;
+          b1 = wJ * my / r / M;
+          b2 = -wJ * ke / r / M - KE * wJ / (M * M);
+          break;
+        }
+        case 3: case 6:
+          b0 = wJ * wt * mx / r; b1 = wJ * wt * my / r; b2 = -wJ * wt * ke / r; break;
+        case 4: b0 = wJ * gg; break;
+        case 5: b2 = wJ * wt; break;
+      }
+      a.qbar[gi] = b0; a.qbar[N + gi] = b1; a.qbar[2 * N + gi] = b2; a.qbar[3 * N + gi] = b3;
+    }
+  }
+  double* out = a.out;
+  double* gbar = a.gbar;
+  const int obj = a.obj;
+  if (!GRAD) {
+    grid_sum<2>(acc, red, a.partials, a.counter, gridDim.x, blockIdx.x, [out, obj](double* tot) {
+      out[1] = tot[0];
+      out[2] = tot[1];
+      out[0] = (obj == 2) ? tot[0] / tot[1] : tot[0];
+    });
+  } else {
+    double one[1] = {acc[0]};
+    grid_sum<1>(one, red, a.partials, a.counter, gridDim.x, blockIdx.x, [gbar](double* tot) { gbar[0] += tot[0]; });
+  }
+}
+
+// Sum of per-workgroup accumulators into out[0] (single workgroup, fixed order).
+__global__ void k_sum_blocks(const double* blk, int nb, double* out) {
+  __shared__ double red[16];
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += blk[b];
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) out[0] += v[0];
+}
+__global__ void k_axpy(double* y, const double* x, double c, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] += c * x[k];
+}
+
+// ------------------------------------------------------------------------
+template <int NP, int TX, int TY>
+struct LaunchRev {
+  static dim3 grid(const Geo2& g) {
+    return dim3(((g.Nex + TX - 1) / TX) * ((g.Ney + TY - 1) / TY));
+  }
+  static dim3 block() { return dim3(TX * TY * NP * NP); }
+  static void hyp(const HypRevArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k2_hyp_rev<NP, TX, TY>), grid(a.g), block(), 0, st, a);
+  }
+  static void rev(int mode, const RevArgs& a, hipStream_t st) {
+    dim3 G = grid(a.g), Bk = block();
+    switch (mode) {
+      case RV_VJP: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_VJP>), G, Bk, 0, st, a); break;
+      case RV_PCG_START: hipLaunchKernelGGL((k2_pcg_rev_start<NP, TX, TY>), G, Bk, 0, st, a); break;
+      case RV_PCG_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_ITER>), G, Bk, 0, st, a); break;
+      case RV_PCG_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_FIN>), G, Bk, 0, st, a); break;
+    }
+  }
+  static void obj(const ObjArgs& a, bool grad, hipStream_t st) {
+    if (grad) hipLaunchKernelGGL((k2_objective<NP, TX, TY, true>), grid(a.g), block(), 0, st, a);
+    else hipLaunchKernelGGL((k2_objective<NP, TX, TY, false>), grid(a.g), block(), 0, st, a);
+  }
+};
+
+#define IGR_DISPATCH_TILE(c, CALL)                                              \
+  switch ((c).np) {                                                             \
+    case 2: LaunchRev<2, 8, 8>::CALL; break;                                    \
+    case 3: LaunchRev<3, 4, 4>::CALL; break;                                    \
+    case 4:                                                                     \
+      if ((c).tile_x == 8 && (c).tile_y == 4) LaunchRev<4, 8, 4>::CALL;        \
+      else if ((c).tile_x == 2 && (c).tile_y == 2) LaunchRev<4, 2, 2>::CALL;   \
+      else LaunchRev<4, 4, 4>::CALL;                                            \
+      break;                                                                    \
+    case 5: LaunchRev<5, 2, 4>::CALL; break;                                    \
+  }
+
+void launch2_hyp_rev(const Ctx& c, const HypRevArgs& a) { IGR_DISPATCH_TILE(c, hyp(a, c.stream)); }
+void launch2_rev(const Ctx& c, int mode, const RevArgs& a) { IGR_DISPATCH_TILE(c, rev(mode, a, c.stream)); }
+void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad) { IGR_DISPATCH_TILE(c, obj(a, grad, c.stream)); }
+void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out) {
+  hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(1024), 0, c.stream, blk, nb, out);
+}
+void launch_axpy(const Ctx& c, double* y, const double* x, double cf, size_t n) {
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, y, x, cf, n);
+}
+
+}  // namespace igr

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
     const size_t gi = t.gidx(s, n);
     const double ak = sc[2 * nit + 1 + it];
     a.xout[gi] = a.xout[gi] + ak * a.pold[gi];
-    const double rn = a.rout[gi] - ak * a.y[gi];
+    const double rn = (a.r ? a.r[gi] : a.rout[gi]) - ak * a.y[gi];
     a.rout[gi] = rn;
     red_v[0] = rn * (rn * a.invM[gi]);
   }

@@ -97,6 +97,47 @@ int igr_run_forward(igr_ctx* ctx, double dt, double t_end, double gamma, double 
 int igr_run_steps(igr_ctx* ctx, int n_steps, double dt, double gamma, double alpha, int n_iter,
                   int solver);
 
+/* Objectives: replace the `objective::Symbol` of enzyme_adjoint_ic
+ * (IGR_adjoints_2d/enzyme_adjoint.jl:240-254).  0..4 are the reference's
+ * :l2_density, :l2_pressure, :kinetic_energy (KE / mass),
+ * :windowed_kinetic_energy (window prm = (xc, yc, sigma); NULL prm =
+ * KE_WINDOW (0.55, 0.50, 0.15), enzyme_adjoint.jl:26) and
+ * :weighted_momentum; 5 = int rho exp(-|x-c|^2/sigma^2) and 6 = kinetic
+ * energy inside the disc |x-c| < R (prm = (xc, yc, R)) have no counterpart
+ * in the reference. */
+enum igr_objective {
+  IGR_OBJ_L2_DENSITY = 0, IGR_OBJ_L2_PRESSURE = 1, IGR_OBJ_KINETIC_ENERGY = 2,
+  IGR_OBJ_WINDOWED_KE = 3, IGR_OBJ_WEIGHTED_MOMENTUM = 4, IGR_OBJ_WINDOWED_DENSITY = 5,
+  IGR_OBJ_DISC_KE = 6
+};
+/* Objective value on the context state (2D). */
+int igr_objective(igr_ctx* ctx, int objective, const double* prm, double gamma, double* J);
+
+/* Discrete adjoint (2D): enzyme_adjoint_ic(q0..., n_steps, dt, basis, mesh,
+ * gamma, alpha; n_iter, solver, objective) (IGR_adjoints_2d/enzyme_adjoint.jl:
+ * 270-296) extended to gamma and alpha.  q0 is the context state; the map is
+ * the Enzyme objectives' loop (Sigma <- 0, n_steps fixed steps of dt).
+ * Returns J, dJ/dq0 (4N, conservative variables; may be NULL), dJ/dgamma
+ * and dJ/dalpha.  ckpt_slots: number of (q, Sigma) checkpoints kept in HBM
+ * (<= 0: as many as half the free HBM allows, up to n_steps); steps between
+ * checkpoints are recomputed (optimal binomial schedule).  On return the
+ * context state holds q(T).  Sigma solver: PCG (any solver when alpha <= 0). */
+int igr_gradient(igr_ctx* ctx, int objective, const double* prm, int n_steps, double dt, double gamma,
+                 double alpha, int n_iter, int solver, int ckpt_slots, double* J, double* qbar0,
+                 double* gammabar, double* alphabar);
+
+/* Reverse-mode building blocks (2D), each the exact VJP of the forward kernel:
+ * compute_hyperbolic_rhs! (hyperbolic.jl:27): dq-bar -> q-bar (4N), Sigma-bar, gamma-bar. */
+int igr_hyperbolic_vjp(igr_ctx* ctx, double gamma, const double* q, const double* sigma, const double* dqbar,
+                       double* qbar, double* sigmabar, double* gammabar);
+/* apply_sip! (elliptic.jl:30): y-bar -> x-bar = A^T y-bar, rho-bar, alpha-bar. */
+int igr_apply_sip_vjp(igr_ctx* ctx, double alpha, const double* x, const double* rho, const double* ybar,
+                      double* xbar, double* rhobar, double* alphabar);
+/* solve_elliptic! with PCG (elliptic.jl:413) on the context state q and warm
+ * start sigma0: Sigma-bar -> q-bar (4N), sigma0-bar, alpha-bar. */
+int igr_solve_elliptic_vjp(igr_ctx* ctx, double alpha, int n_iter, int solver, const double* sigma0,
+                           const double* sigmabar, double* qbar, double* sigma0bar, double* alphabar);
+
 /* Mean device time (ms) per launch of one hot kernel on the current state,
  * measured with HIP events around `reps` back-to-back launches on the
  * context stream.  kernel: 0 = fused SIP-apply + PCG direction update,

@@ -1,0 +1,151 @@
+"""GPU discrete adjoint against the oracle's exact discrete JVP (dual
+numbers through the restated reference map): the dot-product identity
+<v, J u> = <J^T v, u> for random u, v, component by component and for the
+whole gradient, plus central finite differences of the GPU forward map.
+This stands in for the reference's Enzyme checks (Enzyme == ForwardDiff to
+1e-8 in IGR_adjoints_1d/tests/run_enzyme_test.jl:87; vs FD to 1e-3 in
+IGR_adjoints_2d/tests/run_enzyme_test.jl:88)."""
+import numpy as np
+import pytest
+
+import igr_sensitivities_amd as igr
+from igr_sensitivities_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def ctx2(p, nx, ny, lx=1.0, ly=1.0):
+    print(f"[case] 2D p={p} {nx}x{ny}", flush=True)
+    return igr.Context(igr.CartesianMesh2D(nx, ny, lx, ly, igr.DGBasis(p)))
+
+
+def close(a, b, tol):
+    assert abs(a - b) <= tol * max(abs(a), abs(b), 1e-300), (a, b, abs(a - b) / max(abs(a), abs(b)))
+
+
+@pytest.mark.parametrize("nx,ny", [(6, 5), (8, 8), (3, 1)])
+def test_hyperbolic_vjp_dot_product(orc, nx, ny):
+    c = ctx2(3, nx, ny, 1.0, 1.2)
+    g = orc.Grid2D(3, nx, ny, 1.0, 1.2)
+    q = configs.smooth_random2(3, nx, ny, seed=21, amp=0.3)
+    rng = np.random.default_rng(22)
+    S = 0.01 * rng.standard_normal(g.shape)
+    u = rng.standard_normal(q.shape)
+    uS = rng.standard_normal(g.shape)
+    ug = 0.7
+    v = rng.standard_normal(q.shape)
+    _, Ju = g.jvp_hyperbolic_rhs(1.4, ug, q, u, S, uS)
+    qb, sb, gb = c.hyperbolic_vjp(1.4, q, S, v)
+    close(np.vdot(v, Ju), np.vdot(qb, u) + np.vdot(sb, uS) + gb * ug, 1e-12)
+
+
+@pytest.mark.parametrize("nx,ny", [(6, 5), (8, 8), (2, 3)])
+def test_apply_sip_vjp_dot_product(orc, nx, ny):
+    c = ctx2(3, nx, ny, 1.0, 0.8)
+    g = orc.Grid2D(3, nx, ny, 1.0, 0.8)
+    rng = np.random.default_rng(nx + 10 * ny)
+    rho = 1.0 + 0.5 * rng.random(g.shape)
+    x = rng.standard_normal(g.shape)
+    dx, drho, da = rng.standard_normal(g.shape), rng.standard_normal(g.shape), 0.3
+    yb = rng.standard_normal(g.shape)
+    a = 4e-3
+    _, Ju = g.jvp_apply_sip(a, da, x, dx, rho, drho)
+    xb, rb, ab = c.apply_sip_vjp(a, x, rho, yb)
+    close(np.vdot(yb, Ju), np.vdot(xb, dx) + np.vdot(rb, drho) + ab * da, 1e-12)
+
+
+@pytest.mark.parametrize("n_iter", [0, 1, 5, 30])
+def test_solve_vjp_dot_product(orc, n_iter):
+    c = ctx2(3, 6, 5)
+    g = orc.Grid2D(3, 6, 5)
+    rng = np.random.default_rng(30 + n_iter)
+    q = configs.smooth_random2(3, 6, 5, seed=31, amp=0.3)
+    S0 = 0.01 * rng.standard_normal(g.shape)
+    dq, dS0, da = rng.standard_normal(q.shape), rng.standard_normal(g.shape), 0.2
+    Sb = rng.standard_normal(g.shape)
+    a = 5e-3
+    _, dS = g.jvp_solve_elliptic(a, da * a, n_iter, orc.PCG, q, dq, S0, dS0)
+    c.set_state(q)
+    qb, s0b, ab = c.solve_elliptic_vjp(a, n_iter, "pcg", S0, Sb)
+    close(np.vdot(Sb, dS), np.vdot(qb, dq) + np.vdot(s0b, dS0) + ab * da * a, 1e-10)
+
+
+OBJS = [(0, None), (1, None), (2, None), (3, (0.55, 0.5, 0.15, 0.0)), (4, None), (5, (0.4, 0.6, 0.1, 0.0)),
+        (6, (0.5, 0.45, 0.3, 0.0))]
+
+
+@pytest.mark.parametrize("obj,prm", OBJS)
+def test_gradient_dot_product(orc, obj, prm):
+    nx, ny, ns = 6, 5, 3
+    c = ctx2(3, nx, ny)
+    g = orc.Grid2D(3, nx, ny)
+    q0 = configs.smooth_random2(3, nx, ny, seed=40 + obj, amp=0.3)
+    gam, a = 1.4, 4e-3
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, gam))
+    p4 = prm if prm is not None else (0.0, 0.0, 1.0, 0.0)
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(obj, ns, dt, gam, a, 20, "pcg", prm=p4)
+    rng = np.random.default_rng(50 + obj)
+    for trial in range(2):
+        u = rng.standard_normal(q0.shape)
+        ug, ua = rng.standard_normal(), rng.standard_normal()
+        Jr, dJ, _, _ = g.jvp_objective_run(gam, ug, a, ua * a, 20, orc.PCG, ns, dt, obj, q0, u, p4)
+        close(J, Jr, 1e-12)
+        close(dJ, np.vdot(qb, u) + gb * ug + ab * ua * a, 1e-10)
+
+
+def test_gradient_alpha_zero_and_deterministic(orc):
+    nx, ny, ns = 5, 4, 4
+    q0 = configs.smooth_random2(3, nx, ny, seed=60, amp=0.3)
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
+    c = ctx2(3, nx, ny)
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(1, ns, dt, 1.4, 0.0, 1, "jacobi")
+    g = orc.Grid2D(3, nx, ny)
+    u = np.random.default_rng(61).standard_normal(q0.shape)
+    Jr, dJ, _, _ = g.jvp_objective_run(1.4, 0.5, 0.0, 0.0, 1, orc.JACOBI, ns, dt, 1, q0, u)
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.5 * gb, 1e-10)
+    assert ab == 0.0
+
+
+def test_checkpoint_schedules_bitwise_equal():
+    nx, ny, ns = 6, 6, 9
+    q0 = configs.smooth_random2(3, nx, ny, seed=70, amp=0.3)
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
+    res = []
+    for slots in (2, 3, 5, 10):
+        c = ctx2(3, nx, ny)
+        c.set_state(q0)
+        res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 15, "pcg", prm=(0.55, 0.5, 0.15, 0.0), ckpt_slots=slots))
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        np.testing.assert_array_equal(r[1], res[0][1])
+        assert r[2] == res[0][2] and r[3] == res[0][3]
+
+
+def test_gradient_c3_32_vs_oracle_and_fd(orc):
+    # the C3 gradient check mesh: 32x32 triple deposit, windowed-density objective, PCG-60
+    cfg = configs.config2(N=32, seed=2)
+    prm = configs.config3_objective()
+    ns = 5
+    q0, gam, a, dt = cfg["q"], cfg["gamma"], cfg["alpha"], cfg["dt"]
+    c = ctx2(3, 32, 32)
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(5, ns, dt, gam, a, 60, "pcg", prm=prm)
+    g = orc.Grid2D(3, 32, 32)
+    rng = np.random.default_rng(80)
+    u = rng.standard_normal(q0.shape) * (q0.std(axis=(1, 2, 3, 4), keepdims=True) + 1e-2)
+    Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.1, a, 0.2 * a, 60, orc.PCG, ns, dt, 5, q0, u, prm)
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.1 * gb + 0.2 * a * ab, 1e-9)
+    # central FD on the GPU forward map in the gamma and alpha directions
+    def Jat(gg, aa):
+        c.set_state(q0)
+        c.run_steps(ns, dt, gg, aa, 60, "pcg")
+        return c.objective(5, gg, prm)
+    h = 1e-6
+    fd_g = (Jat(gam * (1 + h), a) - Jat(gam * (1 - h), a)) / (2 * h * gam)
+    fd_a = (Jat(gam, a * (1 + h)) - Jat(gam, a * (1 - h))) / (2 * h * a)
+    close(fd_g, gb, 1e-6)
+    close(fd_a, ab, 1e-5)
