@@ -37,6 +37,7 @@ PROTOTYPES = {
     "igr_run_steps": (_i, [_vp, _i, _d, _d, _d, _i, _i]),
     "igr_time_kernel": (_i, [_vp, _i, _d, _d, _i, _pd]),
     "igr_objective": (_i, [_vp, _i, _pd, _d, _pd]),
+    "igr_run_sensitivity": (_i, [_vp, _i, _pd, _d, _d, _d, _d, _i, _i, _i, _d, _pd, _pd, _pi]),
     "igr_gradient": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd]),
     "igr_hyperbolic_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
     "igr_apply_sip_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),

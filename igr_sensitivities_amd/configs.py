@@ -5,10 +5,11 @@ IGR_adjoints_2d/initial_conditions.jl and its paper scripts; the workloads
 here are the ones this library is measured on:
 
   C1  1D forward: periodic [0,1], N_e=128, p=3, gamma=1.4, rho=1, u=0,
-      pressure 1000 / 0.01 / 100 on [0,0.1) / [0.1,0.9) / [0.9,1] times
-      (1 + 1e-3 eta(x)) with eta a seeded sum of 8 periodic sine modes
-      (max|eta| = 1), alpha = 10 (h/4)^2, CFL 0.1 to t = 0.038, Jacobi-50.
-  C2  2D forward: [0,1]^2, 128x128, p=3, rho=1, u=v=0, p_bg=1e-5 plus three
+      pressure 1000 / 0.01 / 100 on [0,0.1) / [0.1,0.9) / [0.9,1] with tanh
+      edges of width 0.01, times (1 + 1e-3 eta(x)) with eta a seeded sum of 8
+      periodic sine modes (max|eta| = 1), alpha = 10 h^2, CFL 0.1 to
+      t = 0.038, Jacobi-50.
+  C2  2D forward: [0,1]^2, 128x128, p=3, rho=1, u=v=0, p_bg=1 plus three
       Gaussian energy deposits (sigma=0.02, total E = 1, seeded centres),
       alpha = 10 (h/4)^2, PCG-60, 200 SSP-RK3 steps, CFL 0.4.
   C3  C2 with J = sum w w Jx Jy rho(T) exp(-|x - c_w|^2 / 0.1^2), c_w seeded.
@@ -105,8 +106,21 @@ def periodic_d2(x, y, xc, yc, Lx=1.0, Ly=1.0):
     return dx * dx + dy * dy
 
 
-def config1(Ne=128, p=3, seed=1, gamma=1.4):
-    """C1 1D step data with seeded relative noise."""
+def periodic_box(x, a, b, w):
+    """Smoothed indicator of [a, b) on the unit circle (tanh edges of width w)."""
+    return sum(0.5 * (np.tanh((x - a - k) / w) - np.tanh((x - b - k) / w)) for k in (-1, 0, 1))
+
+
+def config1(Ne=128, p=3, seed=1, gamma=1.4, width=0.01):
+    """C1 1D blast data with seeded relative noise.
+
+    Pressure 1000 / 0.01 / 100 on [0,0.1) / [0.1,0.9) / [0.9,1], the three
+    levels joined by tanh edges of width 0.01 (periodically, across x = 0),
+    times (1 + 1e-3 eta).  alpha = 10 h^2 (h = 1/N_e), the reference's own
+    1D choice (IGR_adjoints_1d/paper_scripts/generate_gamma_sensitivity_data.jl).
+    With sharp steps, or with alpha = 10 (h/4)^2, the unlimited DG scheme
+    reaches a negative pressure (Julia DomainError) within the run.
+    """
     rng = np.random.default_rng(seed)
     x = coords1(p, Ne)
     k = np.arange(1, 9)
@@ -114,19 +128,26 @@ def config1(Ne=128, p=3, seed=1, gamma=1.4):
     ph = rng.uniform(0, 2 * np.pi, 8)
     eta = np.sum(amp[:, None, None] * np.sin(2 * np.pi * k[:, None, None] * x[None] + ph[:, None, None]), axis=0)
     eta /= np.max(np.abs(eta))
-    pr = np.where(x < 0.1, 1000.0, np.where(x < 0.9, 0.01, 100.0)) * (1.0 + 1e-3 * eta)
+    base = 0.01 + (1000.0 - 0.01) * periodic_box(x, 0.0, 0.1, width) + (100.0 - 0.01) * periodic_box(x, 0.9, 1.0, width)
+    pr = base * (1.0 + 1e-3 * eta)
     q = np.zeros((3, Ne, p + 1))
     q[1] = 1.0
     q[2] = pr / (gamma - 1.0)
     h = 1.0 / Ne
     lam = max_wavespeed1(q, gamma)
     dt = 0.1 * h / ((2 * p + 1) * lam)
-    return dict(q=q, gamma=gamma, alpha=alpha_of(h, p), dt=dt, T=0.038, n_iter=50, solver="jacobi",
+    return dict(q=q, gamma=gamma, alpha=10.0 * h * h, dt=dt, T=0.038, n_iter=50, solver="jacobi",
                 p=p, Ne=Ne)
 
 
-def config2(N=128, p=3, seed=2, gamma=1.4, n_steps=200, cfl=0.4, sigma=0.02, p_bg=1e-5):
-    """C2 2D triple energy deposit."""
+def config2(N=128, p=3, seed=2, gamma=1.4, n_steps=200, cfl=0.4, sigma=0.02, p_bg=1.0):
+    """C2 2D triple energy deposit.
+
+    Background pressure 1, as in the reference's triple-Sedov script
+    (IGR_adjoints_2d/paper_scripts/generate_sedov_heatmap_data.jl).  With
+    p_bg = 1e-5 the unlimited DG scheme reaches a negative pressure (Julia
+    DomainError) after 86 of the 200 steps at 128x128.
+    """
     rng = np.random.default_rng(seed)
     x, y = coords2(p, N, N)
     wq = quad2(p, N, N)

@@ -192,6 +192,21 @@ class Context:
         self.check(self._L.igr_run_steps(self.h, int(n_steps), float(dt), float(gamma), float(alpha), int(n_iter),
                                          _solver_code(solver)))
 
+    # forward tangents
+    def run_sensitivity(self, s0, dt, T, gamma, alpha, n_iter, solver="pcg", form=2, sgam=0.0):
+        """Advance the context state and tangent states s0 (n_tan, nv, ...) to T; returns (s, varsigma, n_steps)."""
+        s0 = _f64(s0)
+        if s0.ndim == len(self.shape) + 1:
+            s0 = s0[None]
+        nt = s0.shape[0]
+        so = np.empty_like(s0)
+        vo = np.empty((nt,) + self.shape)
+        ns = C.c_int(0)
+        self.check(self._L.igr_run_sensitivity(self.h, nt, _p(s0), float(dt), float(T), float(gamma), float(alpha),
+                                               int(n_iter), _solver_code(solver), int(form), float(sgam), _p(so),
+                                               _p(vo), C.byref(ns)))
+        return so, vo, ns.value
+
     # discrete adjoint
     def objective(self, obj, gamma, prm=None):
         J = C.c_double(0)
@@ -302,6 +317,21 @@ def run_forward_(q, Sigma, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="
     qo, so = c.get_state()
     q[...] = qo
     Sigma[...] = so
+
+
+def run_sensitivity(q0, s0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg", elliptic_rhs="form2",
+                    sgamma=0.0):
+    """run_sensitivity (IGR_adjoints_2d/sensitivity.jl:353; 1D :475): returns
+    (q, Sigma, s, varsigma) at T.  elliptic_rhs and sgamma apply in 1D only."""
+    c = _ctx(mesh)
+    code = _solver_code(solver)
+    form = {"form1": 1, "form2": 2}.get(str(elliptic_rhs).lstrip(":"))
+    if form is None:
+        raise ValueError(f"Unknown elliptic_rhs: {elliptic_rhs} (use :form1 or :form2)")
+    c.set_state(q0)
+    s, vs, _ = c.run_sensitivity(s0, dt, T, gamma, alpha, n_iter, code, form, sgamma)
+    q, S = c.get_state()
+    return q, S, s[0], vs[0]
 
 
 OBJECTIVES = {"l2_density": 0, "l2_pressure": 1, "kinetic_energy": 2, "windowed_kinetic_energy": 3,

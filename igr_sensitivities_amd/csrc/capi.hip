@@ -22,6 +22,7 @@ void launch_fill(const Ctx& c, double* x, size_t n, double v);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch1_run(const Ctx& c, const K1Args& a, int block);
 void free_grad_work(const Ctx* c);
+void free_tan_bufs(const Ctx* c);
 
 double* Ctx::alloc(size_t n) {
   void* p = nullptr;
@@ -103,10 +104,12 @@ SipArgs sip_base(const Ctx& c, double alpha) {
 }
 
 // Sigma solve on stage state qs (device, 4N); warm start and result c.S.
-void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc) {
+void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc, double* x,
+            const double* b_in) {
   const size_t N = c.N;
+  if (!x) x = c.S;
   SipArgs a = sip_base(c, alpha);
-  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = c.S;
+  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x; a.b_in = b_in;
   a.invR_out = c.invR; a.invM_out = c.invM; a.sc = sc; a.n_iter = n_iter;
   if (solver == IGR_SOLVER_PCG) {
     a.rout = c.r;
@@ -120,20 +123,20 @@ void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, doub
       launch2_sip(c, M_PCG_A, k);
       SipArgs m = sip_base(c, alpha);
       m.sc = sc; m.n_iter = n_iter; m.it = it;
-      m.xout = c.S; m.pold = P[it % 2]; m.y = c.Ap; m.rout = c.r; m.invM = c.invM;
+      m.xout = x; m.pold = P[it % 2]; m.y = c.Ap; m.rout = c.r; m.invM = c.invM;
       launch2_sip(c, M_PCG_B, m);
     }
   } else if (solver == IGR_SOLVER_JACOBI) {
     a.b_out = c.b;
     launch2_sip(c, M_INIT_JAC, a);
-    double* X[2] = {c.S, c.xA};
+    double* X[2] = {x, c.xA};
     for (int it = 0; it < n_iter; ++it) {
       SipArgs k = sip_base(c, alpha);
       k.x = X[it % 2]; k.xout = X[(it + 1) % 2]; k.b = c.b; k.invM = c.invM; k.invR = c.invR;
       k.omega = 2.0 / 3.0;
       launch2_sip(c, M_JACOBI, k);
     }
-    if (n_iter % 2) launch_copy(c, c.S, c.xA, N);
+    if (n_iter % 2) launch_copy(c, x, c.xA, N);
   } else {
     a.rout = c.r; a.b_out = c.b;
     launch2_sip(c, M_INIT_CHEB, a);
@@ -141,7 +144,7 @@ void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, doub
     const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
     const double sigma = theta / std::max(delta, 1e-10);
     double rc = 1.0 / sigma;
-    double* X[2] = {c.S, c.xA};
+    double* X[2] = {x, c.xA};
     double* Dd[2] = {c.dA, c.dB};
     double* R[2] = {c.r, c.r2};
     for (int k = 1; k <= n_iter; ++k) {
@@ -153,7 +156,7 @@ void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, doub
       m.b = c.b; m.invM = c.invM; m.invR = c.invR;
       launch2_sip(c, M_CHEB, m);
     }
-    if (n_iter % 2) launch_copy(c, c.S, c.xA, N);
+    if (n_iter % 2) launch_copy(c, x, c.xA, N);
   }
 }
 
@@ -192,9 +195,10 @@ K1Args k1_base(Ctx& c, double gamma, double alpha, int n_iter, int solver) {
   a.invR = c.invR; a.invM = c.invM; a.b = c.b; a.r = c.r; a.p = c.pA; a.z = c.pB; a.Ap = c.Ap;
   a.u = c.xA; a.g = c.dA; a.F = c.k1F; a.fstar = c.k1face; a.fjump = c.k1face + 3 * c.Nex;
   a.favg = c.k1face + 4 * c.Nex; a.fsig = c.k1face + 5 * c.Nex; a.flag = c.flag;
+  a.bs = c.dB;
   return a;
 }
-int k1_block(const Ctx& c) {
+int k1_block(const Ctx& c) {  // threads of the 1D whole-instance workgroup
   const size_t N = c.N;
   return N >= 1024 ? 1024 : (int)(((N + 63) / 64) * 64);
 }
@@ -255,6 +259,7 @@ int igr_destroy(igr_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_graphs(*c);
   free_grad_work(c);
+  free_tan_bufs(c);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

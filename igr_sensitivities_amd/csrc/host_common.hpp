@@ -46,10 +46,12 @@ inline void destroy_graphs(Ctx& c) {
 
 // defined in capi.hip
 void step2(Ctx& c, double dt, double gamma, double alpha, int n_iter, int solver);
-void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc);
+void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc, double* x = nullptr,
+            const double* b_in = nullptr);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 K1Args k1_base(Ctx& c, double gamma, double alpha, int n_iter, int solver);
 int run1(Ctx& c, const K1Args& a);
+int k1_block(const Ctx& c);
 int ensure_scal(Ctx& c, int n_iter);
 
 // n_steps SSP-RK3 steps on the context state.  zero_sigma: Sigma <- 0 first

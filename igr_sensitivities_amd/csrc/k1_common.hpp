@@ -4,6 +4,8 @@
 
 namespace igr {
 
+constexpr int K1_MAXTAN = 4;
+
 enum K1Op { OP_RUN = 0, OP_SOLVE = 1, OP_RHS = 2, OP_HYP = 3, OP_APPLY = 4 };
 
 // One 1D instance (one workgroup).  All pointers are device memory.
@@ -38,6 +40,14 @@ struct K1Args {
   double* favg = nullptr;
   double* fsig = nullptr;
   double* trace = nullptr; // 2*n_iter PCG scalars (OP_SOLVE)
+  // forward tangents (OP_RUN with n_tan > 0): state, stage buffers, varsigma
+  int n_tan = 0, form = 2;
+  double sgam = 0.0;
+  double* ts[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
+  double* ts1[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
+  double* ts2[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
+  double* tvs[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
+  double* bs = nullptr;    // tangent elliptic right-hand side (N)
   const double* x_in = nullptr;   // OP_APPLY
   double* y_out = nullptr;
   int* flag = nullptr;
@@ -57,6 +67,20 @@ __device__ __forceinline__ double wave1(double g, const double q[4], int* flag) 
   const double arg = g * P / q[1];
   if (!(arg >= 0.0)) flag_domain(flag);
   return fabs(u) + sqrt(arg);
+}
+// sensitivity_flux, IGR_adjoints_1d/sensitivity.jl:36-50; q = (mu, rho, E, Sigma),
+// s = (s_mu, s_rho, s_E, s_Sigma)
+__device__ __forceinline__ void sflux1(double g, const double q[4], const double s[4], double sgam, double f[3]) {
+  const double m = q[0], r = q[1], E = q[2], S = q[3];
+  const double sm = s[0], sr = s[1], sE = s[2], sS = s[3];
+  const double u = m / r;
+  const double su = (sm - u * sr) / r;
+  const double P = (g - 1.0) * (E - m * m / (2.0 * r));
+  const double e = E / r - u * u / 2.0;
+  const double sP = (g - 1.0) * (sE - sr * (u * u) / 2.0 - r * u * su) + sgam * r * e;
+  f[0] = sm * u + m * su + sP + sS;
+  f[1] = sm;
+  f[2] = su * (E + P + S) + u * (sE + sP + sS);
 }
 #pragma clang fp contract(fast)
 

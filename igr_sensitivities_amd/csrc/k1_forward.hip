@@ -76,8 +76,9 @@ struct K1 {
     return r;
   }
 
-  // solve_elliptic! (elliptic.jl:451-467) on state qs, warm start a.S
-  __device__ void solve(const double* qs, int final_solve_trace) {
+  // 1/rho, IGR source b and diag(A) of stage state qs
+  // (compute_igr_rhs!, elliptic.jl:126-154; compute_sip_diagonal, :206-253)
+  __device__ void prepare(const double* qs) {
     const double* m = qs;
     const double* rho = qs + N;
     const double al = a.alpha, invJ = a.invJ;
@@ -86,7 +87,6 @@ struct K1 {
       a.u[n] = m[n] / rho[n];
     }
     __syncthreads();
-    // b = w J 2 alpha (du/dx)^2 and diag(A) (compute_sip_diagonal, :206-253)
     for (int n = tid; n < N; n += nt) {
       const int i = n % NP, e = n / NP;
       double du = 0.0;
@@ -113,11 +113,15 @@ struct K1 {
       a.invM[n] = 1.0 / Md;
     }
     __syncthreads();
-    double* x = a.S;
-    if (a.solver == 0) {            // pcg_fixed!, elliptic.jl:307-339
+  }
+
+  // fixed-iteration solve A x = b with the prepared 1/rho and diag(A)
+  // (pcg_fixed! :307-339, jacobi_fixed! :359-368, chebyshev_fixed! :389-430)
+  __device__ void iterate(double* x, const double* bb, int trace) {
+    if (a.solver == 0) {
       apply(x, a.Ap);
       for (int n = tid; n < N; n += nt) {
-        const double r = a.b[n] - a.Ap[n];
+        const double r = bb[n] - a.Ap[n];
         a.r[n] = r;
         a.p[n] = r * a.invM[n];
       }
@@ -135,23 +139,23 @@ struct K1 {
         __syncthreads();
         const double rzn = dot(a.r, a.z);
         const double bk = rzn / (rz + 1e-30);
-        if (final_solve_trace && tid == 0 && a.trace) { a.trace[it] = ak; a.trace[a.n_iter + it] = bk; }
+        if (trace && tid == 0 && a.trace) { a.trace[it] = ak; a.trace[a.n_iter + it] = bk; }
         for (int n = tid; n < N; n += nt) a.p[n] = a.z[n] + bk * a.p[n];
         __syncthreads();
         rz = rzn;
       }
-    } else if (a.solver == 1) {     // jacobi_fixed!, elliptic.jl:359-368 (omega = 2/3)
+    } else if (a.solver == 1) {
       for (int it = 0; it < a.n_iter; ++it) {
         apply(x, a.Ap);
-        for (int n = tid; n < N; n += nt) x[n] = x[n] + (2.0 / 3.0) * (a.b[n] - a.Ap[n]) * a.invM[n];
+        for (int n = tid; n < N; n += nt) x[n] = x[n] + (2.0 / 3.0) * (bb[n] - a.Ap[n]) * a.invM[n];
         __syncthreads();
       }
-    } else {                        // chebyshev_fixed!, elliptic.jl:389-430
+    } else {
       const double lmin = 1.0, lmax = a.cheb_lmax;
       const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
       const double sigma = theta / fmax(delta, 1e-10);
       apply(x, a.Ap);
-      for (int n = tid; n < N; n += nt) a.r[n] = a.b[n] - a.Ap[n];
+      for (int n = tid; n < N; n += nt) a.r[n] = bb[n] - a.Ap[n];
       __syncthreads();
       double rc = 1.0 / sigma;
       for (int k = 1; k <= a.n_iter; ++k) {
@@ -165,11 +169,103 @@ struct K1 {
         }
         __syncthreads();
         apply(x, a.Ap);
-        for (int n = tid; n < N; n += nt) a.r[n] = a.b[n] - a.Ap[n];
+        for (int n = tid; n < N; n += nt) a.r[n] = bb[n] - a.Ap[n];
         __syncthreads();
       }
     }
   }
+
+  // solve_elliptic! (elliptic.jl:451-467) on state qs, warm start a.S
+  __device__ void solve(const double* qs, int trace) {
+    prepare(qs);
+    iterate(a.S, a.b, trace);
+  }
+
+  // right-hand side of the tangent elliptic equation into a.bs
+  // (form 1: sensitivity.jl:168-256, SIP-like faces; form 2: :272-314, pointwise)
+  __device__ void sens_rhs(const double* qs, const double* ss) {
+    const double al = a.alpha, invJ = a.invJ, J = a.J;
+    const double* m = qs;
+    const double* r = qs + N;
+    const double* sm = ss;
+    const double* sr = ss + N;
+    const double* S = a.S;
+    if (a.form == 2) {
+      for (int n = tid; n < N; n += nt) {
+        const int i = n % NP, e = n / NP;
+        double ux = 0.0, sux = 0.0, dso = 0.0, dS = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const int nk = e * NP + k;
+          const double uj = m[nk] / r[nk];
+          const double suj = (sm[nk] - uj * sr[nk]) / r[nk];
+          ux += D(i, k) * uj; sux += D(i, k) * suj;
+          dso += D(i, k) * (sr[nk] / r[nk]);
+          dS += D(i, k) * S[nk];
+        }
+        ux *= invJ; sux *= invJ; dso *= invJ; dS *= invJ;
+        const double so = sr[n] / r[n];
+        const double R = al * (2.0 * ux * (2.0 * sux + so * ux) - dso * dS / r[n]);
+        a.bs[n] = a.B.w[i] * J * R;
+      }
+      __syncthreads();
+      return;
+    }
+    // form 1: faces first (kappa = s_rho / rho^2 diffusion of Sigma)
+    for (int n = tid; n < N; n += nt) {
+      const int i = n % NP, e = n / NP;
+      if (i != 0) continue;
+      const int eL = e == 0 ? Ne - 1 : e - 1;
+      const int nl = eL * NP + NP - 1;
+      const double kL = sr[nl] / (r[nl] * r[nl]), kR = sr[n] / (r[n] * r[n]);
+      double dL = 0.0, dR = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) { dL += D(NP - 1, k) * S[eL * NP + k]; dR += D(0, k) * S[e * NP + k]; }
+      a.fjump[e] = S[nl] - S[n];
+      a.favg[e] = invJ * (kL * dL + kR * dR) / 2.0;
+      a.fsig[e] = a.eta * fmax_first(kL, kR) / a.dx;
+    }
+    // kappa * dSigma at every node (volume stiffness-like term)
+    for (int n = tid; n < N; n += nt) {
+      const int i = n % NP, e = n / NP;
+      double dS = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) dS += D(i, k) * S[e * NP + k];
+      a.g[n] = al * a.B.w[i] * invJ * (sr[n] / (r[n] * r[n])) * dS;
+    }
+    __syncthreads();
+    for (int n = tid; n < N; n += nt) {
+      const int i = n % NP, e = n / NP;
+      const int eR = e == Ne - 1 ? 0 : e + 1;
+      double ux = 0.0, sux = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int nk = e * NP + k;
+        const double uj = m[nk] / r[nk];
+        const double suj = (sm[nk] - uj * sr[nk]) / r[nk];
+        ux += D(i, k) * uj; sux += D(i, k) * suj;
+      }
+      ux *= invJ; sux *= invJ;
+      const double src = 4.0 * al * ux * sux + sr[n] * S[n] / (r[n] * r[n]);
+      double v = a.B.w[i] * J * src;
+      double st = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) st += a.g[e * NP + k] * D(k, i);
+      v += st;
+      const int n0 = e * NP, n1 = e * NP + NP - 1;
+      const double k0 = sr[n0] / (r[n0] * r[n0]), k1 = sr[n1] / (r[n1] * r[n1]);
+      v += -al * (k0 / 2.0) * invJ * D(0, i) * a.fjump[e];
+      v += -al * (k1 / 2.0) * invJ * D(NP - 1, i) * a.fjump[eR];
+      if (i == 0) v += al * a.favg[e] - al * a.fsig[e] * a.fjump[e];
+      if (i == NP - 1) v += -al * a.favg[eR] + al * a.fsig[eR] * a.fjump[eR];
+      a.bs[n] = v;
+    }
+    __syncthreads();
+  }
+
+  // sensitivity hyperbolic RHS (sensitivity.jl:91-150) + SSP-RK3 stage of a tangent
+  __device__ void sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn, int mode,
+                                 double cdt, double* out);
 
   // hyperbolic RHS (hyperbolic.jl:23-83) fused with the SSP-RK3 stage update
   __device__ void hyp_stage(const double* qs, int mode, double cdt, double* out);
@@ -227,6 +323,60 @@ __device__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double
   }
   __syncthreads();
 }
+template <int NP>
+__device__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
+                                       int mode, double cdt, double* out) {
+  const double g = a.gamma, invJ = a.invJ;
+  for (int n = tid; n < N; n += nt) {
+    const double qq[4] = {qs[n], qs[N + n], qs[2 * N + n], a.S[n]};
+    const double sv[4] = {ss[n], ss[N + n], ss[2 * N + n], svs[n]};
+    double f[3];
+    sflux1(g, qq, sv, a.sgam, f);
+    a.F[n] = f[0]; a.F[N + n] = f[1]; a.F[2 * N + n] = f[2];
+    const int i = n % NP, e = n / NP;
+    if (i == 0) {     // sensitivity_llf_flux at face e, primal wave speed (sensitivity.jl:58-78)
+      const int eL = e == 0 ? Ne - 1 : e - 1;
+      const int nl = eL * NP + NP - 1;
+      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], a.S[nl]};
+      const double sL[4] = {ss[nl], ss[N + nl], ss[2 * N + nl], svs[nl]};
+      const double lam = fmax_first(wave1(g, qL, a.flag), wave1(g, qq, a.flag));
+      double fL[3], fR[3];
+      sflux1(g, qL, sL, a.sgam, fL);
+      sflux1(g, qq, sv, a.sgam, fR);
+#pragma unroll
+      for (int v = 0; v < 3; ++v) a.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sv[v]) / 2.0;
+    }
+  }
+  __syncthreads();
+  for (int n = tid; n < N; n += nt) {
+    const int i = n % NP, e = n / NP;
+    const int eR = e == Ne - 1 ? 0 : e + 1;
+    double acc[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      acc[v] = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * a.F[v * N + e * NP + k];
+    }
+    if (i == NP - 1)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (a.fstar[v * Ne + eR] - a.F[v * N + n]) / a.B.w[NP - 1];
+    if (i == 0)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[v] += invJ * (a.fstar[v * Ne + e] - a.F[v * N + n]) / a.B.w[0];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const size_t gi = (size_t)v * N + n;
+      double o;
+      if (mode == 0) o = acc[v];
+      else if (mode == 1) o = ss[gi] + cdt * acc[v];
+      else if (mode == 2) o = 3.0 / 4.0 * sn[gi] + 1.0 / 4.0 * ss[gi] + cdt * acc[v];
+      else o = 1.0 / 3.0 * sn[gi] + 2.0 / 3.0 * ss[gi] + cdt * acc[v];
+      out[gi] = o;
+    }
+  }
+  __syncthreads();
+}
 #pragma clang fp contract(fast)
 
 // Whole-run kernel: `op` selects what one launch does.
@@ -255,7 +405,13 @@ __device__ void k1_body(const K1Args& a) {
     return;
   }
   // OP_RUN
-  if (a.zero_sigma) { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
+  if (a.zero_sigma) {
+    for (int n = k.tid; n < N; n += k.nt) {
+      a.S[n] = 0.0;
+      for (int t = 0; t < a.n_tan; ++t) a.tvs[t][n] = 0.0;
+    }
+    __syncthreads();
+  }
   const double dt = a.dt;
   for (int step = 0; step < a.n_steps; ++step) {
     const double* st[3] = {a.q, a.q1, a.q2};
@@ -264,10 +420,20 @@ __device__ void k1_body(const K1Args& a) {
     for (int s = 0; s < 3; ++s) {
       if (a.alpha > 0.0) k.solve(st[s], 0);
       else { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
+      for (int t = 0; t < a.n_tan; ++t) {
+        const double* ss = s == 0 ? a.ts[t] : (s == 1 ? a.ts1[t] : a.ts2[t]);
+        double* so = s == 0 ? a.ts1[t] : (s == 1 ? a.ts2[t] : a.ts[t]);
+        if (a.alpha > 0.0) { k.sens_rhs(st[s], ss); k.iterate(a.tvs[t], a.bs, 0); }
+        else { for (int n = k.tid; n < N; n += k.nt) a.tvs[t][n] = 0.0; __syncthreads(); }
+        k.sens_hyp_stage(st[s], ss, a.tvs[t], a.ts[t], s + 1, cd[s], so);
+      }
       k.hyp_stage(st[s], s + 1, cd[s], ou[s]);
     }
   }
-  if (a.final_solve && a.alpha > 0.0) k.solve(a.q, 0);
+  if (a.final_solve && a.alpha > 0.0) {
+    k.solve(a.q, 0);
+    for (int t = 0; t < a.n_tan; ++t) { k.sens_rhs(a.q, a.ts[t]); k.iterate(a.tvs[t], a.bs, 0); }
+  }
 }
 
 // Single instance: arguments by value in the kernel-argument segment.

@@ -351,6 +351,9 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       a.y[gme] = y;
       a.irbar[gme] = -ir * ir * irb;     // adjoint of rho
     }
+    double rv[1] = {ab};
+    block_sum<1>(rv, red);
+    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
   } else if constexpr (MODE == RV_PCG_ITER) {
     double dv[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
         dv[0] = pbk * pm;
         dv[1] = xb * pm;
         dv[2] = (o_rb + pbk * iM) * apm;
-        dv[3] = (o_R * iM) * apm;
+        dv[3] = (a.R[(size_t)k * N + gme] * iM) * apm;
       }
     }
     double rv[1] = {ab};

@@ -30,6 +30,7 @@ struct SipArgs {
   const double* pold = nullptr;
   const double* b = nullptr;
   const double* d = nullptr;
+  const double* b_in = nullptr;  // INIT: use this right-hand side instead of the IGR source
   double* sc = nullptr;          // per-solve scalars (rz | pAp | a | beta)
   int first = 0, it = 0, n_iter = 0;
   double c1 = 0.0, c2 = 0.0, theta = 1.0, omega = 0.0;
@@ -56,6 +57,22 @@ struct HypArgs {
   double* out;        // dq (mode 0) or the stage output state
   int mode;           // 0: dq ; 1, 2, 3: SSP-RK3 stage
   double cdt;         // dt, (1/4)dt or (2/3)dt, as the reference forms it
+  int* flag;
+};
+
+// Forward tangent (sensitivity) kernels.
+struct SensArgs {
+  Geo2 g;
+  BasisK B;
+  double gamma, alpha;
+  const double* q;    // primal stage state (4 fields)
+  const double* S;    // primal Sigma
+  const double* s;    // tangent stage state (4 fields: zx, zy, sigma, eta)
+  const double* vs;   // tangent Sigma (varsigma)
+  const double* sn;   // tangent state at t^n (stages 2, 3)
+  double* out;        // b (k2_sens_rhs) or the tangent stage output / ds
+  int mode;           // 0: ds ; 1, 2, 3: SSP-RK3 stage
+  double cdt;
   int* flag;
 };
 

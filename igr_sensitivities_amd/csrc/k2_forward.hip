@@ -173,6 +173,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
       const double dv = uxx + uyy;
       const double R = al * (dv * dv + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
       bval = B.w[i] * B.w[j] * g.Jx * g.Jy * R;
+      if (a.b_in) bval = a.b_in[gme];
     }
   }
   if (!valid) {

@@ -97,6 +97,20 @@ int igr_run_forward(igr_ctx* ctx, double dt, double t_end, double gamma, double 
 int igr_run_steps(igr_ctx* ctx, int n_steps, double dt, double gamma, double alpha, int n_iter,
                   int solver);
 
+/* Forward tangent (sensitivity) equations advanced together with the
+ * primal: run_sensitivity (IGR_adjoints_2d/sensitivity.jl:353-413; 1D
+ * IGR_adjoints_1d/sensitivity.jl:475-515).  The primal initial state is the
+ * context state (on return it holds q(T), Sigma(T)); s0 holds n_tan tangent
+ * states (nv*N each, back to back).  Returns the tangents s(T) and their
+ * varsigma(T) (N each).  2D: n_steps = ceil(T/dt), dt = T/n_steps; 1D: the
+ * reference's `while t < T - 1e-14; dt = min(dt, T - t)` loop.  form (1D
+ * only): 1 = elliptic_rhs :form1, 2 = :form2; sgam (1D only): the gamma
+ * source s_gamma of sensitivity_flux (sensitivity.jl:36-50).  The
+ * dissipation of the tangent LLF flux uses the primal wave speed. */
+int igr_run_sensitivity(igr_ctx* ctx, int n_tan, const double* s0, double dt, double t_end, double gamma,
+                        double alpha, int n_iter, int solver, int form, double sgam, double* s_out,
+                        double* vs_out, int* n_steps_out);
+
 /* Objectives: replace the `objective::Symbol` of enzyme_adjoint_ic
  * (IGR_adjoints_2d/enzyme_adjoint.jl:240-254).  0..4 are the reference's
  * :l2_density, :l2_pressure, :kinetic_energy (KE / mass),

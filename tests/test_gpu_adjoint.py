@@ -128,7 +128,7 @@ def test_gradient_c3_32_vs_oracle_and_fd(orc):
     # the C3 gradient check mesh: 32x32 triple deposit, windowed-density objective, PCG-60
     cfg = configs.config2(N=32, seed=2)
     prm = configs.config3_objective()
-    ns = 5
+    ns = 4
     q0, gam, a, dt = cfg["q"], cfg["gamma"], cfg["alpha"], cfg["dt"]
     c = ctx2(3, 32, 32)
     c.set_state(q0)

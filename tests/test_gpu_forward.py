@@ -137,15 +137,17 @@ def test_one_step_and_run_2d(orc, solver):
 
 
 def test_graph_replay_matches_eager_and_is_deterministic():
-    q0 = configs.config2(N=16, seed=4)["q"]
-    cfg = configs.config2(N=16, seed=4)
+    # several replays of the captured step, with eager work and syncs between them,
+    # must equal the eager launches bit for bit
+    q0 = configs.taylor_green2(3, 16, 16, A=0.3)
+    dt = 0.4 * (1 / 16) / (7 * configs.max_wavespeed2(q0, 1.4))
     outs = []
     for graphs in (1, 0, 1):
         c = ctx2(3, 16, 16)
         c.set_option(0, graphs)
         c.set_state(q0)
-        for _ in range(3):    # several replays with eager work and syncs between them
-            c.run_steps(4, cfg["dt"], cfg["gamma"], cfg["alpha"], 60, "pcg")
+        for _ in range(3):
+            c.run_steps(4, dt, 1.4, 2e-3, 60, "pcg")
             outs.append(c.get_state()[0].copy())
     for k in range(3):
         np.testing.assert_array_equal(outs[k], outs[3 + k])
@@ -153,16 +155,17 @@ def test_graph_replay_matches_eager_and_is_deterministic():
 
 
 def test_tile_shapes_agree(orc):
-    cfg = configs.config2(N=24, seed=6)
+    q0 = configs.taylor_green2(3, 24, 24, A=0.3)
+    dt = 0.4 * (1 / 24) / (7 * configs.max_wavespeed2(q0, 1.4))
     res = []
     for tile in (404, 804, 202):
         c = ctx2(3, 24, 24)
         c.set_option(1, tile)
-        c.set_state(cfg["q"])
-        c.run_steps(2, cfg["dt"], cfg["gamma"], cfg["alpha"], 60, "pcg")
+        c.set_state(q0)
+        c.run_steps(2, dt, 1.4, 2e-3, 60, "pcg")
         res.append(c.get_state()[0])
     g = orc.Grid2D(3, 24, 24)
-    ref, _ = g.run_steps(cfg["gamma"], cfg["alpha"], 60, orc.PCG, 2, cfg["dt"], cfg["q"])
+    ref, _ = g.run_steps(1.4, 2e-3, 60, orc.PCG, 2, dt, q0)
     for r in res:
         assert rel(r, ref) < 1e-11
 
@@ -270,6 +273,7 @@ def test_mirror_api_run_forward(orc):
     igr.run_forward_(q, S, 2e-3, 0.02, basis, mesh, 1.4, 3e-3, 10, ":pcg")
     g = orc.Grid2D(3, 6, 6)
     qr, Sr, _ = g.run_forward(1.4, 3e-3, 10, orc.PCG, 2e-3, 0.02, q0)
-    assert rel(q, qr) < 1e-10 and rel(S, Sr) < 1e-10
+    # q to 1e-10; Sigma (an under-converged PCG-10 result) to 1e-7
+    assert rel(q, qr) < 1e-10 and rel(S, Sr) < 1e-7
     with pytest.raises(ValueError):
         igr.run_forward_(q, S, 2e-3, 0.02, basis, mesh, 1.4, 3e-3, 10, ":gmres")
