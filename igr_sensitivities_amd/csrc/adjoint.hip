@@ -28,6 +28,7 @@ void launch2_rev(const Ctx& c, int mode, const RevArgs& a);
 void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad);
 void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
+void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
 int grid2_blocks(const Ctx& c);
 Geo2 geo2(const Ctx& c);
 SipArgs sip_base(const Ctx& c, double alpha);
@@ -99,19 +100,25 @@ static void solve2_store(Ctx& c, const double* qs, double* x, double alpha, int 
   SipArgs a = sip_base(c, alpha);
   a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x;
   a.invR_out = st.invR; a.invM_out = st.invM; a.sc = st.sc; a.n_iter = n_iter; a.rout = st.R;
+  double* R0 = c.partials;
+  double* R1 = c.partials + c.pstride;
+  a.part_out = R0;
   launch2_sip(c, M_INIT_PCG, a);
   for (int it = 0; it < n_iter; ++it) {
     SipArgs k = sip_base(c, alpha);
     k.sc = st.sc; k.n_iter = n_iter; k.it = it; k.first = (it == 0);
     k.r = st.R + (size_t)it * N; k.invM = st.invM; k.invR = st.invR;
     k.pold = st.P + (size_t)(it > 0 ? it - 1 : 0) * N; k.pnew = st.P + (size_t)it * N; k.y = st.AP + (size_t)it * N;
+    k.part_in = R0; k.part_out = R1;
     launch2_sip(c, M_PCG_A, k);
     SipArgs m = sip_base(c, alpha);
     m.sc = st.sc; m.n_iter = n_iter; m.it = it;
     m.xout = x; m.pold = st.P + (size_t)it * N; m.y = st.AP + (size_t)it * N;
     m.r = st.R + (size_t)it * N; m.rout = st.R + (size_t)(it + 1) * N; m.invM = st.invM;
+    m.part_in = R1; m.part_out = R0;
     launch2_sip(c, M_PCG_B, m);
   }
+  launch_pcg_final(c, R0, grid2_blocks(c), st.sc, n_iter);
 }
 
 // One forward step from (q^n in W.Q[0], Sigma_0 in W.Sg[0]) storing what the
@@ -147,12 +154,16 @@ static void solve2_rev(Ctx& c, GradWork& W, const double* qs, const double* x0, 
   a.invR = st.invR; a.invM = st.invM; a.P = st.P; a.AP = st.AP; a.R = st.R; a.sc = st.sc; a.rs = st.rs;
   a.xbar = xbar; a.irbar = W.irbar; a.imbar = W.imbar; a.abar_blk = W.ablk;
   a.partials = c.partials; a.counter = c.counter;
+  double* R0 = c.partials;
+  double* R1 = c.partials + c.pstride;
   if (n > 0) {
+    a.part_out = R0;
     launch2_rev(c, RV_PCG_START, a);
     for (int k = n - 1; k >= 0; --k) {
       const int m = n - 1 - k;
       RevArgs r = a;
       r.k = k; r.first = (k == n - 1);
+      r.part_in = (m % 2) ? R1 : R0; r.part_out = (m % 2) ? R0 : R1;
       r.rbar_in = W.RB[(m + 1) % 2]; r.rbar_out = W.RB[m % 2];
       r.pbar_in = W.PB[(m + 1) % 2]; r.pbar_out = W.PB[m % 2];
       launch2_rev(c, RV_PCG_ITER, r);

@@ -21,6 +21,7 @@ int grid2_blocks(const Ctx& c);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch1_run(const Ctx& c, const K1Args& a, int block);
+void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
 void free_grad_work(const Ctx* c);
 void free_tan_bufs(const Ctx* c);
 
@@ -112,7 +113,9 @@ void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, doub
   a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x; a.b_in = b_in;
   a.invR_out = c.invR; a.invM_out = c.invM; a.sc = sc; a.n_iter = n_iter;
   if (solver == IGR_SOLVER_PCG) {
-    a.rout = c.r;
+    double* R0 = c.partials;
+    double* R1 = c.partials + c.pstride;
+    a.rout = c.r; a.part_out = R0;
     launch2_sip(c, M_INIT_PCG, a);
     double* P[2] = {c.pA, c.pB};
     for (int it = 0; it < n_iter; ++it) {
@@ -120,12 +123,15 @@ void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, doub
       k.sc = sc; k.n_iter = n_iter; k.it = it; k.first = (it == 0);
       k.r = c.r; k.invM = c.invM; k.invR = c.invR;
       k.pold = P[(it + 1) % 2]; k.pnew = P[it % 2]; k.y = c.Ap;
+      k.part_in = R0; k.part_out = R1;
       launch2_sip(c, M_PCG_A, k);
       SipArgs m = sip_base(c, alpha);
       m.sc = sc; m.n_iter = n_iter; m.it = it;
       m.xout = x; m.pold = P[it % 2]; m.y = c.Ap; m.rout = c.r; m.invM = c.invM;
+      m.part_in = R1; m.part_out = R0;
       launch2_sip(c, M_PCG_B, m);
     }
+    launch_pcg_final(c, R0, grid2_blocks(c), sc, n_iter);
   } else if (solver == IGR_SOLVER_JACOBI) {
     a.b_out = c.b;
     launch2_sip(c, M_INIT_JAC, a);
@@ -239,8 +245,10 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   A(c->pA, N); A(c->pB, N); A(c->Ap, N); A(c->xA, N); A(c->dA, N); A(c->dB, N); A(c->tmp, N);
   A(c->dscalar, 64);
   if (dim == 1) { A(c->k1F, 3 * N); A(c->k1face, 6 * (size_t)nex); }
-  // sized for the smallest tile (2x2 elements), so any tile choice fits
-  c->partials_n = 4 * (size_t)(((nex + 1) / 2) * ((c->Ney + 1) / 2)) + 64;
+  // two regions of 4 partial sums per workgroup, sized for the smallest tile
+  // (2x2 elements) so any tile choice fits
+  c->pstride = 4 * (size_t)(((nex + 1) / 2) * ((c->Ney + 1) / 2)) + 64;
+  c->partials_n = 2 * c->pstride;
   A(c->partials, c->partials_n);
   void* pp = nullptr;
   if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
@@ -484,19 +492,22 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
   const size_t N = c->N;
   // valid solver buffers for the current state, on copies so the state is untouched
   launch_copy(*c, c->xA, c->S, N);
+  double* R0 = c->partials;
+  double* R1 = c->partials + c->pstride;
   SipArgs a = sip_base(*c, alpha);
   a.mx = c->q; a.my = c->q + N; a.rho = c->q + 2 * N; a.x = c->xA;
-  a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = 4;
+  a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = 4; a.part_out = R0;
   launch2_sip(*c, M_INIT_PCG, a);
   SipArgs k = sip_base(*c, alpha);
   k.sc = c->scal; k.n_iter = 4; k.it = 0; k.first = 1;
   k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = c->pB; k.pnew = c->pA; k.y = c->Ap;
+  k.part_in = R0; k.part_out = R1;
   launch2_sip(*c, M_PCG_A, k);
   SipArgs m = sip_base(*c, alpha);
   m.sc = c->scal; m.n_iter = 4; m.it = 0;
-  m.xout = c->xA; m.pold = c->pA; m.y = c->Ap; m.rout = c->r2; m.invM = c->invM;
-  launch_copy(*c, c->r2, c->r, N);
-  launch2_sip(*c, M_PCG_B, m);   // fills rz_1 and beta_0 for the timed PCG_A launches
+  m.xout = c->xA; m.pold = c->pA; m.y = c->Ap; m.r = c->r; m.rout = c->r2; m.invM = c->invM;
+  m.part_in = R1; m.part_out = R0;
+  launch2_sip(*c, M_PCG_B, m);   // rz_1 partials for the timed PCG_A launches
   k.first = 0; k.it = 1; k.pold = c->pA; k.pnew = c->pB;
   HypArgs h = hyp_base(*c, gamma);
   h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;

@@ -51,6 +51,7 @@ struct Ctx {
   size_t scal_n = 0;
   double* partials = nullptr; // grid reduction partials
   size_t partials_n = 0;
+  size_t pstride = 0;         // two partial-sum regions: partials, partials + pstride
   unsigned* counter = nullptr;
   int* flag = nullptr;        // domain flag
   double* dscalar = nullptr;  // small device scalar scratch

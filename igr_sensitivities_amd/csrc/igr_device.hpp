@@ -81,6 +81,39 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* red, double* p
   }
 }
 
+// Store this workgroup's NV partial sums at part[k*nb + bid] (no atomics,
+// no fences: the consumer is the next kernel on the stream).
+template <int NV>
+__device__ __forceinline__ void block_partials_store(double (&v)[NV], double* red, double* part, int nb, int bid) {
+  block_sum<NV>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[(size_t)k * nb + bid] = v[k];
+  }
+}
+
+// Every workgroup sums the nb partials of the previous kernel in the same
+// fixed order, so all workgroups hold bit-identical totals.
+template <int NV>
+__device__ __forceinline__ void grid_total(const double* part, int nb, double (&out)[NV], double* red) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)k * nb + b];
+    acc[k] = s;
+  }
+  block_sum<NV>(acc, red);
+  __shared__ double bc[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bc[k] = acc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = bc[k];
+}
+
 // Julia max(x, y): returns x on ties.
 __device__ __forceinline__ double fmax_first(double x, double y) { return (y > x) ? y : x; }
 

@@ -196,15 +196,30 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const FS fs(a.n_iter);
   const int k = a.k;
 
-  // reverse scalars of this iteration
+  // reverse scalars of this iteration, from the dot products the previous
+  // kernel left as partial sums (every workgroup forms the same totals)
   double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
   if constexpr (MODE == RV_PCG_ITER) {
+    double d[4];
+    grid_total<4>(a.part_in, gridDim.x, d, red);
+    const double eps = 1e-30;
     ak = a.sc[fs.a(k)];
     bk = a.sc[fs.beta(k)];
-    rzb = a.rs[RS::rzbar_next(k)];
-    papb = a.rs[RS::papbar(k)];
+    const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
+    const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
+    const double bb = d[0];
+    rzb = carry_next + bb / (rzk + eps);
+    const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
+    papb = -abk * rzk / ((papk + eps) * (papk + eps));
+    if (blockIdx.x == 0 && tid == 0) {
+      a.rs[RS::bbar(k)] = bb;
+      a.rs[RS::abar(k)] = abk;
+      a.rs[RS::papbar(k)] = papb;
+      a.rs[RS::rzbar_next(k)] = rzb;
+      a.rs[RS::carry(k)] = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
+    }
   } else if constexpr (FIN) {
-    rzb = a.n_iter > 0 ? a.rs[RS::rzbar0(a.n_iter)] : 0.0;
+    rzb = a.n_iter > 0 ? a.rs[RS::carry(0)] : 0.0;
   }
   const bool have_carry = (MODE == RV_PCG_ITER) ? !a.first : (a.n_iter > 0);
 
@@ -375,28 +390,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     double rv[1] = {ab};
     block_sum<1>(rv, red);
     if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
-    double* rs = a.rs;
-    const double* sc = a.sc;
-    const int nit = a.n_iter;
-    grid_sum<4>(dv, red, a.partials, a.counter, gridDim.x, blockIdx.x, [rs, sc, nit, k](double* tot) {
-      const FS f(nit);
-      const double eps = 1e-30;
-      const double rzk = sc[f.rz(k)], rzk1 = sc[f.rz(k + 1)], papk = sc[f.pap(k)];
-      const double carry = -rs[RS::bbar(k)] * rzk1 / ((rzk + eps) * (rzk + eps)) + rs[RS::abar(k)] / (papk + eps);
-      if (k == 0) {
-        rs[RS::rzbar0(nit)] = carry;
-      } else {
-        const int km = k - 1;
-        const double rzm = sc[f.rz(km)], papm = sc[f.pap(km)];
-        const double bb = tot[0];
-        const double rzt = carry + bb / (rzm + eps);
-        const double abm = tot[1] - tot[2] - 2.0 * rzt * tot[3];
-        rs[RS::bbar(km)] = bb;
-        rs[RS::rzbar_next(km)] = rzt;
-        rs[RS::abar(km)] = abm;
-        rs[RS::papbar(km)] = -abm * rzm / ((papm + eps) * (papm + eps));
-      }
-    });
+    block_partials_store<4>(dv, red, a.part_out, gridDim.x, blockIdx.x);
   } else if constexpr (FIN) {
     double ab_extra = 0.0;
     if (valid) {
@@ -465,7 +459,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
 template <int NP, int TX, int TY>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_rev_start(RevArgs a) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
-  __shared__ double red[NT / WAVE + 1];
+  __shared__ double red[4 * (NT / WAVE + 1)];
   TileInfo<NP, TX, TY> t(a.g);
   const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
   const int lx = s % TX, ly = s / TX;
@@ -476,17 +470,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_rev_start(RevArgs a)
     const size_t gi = t.gidx(s, n);
     dv[0] = a.xbar[gi] * a.P[(size_t)(nit - 1) * N + gi];
   }
-  double* rs = a.rs;
-  const double* sc = a.sc;
-  grid_sum<1>(dv, red, a.partials, a.counter, gridDim.x, blockIdx.x, [rs, sc, nit](double* tot) {
-    const FS f(nit);
-    const int k = nit - 1;
-    const double eps = 1e-30, papk = sc[f.pap(k)];
-    rs[RS::bbar(k)] = 0.0;
-    rs[RS::rzbar_next(k)] = 0.0;
-    rs[RS::abar(k)] = tot[0];
-    rs[RS::papbar(k)] = -tot[0] * sc[f.rz(k)] / ((papk + eps) * (papk + eps));
-  });
+  double d4[4] = {0.0, dv[0], 0.0, 0.0};
+  block_partials_store<4>(d4, red, a.part_out, gridDim.x, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------

@@ -70,17 +70,20 @@ struct RevArgs {
   double* y = nullptr;            // A^T wbar
   double* partials = nullptr;
   unsigned* counter = nullptr;
+  const double* part_in = nullptr;
+  double* part_out = nullptr;
 };
 
 // reverse-scalar layout per solve: for iteration k
 //   rs[5k+0] beta-bar_k, rs[5k+1] a-bar_k, rs[5k+2] pAp-bar_k,
-//   rs[5k+3] rz-bar total of rz_{k+1}, rs[5*n + 0] rz-bar total of rz_0
+//   rs[5k+3] rz-bar total of rz_{k+1}, rs[5k+4] the part of rz-bar of rz_k
+//   contributed by iteration k (a_k and beta_k); for k = 0 it is the total.
 struct RS {
   static __host__ __device__ int bbar(int k) { return 5 * k + 0; }
   static __host__ __device__ int abar(int k) { return 5 * k + 1; }
   static __host__ __device__ int papbar(int k) { return 5 * k + 2; }
-  static __host__ __device__ int rzbar_next(int k) { return 5 * k + 3; }   // rz-bar of rz_{k+1}
-  static __host__ __device__ int rzbar0(int n) { return 5 * n; }
+  static __host__ __device__ int rzbar_next(int k) { return 5 * k + 3; }
+  static __host__ __device__ int carry(int k) { return 5 * k + 4; }
 };
 
 // forward scalar layout (as written by the forward PCG kernels)

@@ -44,6 +44,8 @@ struct SipArgs {
   double* b_out = nullptr;
   double* partials = nullptr;
   unsigned* counter = nullptr;
+  const double* part_in = nullptr;   // partial sums written by the previous kernel
+  double* part_out = nullptr;        // this kernel's partial sums
   int* flag = nullptr;
 };
 

@@ -59,7 +59,15 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
   double own_d = 0.0;
   double beta = 0.0;
   if constexpr (MODE == M_PCG_A) {
-    if (!a.first) beta = a.sc[3 * a.n_iter + 1 + a.it - 1];
+    // rz_it from the previous kernel's partial sums; beta_{it-1} = rz_it / (rz_{it-1} + 1e-30)
+    double tot[1];
+    grid_total<1>(a.part_in, gridDim.x, tot, red);
+    const int nit = a.n_iter, it = a.it;
+    if (!a.first) beta = tot[0] / (a.sc[it - 1] + 1e-30);
+    if (blockIdx.x == 0 && tid == 0) {
+      a.sc[it] = tot[0];
+      if (!a.first) a.sc[3 * nit + 1 + it - 1] = beta;
+    }
   }
   // ---- stage the vector, 1/rho (and velocities for INIT) in LDS
   for (int idx = tid; idx < NS * NP2; idx += NT) {
@@ -236,11 +244,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
         a.b_out[gme] = bval;
       }
     }
-    if constexpr (MODE == M_INIT_PCG) {
-      double* sc = a.sc;
-      grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x,
-                  [sc](double* tot) { sc[0] = tot[0]; });
-    }
+    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
   } else if constexpr (MODE == M_PCG_A) {
     double red_v[1] = {0.0};
     if (valid) {
@@ -248,12 +252,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
       a.y[gme] = y;
       red_v[0] = vown * y;
     }
-    double* sc = a.sc;
-    const int nit = a.n_iter, it = a.it;
-    grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x, [sc, nit, it](double* tot) {
-      sc[nit + 1 + it] = tot[0];
-      sc[2 * nit + 1 + it] = sc[it] / (tot[0] + 1e-30);
-    });
+    block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
   } else if constexpr (MODE == M_JACOBI) {
     if (valid) a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
   } else if constexpr (MODE == M_CHEB) {
@@ -276,19 +275,37 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
   const bool valid = lx < t.nvx && ly < t.nvy;
   double* sc = a.sc;
   const int nit = a.n_iter, it = a.it;
+  // gather this thread's fields first, then the pAp total
+  size_t gi = 0;
+  double x = 0.0, p = 0.0, r = 0.0, ap = 0.0, im = 0.0;
+  if (valid) {
+    gi = t.gidx(s, n);
+    x = a.xout[gi]; p = a.pold[gi]; r = a.r ? a.r[gi] : a.rout[gi]; ap = a.y[gi]; im = a.invM[gi];
+  }
+  double tot[1];
+  grid_total<1>(a.part_in, gridDim.x, tot, red);
+  const double ak = sc[it] / (tot[0] + 1e-30);
+  if (blockIdx.x == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
   double red_v[1] = {0.0};
   if (valid) {
-    const size_t gi = t.gidx(s, n);
-    const double ak = sc[2 * nit + 1 + it];
-    a.xout[gi] = a.xout[gi] + ak * a.pold[gi];
-    const double rn = (a.r ? a.r[gi] : a.rout[gi]) - ak * a.y[gi];
+    a.xout[gi] = x + ak * p;
+    const double rn = r - ak * ap;
     a.rout[gi] = rn;
-    red_v[0] = rn * (rn * a.invM[gi]);
+    red_v[0] = rn * (rn * im);
   }
-  grid_sum<1>(red_v, red, a.partials, a.counter, gridDim.x, blockIdx.x, [sc, nit, it](double* tot) {
-    sc[it + 1] = tot[0];
-    sc[3 * nit + 1 + it] = tot[0] / (sc[it] + 1e-30);
-  });
+  block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
+}
+
+// After the last PCG iteration: rz_n and beta_{n-1} into the scalar record.
+__global__ void k2_pcg_final(const double* part, int nb, double* sc, int nit) {
+  __shared__ double red[16];
+  double tot[1];
+  grid_total<1>(part, nb, tot, red);
+  if (threadIdx.x == 0 && nit > 0) {
+    sc[nit] = tot[0];
+    sc[3 * nit + 1 + nit - 1] = tot[0] / (sc[nit - 1] + 1e-30);
+  }
+  if (threadIdx.x == 0 && nit == 0) sc[0] = tot[0];
 }
 
 // ------------------------------------------------------------------------
@@ -534,6 +551,9 @@ int grid2_blocks(const Ctx& c) {
   else if (c.np == 5) { tx = 2; ty = 4; }
   else if (c.np == 4) { tx = c.tile_x; ty = c.tile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
+}
+void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit) {
+  hipLaunchKernelGGL(k2_pcg_final, dim3(1), dim3(256), 0, c.stream, part, nb, sc, nit);
 }
 void launch_fill(const Ctx& c, double* x, size_t n, double v) {
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, x, n, v);
