@@ -7,10 +7,11 @@ gamma=1.4, alpha = 10 (h/4)^2, CFL 0.4.  Synthetic seeded data, resident in
 HBM before the timed region.
 
 Prints one JSON line with the throughput in DOF-stage updates/s
-(DOF = one nodal value of one conserved variable), the roofline of the
-dominant kernel (fused SIP apply + PCG direction update, timed with HIP
-events on its stream) and the single-thread CPU oracle timed on a bounded
-sample of the same workload.
+(DOF = one nodal value of one conserved variable), the wall time of the
+adjoint gradient over the same steps divided by the forward's, the roofline
+of the dominant kernel (fused SIP apply + PCG direction update, every launch
+of a PCG solve bracketed by HIP events on the context's stream) and the
+single-thread CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -76,7 +77,7 @@ def main():
     alg_bytes = 6 * 8 * n_nodes
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pcg_a_c2_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "pcg_a_2d_128x128_p3_traffic.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
@@ -84,6 +85,18 @@ def main():
             traffic = None
     ms_b = ctx.time_kernel(1, gam, al, reps)
     ms_h = ctx.time_kernel(2, gam, al, reps)
+
+    # adjoint gradient of J = int rho(T) w (Gaussian window, sigma 0.1) over the
+    # same steps, w.r.t. the initial state, gamma and alpha; checkpoint slots
+    # sized from free HBM (every step's state fits at this size)
+    prm = configs.config3_objective()
+    ctx.set_state(cfg["q"])
+    ctx.gradient(igr.OBJECTIVES["windowed_density"], 2, dt, gam, al, n_iter, "pcg", prm=prm)   # allocates work buffers
+    ctx.set_state(cfg["q"])
+    g0 = time.perf_counter()
+    J, _, dgam, dal = ctx.gradient(igr.OBJECTIVES["windowed_density"], args.steps, dt, gam, al, n_iter, "pcg", prm=prm)
+    g1 = time.perf_counter()
+    t_grad = g1 - g0
 
     cpu = None
     if not args.no_cpu:
@@ -132,7 +145,9 @@ def main():
             "alg_bytes_per_launch": alg_bytes,
         },
         "kernels_us": {"pcg_a_sip": ms * 1e3, "pcg_b_update": ms_b * 1e3, "hyp_rk3": ms_h * 1e3},
-        "gradient_over_forward": None,
+        "gradient_over_forward": t_grad / elapsed,
+        "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,
+                     "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

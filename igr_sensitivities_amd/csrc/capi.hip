@@ -21,6 +21,7 @@ int grid2_blocks(const Ctx& c);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch1_run(const Ctx& c, const K1Args& a, int block);
+void set_kernel_timer(hipEvent_t t0, hipEvent_t t1);
 void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
 void free_grad_work(const Ctx* c);
 void free_tan_bufs(const Ctx* c);
@@ -484,52 +485,71 @@ int igr_run_forward(igr_ctx* c, double dt, double t_end, double gamma, double al
 }
 
 int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps, double* ms) {
-  if (!c || !ms || reps < 1 || kernel < 0 || kernel > 2 || c->dim != 2) return IGR_EINVAL;
+  if (!c || !ms || reps < 1 || reps > 4096 || kernel < 0 || kernel > 2 || c->dim != 2) return IGR_EINVAL;
   int rc = begin_call(*c);
   if (rc) return rc;
-  rc = ensure_scal(*c, 4);
-  if (rc) return rc;
   const size_t N = c->N;
-  // valid solver buffers for the current state, on copies so the state is untouched
+  if (kernel == 2) {
+    // the hyperbolic RHS + RK3 stage-2 update, launched back to back on the current state
+    HypArgs h = hyp_base(*c, gamma);
+    h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;
+    launch_copy(*c, c->q1, c->q, c->nv * N);
+    std::vector<hipEvent_t> ev(2 * (size_t)reps);
+    for (auto& e : ev) HIPCHK(*c, hipEventCreate(&e));
+    for (int w = 0; w < 3; ++w) launch2_hyp(*c, h);
+    for (int r = 0; r < reps; ++r) {
+      set_kernel_timer(ev[2 * r], ev[2 * r + 1]);
+      launch2_hyp(*c, h);
+    }
+    HIPCHK(*c, hipEventSynchronize(ev.back()));
+    double tot = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      float t = 0.f;
+      HIPCHK(*c, hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]));
+      tot += t;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms = tot / reps;
+    return end_call(*c);
+  }
+  // PCG_A / PCG_B in the context they run in: a real PCG solve of `reps`
+  // iterations on the current state (into scratch vectors), every launch of
+  // the timed kernel stamped with start/end events by the kernel launch.
+  rc = ensure_scal(*c, reps);
+  if (rc) return rc;
   launch_copy(*c, c->xA, c->S, N);
   double* R0 = c->partials;
   double* R1 = c->partials + c->pstride;
   SipArgs a = sip_base(*c, alpha);
   a.mx = c->q; a.my = c->q + N; a.rho = c->q + 2 * N; a.x = c->xA;
-  a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = 4; a.part_out = R0;
+  a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = reps; a.part_out = R0;
   launch2_sip(*c, M_INIT_PCG, a);
-  SipArgs k = sip_base(*c, alpha);
-  k.sc = c->scal; k.n_iter = 4; k.it = 0; k.first = 1;
-  k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = c->pB; k.pnew = c->pA; k.y = c->Ap;
-  k.part_in = R0; k.part_out = R1;
-  launch2_sip(*c, M_PCG_A, k);
-  SipArgs m = sip_base(*c, alpha);
-  m.sc = c->scal; m.n_iter = 4; m.it = 0;
-  m.xout = c->xA; m.pold = c->pA; m.y = c->Ap; m.r = c->r; m.rout = c->r2; m.invM = c->invM;
-  m.part_in = R1; m.part_out = R0;
-  launch2_sip(*c, M_PCG_B, m);   // rz_1 partials for the timed PCG_A launches
-  k.first = 0; k.it = 1; k.pold = c->pA; k.pnew = c->pB;
-  HypArgs h = hyp_base(*c, gamma);
-  h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;
-  launch_copy(*c, c->q1, c->q, c->nv * N);
-  hipEvent_t e0, e1;
-  HIPCHK(*c, hipEventCreate(&e0));
-  HIPCHK(*c, hipEventCreate(&e1));
-  auto once = [&]() {
-    if (kernel == 0) launch2_sip(*c, M_PCG_A, k);
-    else if (kernel == 1) launch2_sip(*c, M_PCG_B, m);
-    else launch2_hyp(*c, h);
-  };
-  for (int w = 0; w < 3; ++w) once();
-  HIPCHK(*c, hipEventRecord(e0, c->stream));
-  for (int r = 0; r < reps; ++r) once();
-  HIPCHK(*c, hipEventRecord(e1, c->stream));
-  HIPCHK(*c, hipEventSynchronize(e1));
-  float t = 0.f;
-  HIPCHK(*c, hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *ms = (double)t / reps;
+  std::vector<hipEvent_t> ev(2 * (size_t)reps);
+  for (auto& e : ev) HIPCHK(*c, hipEventCreate(&e));
+  double* pbuf[2] = {c->pA, c->pB};
+  for (int it = 0; it < reps; ++it) {
+    SipArgs k = sip_base(*c, alpha);
+    k.sc = c->scal; k.n_iter = reps; k.it = it; k.first = (it == 0);
+    k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = pbuf[(it + 1) % 2]; k.pnew = pbuf[it % 2]; k.y = c->Ap;
+    k.part_in = R0; k.part_out = R1;
+    SipArgs m = sip_base(*c, alpha);
+    m.sc = c->scal; m.n_iter = reps; m.it = it;
+    m.xout = c->xA; m.pold = pbuf[it % 2]; m.y = c->Ap; m.rout = c->r; m.r = nullptr; m.invM = c->invM;
+    m.part_in = R1; m.part_out = R0;
+    if (kernel == 0) set_kernel_timer(ev[2 * it], ev[2 * it + 1]);
+    launch2_sip(*c, M_PCG_A, k);
+    if (kernel == 1) set_kernel_timer(ev[2 * it], ev[2 * it + 1]);
+    launch2_sip(*c, M_PCG_B, m);
+  }
+  HIPCHK(*c, hipEventSynchronize(ev.back()));
+  double tot = 0.0;
+  for (int it = 0; it < reps; ++it) {
+    float t = 0.f;
+    HIPCHK(*c, hipEventElapsedTime(&t, ev[2 * it], ev[2 * it + 1]));
+    tot += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *ms = tot / reps;
   return end_call(*c);
 }
 

@@ -28,6 +28,7 @@
 //                  SSP-RK3 stage combination (forward.jl:170-188), each face
 //                  LLF flux computed once per tile (euler.jl:222-258).
 // Dot products finish with the deterministic last-workgroup reduction.
+#include <hip/hip_ext.h>
 #include "igr_ctx.hpp"
 #include "k2_common.hpp"
 
@@ -496,6 +497,22 @@ __global__ void k_copy(double* dst, const double* src, size_t n) {
 // ------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------
+// Kernel-timestamp events for the next SIP/hyperbolic launch (set by
+// igr_time_kernel only): hipExtLaunchKernelGGL stamps them at the start and
+// end of the kernel itself, so their elapsed time excludes launch gaps.
+static hipEvent_t g_kt0 = nullptr, g_kt1 = nullptr;
+void set_kernel_timer(hipEvent_t t0, hipEvent_t t1) { g_kt0 = t0; g_kt1 = t1; }
+
+template <typename... Args, typename F = void (*)(Args...)>
+static void launch_k(F k, dim3 G, dim3 B, hipStream_t st, Args... args) {
+  if (g_kt0) {
+    hipExtLaunchKernelGGL(k, G, B, 0, st, g_kt0, g_kt1, 0, args...);
+    g_kt0 = g_kt1 = nullptr;
+  } else {
+    hipLaunchKernelGGL(k, G, B, 0, st, args...);
+  }
+}
+
 template <int NP, int TX, int TY>
 struct Launch2 {
   static dim3 grid(const Geo2& g) {
@@ -506,18 +523,18 @@ struct Launch2 {
   static void sip(int mode, const SipArgs& a, hipStream_t st) {
     dim3 G = grid(a.g), Bk = block();
     switch (mode) {
-      case M_APPLY: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_APPLY>), G, Bk, 0, st, a); break;
-      case M_INIT_PCG: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_PCG>), G, Bk, 0, st, a); break;
-      case M_INIT_CHEB: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_CHEB>), G, Bk, 0, st, a); break;
-      case M_INIT_JAC: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_INIT_JAC>), G, Bk, 0, st, a); break;
-      case M_PCG_A: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_PCG_A>), G, Bk, 0, st, a); break;
-      case M_JACOBI: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_JACOBI>), G, Bk, 0, st, a); break;
-      case M_CHEB: hipLaunchKernelGGL((k2_sip<NP, TX, TY, M_CHEB>), G, Bk, 0, st, a); break;
-      case M_PCG_B: hipLaunchKernelGGL((k2_pcg_b<NP, TX, TY>), G, Bk, 0, st, a); break;
+      case M_APPLY: launch_k(k2_sip<NP, TX, TY, M_APPLY>, G, Bk, st, a); break;
+      case M_INIT_PCG: launch_k(k2_sip<NP, TX, TY, M_INIT_PCG>, G, Bk, st, a); break;
+      case M_INIT_CHEB: launch_k(k2_sip<NP, TX, TY, M_INIT_CHEB>, G, Bk, st, a); break;
+      case M_INIT_JAC: launch_k(k2_sip<NP, TX, TY, M_INIT_JAC>, G, Bk, st, a); break;
+      case M_PCG_A: launch_k(k2_sip<NP, TX, TY, M_PCG_A>, G, Bk, st, a); break;
+      case M_JACOBI: launch_k(k2_sip<NP, TX, TY, M_JACOBI>, G, Bk, st, a); break;
+      case M_CHEB: launch_k(k2_sip<NP, TX, TY, M_CHEB>, G, Bk, st, a); break;
+      case M_PCG_B: launch_k(k2_pcg_b<NP, TX, TY>, G, Bk, st, a); break;
     }
   }
   static void hyp(const HypArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k2_hyp<NP, TX, TY>), grid(a.g), block(), 0, st, a);
+    launch_k(k2_hyp<NP, TX, TY>, grid(a.g), block(), st, a);
   }
 };
 
