@@ -17,6 +17,13 @@ _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
+
+class BatchItem(C.Structure):
+    """struct igr_batch_item of include/igr.h."""
+    _fields_ = [("ctx", _vp), ("dt", _d), ("t_end", _d), ("gamma", _d), ("alpha", _d), ("sgam", _d),
+                ("n_tan", _i), ("s0", _pd), ("s_out", _pd), ("vs_out", _pd), ("n_steps", _i), ("status", _i)]
+
+
 # name -> (restype, argtypes); every exported symbol of include/igr.h
 PROTOTYPES = {
     "igr_create": (_i, [C.POINTER(_vp), _i, _i, _i, _i, _d, _d]),
@@ -42,6 +49,7 @@ PROTOTYPES = {
     "igr_hyperbolic_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
     "igr_apply_sip_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
     "igr_solve_elliptic_vjp": (_i, [_vp, _d, _i, _i, _pd, _pd, _pd, _pd, _pd]),
+    "igr_batch_run_sensitivity": (_i, [C.POINTER(BatchItem), _i, _i, _i, _i]),
 }
 
 _lib = None

@@ -334,6 +334,53 @@ def run_sensitivity(q0, s0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver=
     return q, S, s[0], vs[0]
 
 
+def run_sensitivity_batch(runs, n_iter=10, solver="pcg", elliptic_rhs="form2"):
+    """run_sensitivity for many instances in one batched GPU run.
+
+    runs: list of dicts with keys ctx (a Context whose state is the primal
+    initial state), s0 (n_tan, nv, ...) tangent initial states (or None),
+    dt, T, gamma, alpha and, in 1D, sgamma.  Each context's state ends at
+    (q(T), Sigma(T)); returns [(s, varsigma, n_steps)] per run, each exactly
+    what Context.run_sensitivity gives for that run alone.  There is no
+    reference counterpart: the reference loops over run_sensitivity calls."""
+    code = _solver_code(solver)
+    form = {"form1": 1, "form2": 2}.get(str(elliptic_rhs).lstrip(":"))
+    if form is None:
+        raise ValueError(f"Unknown elliptic_rhs: {elliptic_rhs} (use :form1 or :form2)")
+    n = len(runs)
+    items = (_lib.BatchItem * n)()
+    keep, outs = [], []
+    for k, r in enumerate(runs):
+        c = r["ctx"]
+        s0 = r.get("s0")
+        if s0 is None:
+            s0 = np.empty((0, c.nv) + c.shape)
+        s0 = _f64(s0)
+        if s0.ndim == len(c.shape) + 1:
+            s0 = s0[None]
+        nt = s0.shape[0]
+        so = np.empty_like(s0)
+        vo = np.empty((nt,) + c.shape)
+        keep += [s0, so, vo]
+        outs.append((so, vo))
+        it = items[k]
+        it.ctx = c.h.value if isinstance(c.h, C.c_void_p) else c.h
+        it.dt, it.t_end = float(r["dt"]), float(r["T"])
+        it.gamma, it.alpha = float(r["gamma"]), float(r["alpha"])
+        it.sgam = float(r.get("sgamma", 0.0))
+        it.n_tan = nt
+        it.s0 = _p(s0) if nt else None
+        it.s_out = _p(so) if nt else None
+        it.vs_out = _p(vo) if nt else None
+    L = _lib.lib()
+    rc = L.igr_batch_run_sensitivity(items, n, int(n_iter), code, form)
+    if rc != 0:
+        bad = next((k for k in range(n) if items[k].status != 0), 0)
+        runs[bad]["ctx"].check(items[bad].status or rc)
+        raise IGRError(f"batched run failed with status {rc}")
+    return [(outs[k][0], outs[k][1], items[k].n_steps) for k in range(n)]
+
+
 OBJECTIVES = {"l2_density": 0, "l2_pressure": 1, "kinetic_energy": 2, "windowed_kinetic_energy": 3,
               "weighted_momentum": 4, "windowed_density": 5, "disc_kinetic_energy": 6}
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <cmath>
 #include <string>
+#include <vector>
 #include "igr_ctx.hpp"
 #include "k1_common.hpp"
 #include "k2_common.hpp"
@@ -43,6 +44,20 @@ inline void destroy_graphs(Ctx& c) {
   }
   c.graphs.clear();
 }
+
+// Tangent states of a context (tangent.hip): state, two stage buffers and
+// varsigma per tangent direction.
+struct TanBufs {
+  int nt = 0;
+  std::vector<double*> s, s1, s2, vs;
+};
+int tan_bufs(Ctx& c, int nt, TanBufs*& T);
+void sens2_begin(Ctx& c, TanBufs& T, int nt);
+void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alpha, int n_iter, int solver);
+void sens2_end(Ctx& c, TanBufs& T, int nt, double gamma, double alpha, int n_iter, int solver);
+int sens2_steps(double dt, double t_end, double& dts);
+int sens1_args(Ctx& c, TanBufs& T, int n_tan, double dt, double t_end, double gamma, double alpha, int n_iter,
+               int solver, int form, double sgam, K1Args& a, int& n_steps);
 
 // defined in capi.hip
 void step2(Ctx& c, double dt, double gamma, double alpha, int n_iter, int solver);

@@ -19,6 +19,8 @@ struct K1Args {
   double cheb_lmax = 2.1;
   int n_steps = 0;
   double dt = 0.0;
+  int n_steps2 = 0;        // OP_RUN: a second run of n_steps2 steps of dt2 after the first
+  double dt2 = 0.0;
   int zero_sigma = 1, final_solve = 0;
   double* q = nullptr;     // state (3N): mu, rho, E
   double* q1 = nullptr;

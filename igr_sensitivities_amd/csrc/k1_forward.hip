@@ -412,8 +412,8 @@ __device__ void k1_body(const K1Args& a) {
     }
     __syncthreads();
   }
-  const double dt = a.dt;
-  for (int step = 0; step < a.n_steps; ++step) {
+  for (int step = 0; step < a.n_steps + a.n_steps2; ++step) {
+    const double dt = step < a.n_steps ? a.dt : a.dt2;
     const double* st[3] = {a.q, a.q1, a.q2};
     double* ou[3] = {a.q1, a.q2, a.q};
     const double cd[3] = {dt, 1.0 / 4.0 * dt, 2.0 / 3.0 * dt};
@@ -439,9 +439,15 @@ __device__ void k1_body(const K1Args& a) {
 // Single instance: arguments by value in the kernel-argument segment.
 template <int NP>
 __global__ __launch_bounds__(1024) void k1_run(K1Args a) { k1_body<NP>(a); }
-// Batch: one workgroup per instance, arguments from a device array.
+// Batch: the device array is the work list (instances sorted by decreasing
+// cost); workgroup b runs entries b, b + gridDim.x, ... one after another.
 template <int NP>
-__global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ args) { k1_body<NP>(args[blockIdx.x]); }
+__global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ args, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    k1_body<NP>(args[k]);
+    __syncthreads();
+  }
+}
 
 void launch1_run(const Ctx& c, const K1Args& a, int block) {
   switch (c.np) {
@@ -451,12 +457,12 @@ void launch1_run(const Ctx& c, const K1Args& a, int block) {
     case 5: hipLaunchKernelGGL(k1_run<5>, dim3(1), dim3(block), 0, c.stream, a); break;
   }
 }
-void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int block) {
+void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int n_wg, int block) {
   switch (np) {
-    case 2: hipLaunchKernelGGL(k1_run_batch<2>, dim3(n_inst), dim3(block), 0, st, dargs); break;
-    case 3: hipLaunchKernelGGL(k1_run_batch<3>, dim3(n_inst), dim3(block), 0, st, dargs); break;
-    case 4: hipLaunchKernelGGL(k1_run_batch<4>, dim3(n_inst), dim3(block), 0, st, dargs); break;
-    case 5: hipLaunchKernelGGL(k1_run_batch<5>, dim3(n_inst), dim3(block), 0, st, dargs); break;
+    case 2: hipLaunchKernelGGL(k1_run_batch<2>, dim3(n_wg), dim3(block), 0, st, dargs, n_inst); break;
+    case 3: hipLaunchKernelGGL(k1_run_batch<3>, dim3(n_wg), dim3(block), 0, st, dargs, n_inst); break;
+    case 4: hipLaunchKernelGGL(k1_run_batch<4>, dim3(n_wg), dim3(block), 0, st, dargs, n_inst); break;
+    case 5: hipLaunchKernelGGL(k1_run_batch<5>, dim3(n_wg), dim3(block), 0, st, dargs, n_inst); break;
   }
 }
 

@@ -78,15 +78,28 @@ struct SensArgs {
   int* flag;
 };
 
+// Batched launches over many meshes: workgroup b of the launch serves
+// work[b] = (instance, workgroup of that instance), and the instance's
+// arguments are entry `instance` of a device table.
+template <int TX, int TY>
+__host__ __device__ inline int tiles_of(const Geo2& g) {
+  return ((g.Nex + TX - 1) / TX) * ((g.Ney + TY - 1) / TY);
+}
+struct FinArgs { const double* part; int nb; double* sc; int nit; };   // end of a PCG solve
+struct FillArgs { double* x; size_t n; double v; };
+struct CopyArgs { double* dst; const double* src; size_t n; };
+
 // Tile of TX x TY elements plus a one-element halo ring (no corners).
 // LDS slots: tile s = ly*TX + lx, then left[TY], right[TY], bottom[TX], top[TX].
 template <int NP, int TX, int TY>
 struct TileInfo {
   static constexpr int T = TX * TY, NP2 = NP * NP;
   int Nex, Ney, ex0, ey0, nvx, nvy;
-  __device__ TileInfo(const Geo2& g) {
+  __device__ TileInfo(const Geo2& g) : TileInfo(g, blockIdx.x, gridDim.x) {}
+  // workgroup blk of the nblk workgroups that cover this mesh
+  __device__ TileInfo(const Geo2& g, int blk, int nblk) {
     const int ntx = (g.Nex + TX - 1) / TX;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = xcd_tile(blk, nblk);
     const int tx = tile % ntx, ty = tile / ntx;
     Nex = g.Nex; Ney = g.Ney;
     ex0 = tx * TX; ey0 = ty * TY;

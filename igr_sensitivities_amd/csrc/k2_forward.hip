@@ -28,9 +28,11 @@
 //                  SSP-RK3 stage combination (forward.jl:170-188), each face
 //                  LLF flux computed once per tile (euler.jl:222-258).
 // Dot products finish with the deterministic last-workgroup reduction.
+#include <algorithm>
 #include <hip/hip_ext.h>
 #include "igr_ctx.hpp"
 #include "k2_common.hpp"
+#include "launch_rec.hpp"
 
 namespace igr {
 
@@ -38,7 +40,7 @@ namespace igr {
 // SIP family
 // ------------------------------------------------------------------------
 template <int NP, int TX, int TY, int MODE>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
+__device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool INIT = (MODE == M_INIT_PCG || MODE == M_INIT_CHEB || MODE == M_INIT_JAC);
   __shared__ double vs[NS * NP2];
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
-  TileInfo<NP, TX, TY> t(g);
+  TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
   const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
   const int lx = s % TX, ly = s / TX;
@@ -62,10 +64,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
   if constexpr (MODE == M_PCG_A) {
     // rz_it from the previous kernel's partial sums; beta_{it-1} = rz_it / (rz_{it-1} + 1e-30)
     double tot[1];
-    grid_total<1>(a.part_in, gridDim.x, tot, red);
+    grid_total<1>(a.part_in, nblk, tot, red);
     const int nit = a.n_iter, it = a.it;
     if (!a.first) beta = tot[0] / (a.sc[it - 1] + 1e-30);
-    if (blockIdx.x == 0 && tid == 0) {
+    if (blk == 0 && tid == 0) {
       a.sc[it] = tot[0];
       if (!a.first) a.sc[3 * nit + 1 + it - 1] = beta;
     }
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
         a.b_out[gme] = bval;
       }
     }
-    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
+    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
   } else if constexpr (MODE == M_PCG_A) {
     double red_v[1] = {0.0};
     if (valid) {
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
       a.y[gme] = y;
       red_v[0] = vown * y;
     }
-    block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
+    block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
   } else if constexpr (MODE == M_JACOBI) {
     if (valid) a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
   } else if constexpr (MODE == M_CHEB) {
@@ -264,13 +266,25 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
     }
   }
 }
+template <int NP, int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip(SipArgs a) {
+  k2_sip_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x);
+}
+template <int NP, int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs* __restrict__ tab,
+                                                                  const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const SipArgs& a = tab[w.x];
+  k2_sip_body<NP, TX, TY, MODE>(a, w.y, tiles_of<TX, TY>(a.g));
+}
+
 
 // x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
 template <int NP, int TX, int TY>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
+__device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
   __shared__ double red[NT / WAVE + 1];
-  TileInfo<NP, TX, TY> t(a.g);
+  TileInfo<NP, TX, TY> t(a.g, blk, nblk);
   const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
   const int lx = s % TX, ly = s / TX;
   const bool valid = lx < t.nvx && ly < t.nvy;
@@ -284,9 +298,9 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
     x = a.xout[gi]; p = a.pold[gi]; r = a.r ? a.r[gi] : a.rout[gi]; ap = a.y[gi]; im = a.invM[gi];
   }
   double tot[1];
-  grid_total<1>(a.part_in, gridDim.x, tot, red);
+  grid_total<1>(a.part_in, nblk, tot, red);
   const double ak = sc[it] / (tot[0] + 1e-30);
-  if (blockIdx.x == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
+  if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
   double red_v[1] = {0.0};
   if (valid) {
     a.xout[gi] = x + ak * p;
@@ -294,11 +308,23 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
     a.rout[gi] = rn;
     red_v[0] = rn * (rn * im);
   }
-  block_partials_store<1>(red_v, red, a.part_out, gridDim.x, blockIdx.x);
+  block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+}
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
+  k2_pcg_b_body<NP, TX, TY>(a, blockIdx.x, gridDim.x);
+}
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b_batch(const SipArgs* __restrict__ tab,
+                                                                    const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const SipArgs& a = tab[w.x];
+  k2_pcg_b_body<NP, TX, TY>(a, w.y, tiles_of<TX, TY>(a.g));
 }
 
+
 // After the last PCG iteration: rz_n and beta_{n-1} into the scalar record.
-__global__ void k2_pcg_final(const double* part, int nb, double* sc, int nit) {
+__device__ __forceinline__ void k2_pcg_final_body(const double* part, int nb, double* sc, int nit) {
   __shared__ double red[16];
   double tot[1];
   grid_total<1>(part, nb, tot, red);
@@ -308,6 +334,13 @@ __global__ void k2_pcg_final(const double* part, int nb, double* sc, int nit) {
   }
   if (threadIdx.x == 0 && nit == 0) sc[0] = tot[0];
 }
+__global__ void k2_pcg_final(const double* part, int nb, double* sc, int nit) { k2_pcg_final_body(part, nb, sc, nit); }
+// one workgroup per instance
+__global__ void k2_pcg_final_batch(const FinArgs* __restrict__ tab) {
+  const FinArgs& f = tab[blockIdx.x];
+  k2_pcg_final_body(f.part, f.nb, f.sc, f.nit);
+}
+
 
 // ------------------------------------------------------------------------
 // Hyperbolic RHS + SSP-RK3 stage update.  Built without FMA contraction and
@@ -316,7 +349,7 @@ __global__ void k2_pcg_final(const double* part, int nb, double* sc, int nit) {
 // ------------------------------------------------------------------------
 #pragma clang fp contract(off)
 template <int NP, int TX, int TY>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp(HypArgs a) {
+__device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY, NH = 2 * (TX + TY), NT = T * NP2;
   __shared__ double qs[5][T * NP2];
   __shared__ double Fx[4][T * NP2], Fy[4][T * NP2];
@@ -327,7 +360,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp(HypArgs a) {
   const Geo2 g = a.g;
   const BasisK& B = a.B;
   const double gam = a.gamma;
-  TileInfo<NP, TX, TY> t(g);
+  TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
   const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
   const int lx = s % TX, ly = s / TX;
@@ -481,6 +514,18 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp(HypArgs a) {
     a.out[gi] = o;
   }
 }
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp(HypArgs a) {
+  k2_hyp_body<NP, TX, TY>(a, blockIdx.x, gridDim.x);
+}
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_batch(const HypArgs* __restrict__ tab,
+                                                                  const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const HypArgs& a = tab[w.x];
+  k2_hyp_body<NP, TX, TY>(a, w.y, tiles_of<TX, TY>(a.g));
+}
+
 #pragma clang fp contract(fast)
 
 // Plain fill (used instead of hipMemsetAsync so the step can be captured in
@@ -492,6 +537,17 @@ __global__ void k_fill(double* x, size_t n, double v) {
 __global__ void k_copy(double* dst, const double* src, size_t n) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) dst[k] = src[k];
+}
+// One fill / copy per blockIdx.y, grid-stride over x.
+__global__ void k_fill_batch(const FillArgs* __restrict__ tab) {
+  const FillArgs& f = tab[blockIdx.y];
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < f.n; k += (size_t)gridDim.x * blockDim.x)
+    f.x[k] = f.v;
+}
+__global__ void k_copy_batch(const CopyArgs* __restrict__ tab) {
+  const CopyArgs& f = tab[blockIdx.y];
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < f.n; k += (size_t)gridDim.x * blockDim.x)
+    f.dst[k] = f.src[k];
 }
 
 // ------------------------------------------------------------------------
@@ -536,9 +592,58 @@ struct Launch2 {
   static void hyp(const HypArgs& a, hipStream_t st) {
     launch_k(k2_hyp<NP, TX, TY>, grid(a.g), block(), st, a);
   }
+  static void sip_batch(int mode, const SipArgs* tab, const int2* work, int nb, hipStream_t st) {
+    const dim3 G(nb), Bk = block();
+    switch (mode) {
+      case M_APPLY: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_APPLY>), G, Bk, 0, st, tab, work); break;
+      case M_INIT_PCG: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_INIT_PCG>), G, Bk, 0, st, tab, work); break;
+      case M_INIT_CHEB: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_INIT_CHEB>), G, Bk, 0, st, tab, work); break;
+      case M_INIT_JAC: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_INIT_JAC>), G, Bk, 0, st, tab, work); break;
+      case M_PCG_A: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_PCG_A>), G, Bk, 0, st, tab, work); break;
+      case M_JACOBI: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_JACOBI>), G, Bk, 0, st, tab, work); break;
+      case M_CHEB: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_CHEB>), G, Bk, 0, st, tab, work); break;
+      case M_PCG_B: hipLaunchKernelGGL((k2_pcg_b_batch<NP, TX, TY>), G, Bk, 0, st, tab, work); break;
+    }
+  }
+  static void hyp_batch(const HypArgs* tab, const int2* work, int nb, hipStream_t st) {
+    hipLaunchKernelGGL((k2_hyp_batch<NP, TX, TY>), dim3(nb), block(), 0, st, tab, work);
+  }
 };
 
+// (np, tile) -> the instantiated tile shape; p = 3 (np = 4) has three tile shapes
+#define IGR_TILE_DISPATCH(np, tx, ty, CALL)                             \
+  switch (np) {                                                         \
+    case 2: Launch2<2, 8, 8>::CALL; break;                              \
+    case 3: Launch2<3, 4, 4>::CALL; break;                              \
+    case 4:                                                             \
+      if ((tx) == 8 && (ty) == 4) Launch2<4, 8, 4>::CALL;               \
+      else if ((tx) == 2 && (ty) == 2) Launch2<4, 2, 2>::CALL;          \
+      else Launch2<4, 4, 4>::CALL;                                      \
+      break;                                                            \
+    case 5: Launch2<5, 2, 4>::CALL; break;                              \
+  }
+
+void launch2_sip_batch(int np, int tx, int ty, int mode, const SipArgs* tab, const int2* work, int nblocks,
+                       hipStream_t st) {
+  IGR_TILE_DISPATCH(np, tx, ty, sip_batch(mode, tab, work, nblocks, st));
+}
+void launch2_hyp_batch(int np, int tx, int ty, const HypArgs* tab, const int2* work, int nblocks, hipStream_t st) {
+  IGR_TILE_DISPATCH(np, tx, ty, hyp_batch(tab, work, nblocks, st));
+}
+void launch_pcg_final_batch(const FinArgs* tab, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k2_pcg_final_batch, dim3(n), dim3(256), 0, st, tab);
+}
+void launch_fill_batch(const FillArgs* tab, int n, size_t maxn, hipStream_t st) {
+  const unsigned gx = (unsigned)std::min<size_t>((maxn + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_fill_batch, dim3(std::max(gx, 1u), n), dim3(256), 0, st, tab);
+}
+void launch_copy_batch(const CopyArgs* tab, int n, size_t maxn, hipStream_t st) {
+  const unsigned gx = (unsigned)std::min<size_t>((maxn + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_copy_batch, dim3(std::max(gx, 1u), n), dim3(256), 0, st, tab);
+}
+
 void launch2_sip(const Ctx& c, int mode, const SipArgs& a) {
+  if (g_rec) { g_rec->recs.push_back({R_SIP, mode, g_rec->sip.size()}); g_rec->sip.push_back(a); return; }
   switch (c.np) {
     case 2: Launch2<2, 8, 8>::sip(mode, a, c.stream); break;
     case 3: Launch2<3, 4, 4>::sip(mode, a, c.stream); break;
@@ -551,6 +656,7 @@ void launch2_sip(const Ctx& c, int mode, const SipArgs& a) {
   }
 }
 void launch2_hyp(const Ctx& c, const HypArgs& a) {
+  if (g_rec) { g_rec->recs.push_back({R_HYP, 0, g_rec->hyp.size()}); g_rec->hyp.push_back(a); return; }
   switch (c.np) {
     case 2: Launch2<2, 8, 8>::hyp(a, c.stream); break;
     case 3: Launch2<3, 4, 4>::hyp(a, c.stream); break;
@@ -570,12 +676,15 @@ int grid2_blocks(const Ctx& c) {
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
 void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit) {
+  if (g_rec) { g_rec->recs.push_back({R_FINAL, 0, g_rec->fin.size()}); g_rec->fin.push_back({part, nb, sc, nit}); return; }
   hipLaunchKernelGGL(k2_pcg_final, dim3(1), dim3(256), 0, c.stream, part, nb, sc, nit);
 }
 void launch_fill(const Ctx& c, double* x, size_t n, double v) {
+  if (g_rec) { g_rec->recs.push_back({R_FILL, 0, g_rec->fill.size()}); g_rec->fill.push_back({x, n, v}); return; }
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, x, n, v);
 }
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n) {
+  if (g_rec) { g_rec->recs.push_back({R_COPY, 0, g_rec->copy.size()}); g_rec->copy.push_back({dst, src, n}); return; }
   hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, dst, src, n);
 }
 

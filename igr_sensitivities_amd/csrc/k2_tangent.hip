@@ -12,17 +12,18 @@
 //                 evaluation order without FMA contraction, like k2_hyp.
 #include "igr_ctx.hpp"
 #include "k2_common.hpp"
+#include "launch_rec.hpp"
 
 namespace igr {
 
 
 template <int NP, int TX, int TY>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs(SensArgs a) {
+__device__ __forceinline__ void k2_sens_rhs_body(const SensArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY;
   __shared__ double uu[T * NP2], vv[T * NP2], su[T * NP2], sv[T * NP2], Ss[T * NP2], sr[T * NP2];
   const Geo2 g = a.g;
   const BasisK& B = a.B;
-  TileInfo<NP, TX, TY> t(g);
+  TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
   const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
   const int lx = s % TX, ly = s / TX;
@@ -67,6 +68,18 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs(SensArgs a) {
   const double bpt = Rs + sr[tid] * Rp - al * (dxs * Sx + dys * Sy) / r;
   a.out[gi] = B.w[i] * B.w[j] * g.Jx * g.Jy * bpt;
 }
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs(SensArgs a) {
+  k2_sens_rhs_body<NP, TX, TY>(a, blockIdx.x, gridDim.x);
+}
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs_batch(const SensArgs* __restrict__ tab,
+                                                                  const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const SensArgs& a = tab[w.x];
+  k2_sens_rhs_body<NP, TX, TY>(a, w.y, tiles_of<TX, TY>(a.g));
+}
+
 
 #pragma clang fp contract(off)
 // sensitivity_flux_x / _y with _pressure_sensitivity (sensitivity.jl:34-76).
@@ -96,7 +109,7 @@ __device__ __forceinline__ double pwave(int dir, double gam, const double q[5], 
 }
 
 template <int NP, int TX, int TY>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_hyp(SensArgs a) {
+__device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY, NH = 2 * (TX + TY), NT = T * NP2;
   __shared__ double qs[5][T * NP2], ss[5][T * NP2];
   __shared__ double Fx[4][T * NP2], Fy[4][T * NP2];
@@ -106,7 +119,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_hyp(SensArgs a) {
   const Geo2 g = a.g;
   const BasisK& B = a.B;
   const double gam = a.gamma;
-  TileInfo<NP, TX, TY> t(g);
+  TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
   const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
   const int lx = s % TX, ly = s / TX;
@@ -233,6 +246,18 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_hyp(SensArgs a) {
     a.out[gi] = o;
   }
 }
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_hyp(SensArgs a) {
+  k2_sens_hyp_body<NP, TX, TY>(a, blockIdx.x, gridDim.x);
+}
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_hyp_batch(const SensArgs* __restrict__ tab,
+                                                                  const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const SensArgs& a = tab[w.x];
+  k2_sens_hyp_body<NP, TX, TY>(a, w.y, tiles_of<TX, TY>(a.g));
+}
+
 #pragma clang fp contract(fast)
 
 template <int NP, int TX, int TY>
@@ -244,19 +269,39 @@ struct LaunchSens {
   static void hyp(const SensArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k2_sens_hyp<NP, TX, TY>), grid(a.g), dim3(TX * TY * NP * NP), 0, st, a);
   }
+  static void rhs_batch(const SensArgs* tab, const int2* work, int nb, hipStream_t st) {
+    hipLaunchKernelGGL((k2_sens_rhs_batch<NP, TX, TY>), dim3(nb), dim3(TX * TY * NP * NP), 0, st, tab, work);
+  }
+  static void hyp_batch(const SensArgs* tab, const int2* work, int nb, hipStream_t st) {
+    hipLaunchKernelGGL((k2_sens_hyp_batch<NP, TX, TY>), dim3(nb), dim3(TX * TY * NP * NP), 0, st, tab, work);
+  }
 };
-#define IGR_SENS_DISPATCH(c, CALL)                                              \
-  switch ((c).np) {                                                             \
+#define IGR_SENS_DISPATCH(np, tx, ty, CALL)                                     \
+  switch (np) {                                                                 \
     case 2: LaunchSens<2, 8, 8>::CALL; break;                                   \
     case 3: LaunchSens<3, 4, 4>::CALL; break;                                   \
     case 4:                                                                     \
-      if ((c).tile_x == 8 && (c).tile_y == 4) LaunchSens<4, 8, 4>::CALL;       \
-      else if ((c).tile_x == 2 && (c).tile_y == 2) LaunchSens<4, 2, 2>::CALL;  \
+      if ((tx) == 8 && (ty) == 4) LaunchSens<4, 8, 4>::CALL;                    \
+      else if ((tx) == 2 && (ty) == 2) LaunchSens<4, 2, 2>::CALL;               \
       else LaunchSens<4, 4, 4>::CALL;                                           \
       break;                                                                    \
     case 5: LaunchSens<5, 2, 4>::CALL; break;                                   \
   }
-void launch2_sens_rhs(const Ctx& c, const SensArgs& a) { IGR_SENS_DISPATCH(c, rhs(a, c.stream)); }
-void launch2_sens_hyp(const Ctx& c, const SensArgs& a) { IGR_SENS_DISPATCH(c, hyp(a, c.stream)); }
+void launch2_sens_rhs(const Ctx& c, const SensArgs& a) {
+  if (g_rec) { g_rec->recs.push_back({R_SENS_RHS, 0, g_rec->sens.size()}); g_rec->sens.push_back(a); return; }
+  IGR_SENS_DISPATCH(c.np, c.tile_x, c.tile_y, rhs(a, c.stream));
+}
+void launch2_sens_hyp(const Ctx& c, const SensArgs& a) {
+  if (g_rec) { g_rec->recs.push_back({R_SENS_HYP, 0, g_rec->sens.size()}); g_rec->sens.push_back(a); return; }
+  IGR_SENS_DISPATCH(c.np, c.tile_x, c.tile_y, hyp(a, c.stream));
+}
+void launch2_sens_rhs_batch(int np, int tx, int ty, const SensArgs* tab, const int2* work, int nblocks,
+                            hipStream_t st) {
+  IGR_SENS_DISPATCH(np, tx, ty, rhs_batch(tab, work, nblocks, st));
+}
+void launch2_sens_hyp_batch(int np, int tx, int ty, const SensArgs* tab, const int2* work, int nblocks,
+                            hipStream_t st) {
+  IGR_SENS_DISPATCH(np, tx, ty, hyp_batch(tab, work, nblocks, st));
+}
 
 }  // namespace igr

@@ -111,6 +111,33 @@ int igr_run_sensitivity(igr_ctx* ctx, int n_tan, const double* s0, double dt, do
                         double alpha, int n_iter, int solver, int form, double sgam, double* s_out,
                         double* vs_out, int* n_steps_out);
 
+/* One instance of a batched ensemble: the arguments of its own
+ * igr_run_sensitivity call.  status and n_steps are outputs. */
+typedef struct igr_batch_item {
+  igr_ctx* ctx;          /* the instance's mesh; its state set with igr_set_state */
+  double dt, t_end;      /* stepping as in igr_run_sensitivity */
+  double gamma, alpha;
+  double sgam;           /* 1D gamma source of the tangent flux (0 in 2D) */
+  int n_tan;             /* 0..4 tangent directions */
+  const double* s0;      /* n_tan*nv*N tangent initial states (host); NULL when n_tan = 0 */
+  double* s_out;         /* optional: n_tan*nv*N tangents s(T) */
+  double* vs_out;        /* optional: n_tan*N varsigma(T) */
+  int n_steps;           /* out: SSP-RK3 steps taken */
+  int status;            /* out: IGR_* of this instance */
+} igr_batch_item;
+
+/* Batched ensembles (no reference counterpart: the reference runs
+ * run_sensitivity (IGR_adjoints_2d/sensitivity.jl:353; IGR_adjoints_1d/
+ * sensitivity.jl:475) once per instance in a serial loop, e.g. the gamma and
+ * shift sweeps of its paper scripts).  Runs every item's run_sensitivity at
+ * once: the 1D items in one persistent launch driven by a work list, the 2D
+ * items as batched launches over (instance, tile) work items with each step
+ * replayed from a hipGraph.  Every item gets exactly the result its own
+ * igr_run_sensitivity call would give.  n_iter, solver and form (1D) are
+ * shared; each context may appear once.  Returns the first failing item's
+ * status (each item's own status in items[k].status). */
+int igr_batch_run_sensitivity(igr_batch_item* items, int n_items, int n_iter, int solver, int form);
+
 /* Objectives: replace the `objective::Symbol` of enzyme_adjoint_ic
  * (IGR_adjoints_2d/enzyme_adjoint.jl:240-254).  0..4 are the reference's
  * :l2_density, :l2_pressure, :kinetic_energy (KE / mass),
@@ -153,9 +180,10 @@ int igr_solve_elliptic_vjp(igr_ctx* ctx, double alpha, int n_iter, int solver, c
                            const double* sigmabar, double* qbar, double* sigma0bar, double* alphabar);
 
 /* Mean device time (ms) per launch of one hot kernel on the current state,
- * measured with HIP events around `reps` back-to-back launches on the
- * context stream.  kernel: 0 = fused SIP-apply + PCG direction update,
- * 1 = PCG x/r update, 2 = hyperbolic RHS + RK3 stage update. */
+ * each launch stamped with start/end events by hipExtLaunchKernelGGL on the
+ * context stream.  kernel 0 (fused SIP apply + PCG direction update) and 1
+ * (PCG x/r update) are timed inside a PCG solve of `reps` iterations;
+ * kernel 2 (hyperbolic RHS + RK3 stage update) over `reps` launches. */
 int igr_time_kernel(igr_ctx* ctx, int kernel, double gamma, double alpha, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus
