@@ -38,6 +38,59 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def ensemble_leg(igr, configs):
+    """C4: 64 uneven 1D/2D instances with 4 tangents each, one batched run
+    against the same runs one at a time; DOF-stage updates count the primal
+    and every tangent (nv * N * 3 * steps * (1 + n_tan))."""
+    c4 = configs.config4()
+    runs = c4["runs"]
+    ctxs = []
+    for r in runs:
+        mesh = (igr.PeriodicMesh1D(r["N"], 1.0, igr.DGBasis(3)) if r["dim"] == 1
+                else igr.CartesianMesh2D(r["N"], r["N"], 1.0, 1.0, igr.DGBasis(3)))
+        ctxs.append(igr.Context(mesh))
+    items = [dict(r, ctx=c) for r, c in zip(runs, ctxs)]
+
+    def batched():
+        for r, c in zip(runs, ctxs):
+            c.set_state(r["q0"])
+        t0 = time.perf_counter()
+        out = igr.run_sensitivity_batch(items, n_iter=c4["n_iter"], solver=c4["solver"])
+        return time.perf_counter() - t0, out
+
+    def sequential():
+        for r, c in zip(runs, ctxs):
+            c.set_state(r["q0"])
+        t0 = time.perf_counter()
+        out = [c.run_sensitivity(r["s0"], r["dt"], r["T"], r["gamma"], r["alpha"], c4["n_iter"], c4["solver"])
+               for r, c in zip(runs, ctxs)]
+        return time.perf_counter() - t0, out
+
+    def subset(dim):
+        sub = [it for it in items if it["dim"] == dim]
+        for it in sub:
+            it["ctx"].set_state(it["q0"])
+        t0 = time.perf_counter()
+        igr.run_sensitivity_batch(sub, n_iter=c4["n_iter"], solver=c4["solver"])
+        return time.perf_counter() - t0
+
+    batched()                                    # first call: code objects, tangent buffers
+    tb, ob = batched()
+    t1d, t2d = subset(1), subset(2)
+    ts, os_ = sequential()
+    same = all(np.array_equal(a[0], b[0]) for a, b in zip(ob, os_))
+    work = sum(r["q0"].shape[0] * r["q0"][0].size * 3 * r["n_steps"] * (1 + r["s0"].shape[0]) for r in runs)
+    for c in ctxs:
+        c.close()
+    return {
+        "workload": "64 instances: 1D N_e in {256..4096} and y-uniform 2D N^2, N in {32..128}, p=3, shifted 1D "
+                    "blast data, gamma ~ U[1.2,1.67], 4 tangent directions each, damped Jacobi-50, 3-6 steps",
+        "batched_s": tb, "batched_1d_only_s": t1d, "batched_2d_only_s": t2d, "sequential_s": ts, "speedup": ts / tb,
+        "batched_dof_stage_per_s": work / tb, "sequential_dof_stage_per_s": work / ts,
+        "tangents_equal_single_runs": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +98,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=8, help="SSP-RK3 steps of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by 2D block "
@@ -98,6 +152,8 @@ def main():
     g1 = time.perf_counter()
     t_grad = g1 - g0
 
+    ens = None if args.no_ensemble else ensemble_leg(igr, configs)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -148,6 +204,7 @@ def main():
         "gradient_over_forward": t_grad / elapsed,
         "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,
                      "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

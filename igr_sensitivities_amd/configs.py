@@ -13,6 +13,8 @@ here are the ones this library is measured on:
       Gaussian energy deposits (sigma=0.02, total E = 1, seeded centres),
       alpha = 10 (h/4)^2, PCG-60, 200 SSP-RK3 steps, CFL 0.4.
   C3  C2 with J = sum w w Jx Jy rho(T) exp(-|x - c_w|^2 / 0.1^2), c_w seeded.
+  C4  64 instances, 1D and y-uniform 2D of mixed sizes, shifted C1 data,
+      4 tangent directions each, run as one batch.
   C5  2D blast + Lamb-Oseen vortex, 256x256, PCG-60, 400 steps.
 
 The time step follows the reference scripts' formula
@@ -138,6 +140,82 @@ def config1(Ne=128, p=3, seed=1, gamma=1.4, width=0.01):
     dt = 0.1 * h / ((2 * p + 1) * lam)
     return dict(q=q, gamma=gamma, alpha=10.0 * h * h, dt=dt, T=0.038, n_iter=50, solver="jacobi",
                 p=p, Ne=Ne)
+
+
+def c1_pressure(x, seed=1, width=0.01, d=False):
+    """C1 pressure profile as a periodic function of x (and its x-derivative
+    when d=True): 1000 / 0.01 / 100 on [0,0.1) / [0.1,0.9) / [0.9,1] with tanh
+    edges of width `width`, times (1 + 1e-3 eta(x)), eta the seeded sum of 8
+    sine modes normalised to max|eta| = 1 on a 20001-point grid.  Returns
+    (p, base, eta) or, with d=True, (dp/dx, base * eta)."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, 9)
+    amp = rng.standard_normal(8)
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    xf = np.linspace(0.0, 1.0, 20001)
+    scale = np.max(np.abs(np.sum(amp[:, None] * np.sin(2 * np.pi * k[:, None] * xf[None] + ph[:, None]), axis=0)))
+    arg = 2 * np.pi * k[:, None] * x.ravel()[None] + ph[:, None]
+    eta = (np.sum(amp[:, None] * np.sin(arg), axis=0) / scale).reshape(x.shape)
+    deta = (np.sum(amp[:, None] * 2 * np.pi * k[:, None] * np.cos(arg), axis=0) / scale).reshape(x.shape)
+
+    def box(a, b):
+        return sum(0.5 * (np.tanh((x - a - j) / width) - np.tanh((x - b - j) / width)) for j in (-1, 0, 1))
+
+    def dbox(a, b):
+        return sum(0.5 / width * (np.cosh((x - a - j) / width) ** -2 - np.cosh((x - b - j) / width) ** -2)
+                   for j in (-1, 0, 1))
+
+    base = 0.01 + (1000.0 - 0.01) * box(0.0, 0.1) + (100.0 - 0.01) * box(0.9, 1.0)
+    if d:
+        dbase = (1000.0 - 0.01) * dbox(0.0, 0.1) + (100.0 - 0.01) * dbox(0.9, 1.0)
+        return dbase * (1.0 + 1e-3 * eta) + base * 1e-3 * deta, base * eta
+    return base * (1.0 + 1e-3 * eta), base, eta
+
+
+def config4(n_inst=64, seed=4, p=3, steps=(3, 6), n_tan=4):
+    """C4 batched ensemble: n_inst instances, alternately 1D (N_e in
+    {256, ..., 4096}) and 2D (N x N, N in {32, ..., 128}, y-uniform), of the
+    C1 data cyclically shifted by a seeded s in [0, 1), gamma ~ U[1.2, 1.67],
+    alpha = 10 h^2, CFL 0.1, a seeded number of SSP-RK3 steps in `steps`,
+    damped Jacobi-50.  Tangent directions: d/ds (the shift, -dq0/dx), d/d eps
+    (the noise amplitude 1e-3) and two seeded smooth random fields."""
+    rng = np.random.default_rng(seed)
+    runs = []
+    for k in range(n_inst):
+        dim = 1 if k % 2 == 0 else 2
+        N = int(rng.choice([256, 512, 1024, 2048, 4096])) if dim == 1 else int(rng.choice([32, 48, 64, 96, 128]))
+        gam = float(rng.uniform(1.2, 1.67))
+        sh = float(rng.uniform(0.0, 1.0))
+        ns = int(rng.integers(steps[0], steps[1] + 1))
+        if dim == 1:
+            x = coords1(p, N)
+            shape = x.shape
+        else:
+            x, _ = coords2(p, N, N)
+            shape = x.shape
+        xs = (x - sh) % 1.0
+        pr, _, _ = c1_pressure(xs)
+        dp, beta = c1_pressure(xs, d=True)
+        nv = dim + 2
+        q = np.zeros((nv,) + shape)
+        q[nv - 2] = 1.0
+        q[nv - 1] = pr / (gam - 1.0)
+        s0 = np.zeros((n_tan, nv) + shape)
+        s0[0, nv - 1] = -dp / (gam - 1.0)
+        if n_tan > 1:
+            s0[1, nv - 1] = beta / (gam - 1.0)
+        for t in range(2, n_tan):
+            for v in range(nv):
+                f = np.zeros(shape)
+                for m in range(1, 4):
+                    f += rng.standard_normal() * np.cos(2 * np.pi * m * x + rng.uniform(0, 2 * np.pi))
+                s0[t, v] = 0.1 * f / 3.0
+        h = 1.0 / N
+        lam = max_wavespeed1(q, gam) if dim == 1 else max_wavespeed2(q, gam)
+        dt = 0.1 * h / ((2 * p + 1) * lam)
+        runs.append(dict(dim=dim, N=N, q0=q, s0=s0, gamma=gam, alpha=10.0 * h * h, dt=dt, T=(ns - 0.5) * dt,
+                         n_steps=ns, shift=sh))
+    return dict(runs=runs, n_iter=50, solver="jacobi", p=p)
 
 
 def config2(N=128, p=3, seed=2, gamma=1.4, n_steps=200, cfl=0.4, sigma=0.02, p_bg=1.0):
