@@ -495,7 +495,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
     h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;
     launch_copy(*c, c->q1, c->q, c->nv * N);
     std::vector<hipEvent_t> ev(2 * (size_t)reps);
-    for (auto& e : ev) HIPCHK(*c, hipEventCreate(&e));
+    for (auto& e : ev) HIPCHK(*c, hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
     for (int w = 0; w < 3; ++w) launch2_hyp(*c, h);
     for (int r = 0; r < reps; ++r) {
       set_kernel_timer(ev[2 * r], ev[2 * r + 1]);
@@ -525,7 +525,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
   a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = reps; a.part_out = R0;
   launch2_sip(*c, M_INIT_PCG, a);
   std::vector<hipEvent_t> ev(2 * (size_t)reps);
-  for (auto& e : ev) HIPCHK(*c, hipEventCreate(&e));
+  for (auto& e : ev) HIPCHK(*c, hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
   double* pbuf[2] = {c->pA, c->pB};
   for (int it = 0; it < reps; ++it) {
     SipArgs k = sip_base(*c, alpha);

@@ -62,7 +62,22 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   double own_d = 0.0;
   double beta = 0.0;
   if constexpr (MODE == M_PCG_A) {
-    // rz_it from the previous kernel's partial sums; beta_{it-1} = rz_it / (rz_{it-1} + 1e-30)
+    // Load z = r/M, p_old and 1/rho of the tile and halo first, so these
+    // loads are in flight together with the partial sums; then
+    // beta_{it-1} = rz_it / (rz_{it-1} + 1e-30) and p = z + beta p_old in LDS.
+    constexpr int NL = (NS * NP2 + NT - 1) / NT;
+    double lz[NL], lp[NL], lr[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int idx = tid + l * NT;
+      lz[l] = 0.0; lp[l] = 0.0; lr[l] = 0.0;
+      size_t gi;
+      if (idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi)) {
+        lz[l] = a.r[gi] * a.invM[gi];
+        if (!a.first) lp[l] = a.pold[gi];
+        lr[l] = a.invR[gi];
+      }
+    }
     double tot[1];
     grid_total<1>(a.part_in, nblk, tot, red);
     const int nit = a.n_iter, it = a.it;
@@ -71,9 +86,17 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       a.sc[it] = tot[0];
       if (!a.first) a.sc[3 * nit + 1 + it - 1] = beta;
     }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int idx = tid + l * NT;
+      if (idx < NS * NP2) {
+        vs[idx] = a.first ? lz[l] : lz[l] + beta * lp[l];
+        irs[idx] = lr[l];
+      }
+    }
   }
   // ---- stage the vector, 1/rho (and velocities for INIT) in LDS
-  for (int idx = tid; idx < NS * NP2; idx += NT) {
+  for (int idx = tid; MODE != M_PCG_A && idx < NS * NP2; idx += NT) {
     const int sl = idx / NP2, nn = idx % NP2;
     double v = 0.0, ir = 0.0;
     size_t gi;
@@ -85,10 +108,6 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         double rh = a.rho[gi];
         ir = 1.0 / rh;
         if (sl < T) { us[idx] = a.mx[gi] / rh; ws_[idx] = a.my[gi] / rh; }
-      } else if constexpr (MODE == M_PCG_A) {
-        double z = a.r[gi] * a.invM[gi];
-        v = a.first ? z : z + beta * a.pold[gi];
-        ir = a.invR[gi];
       } else if constexpr (MODE == M_JACOBI) {
         v = a.x[gi]; ir = a.invR[gi];
       } else if constexpr (MODE == M_CHEB) {

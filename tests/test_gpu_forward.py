@@ -277,3 +277,38 @@ def test_mirror_api_run_forward(orc):
     assert rel(q, qr) < 1e-10 and rel(S, Sr) < 1e-7
     with pytest.raises(ValueError):
         igr.run_forward_(q, S, 2e-3, 0.02, basis, mesh, 1.4, 3e-3, 10, ":gmres")
+
+
+@pytest.mark.parametrize("alpha_on", [False, True])
+def test_quasi1d_gpu_2d_equals_gpu_1d(alpha_on):
+    """A y-uniform 2D run reproduces the 1D run on the GPU path
+    (IGR_adjoints_2d/tests/run_quasi1d_test.jl:27-98: deviation <= 1e-9,
+    y-nonuniformity <= 1e-11, PCG-400 when alpha > 0)."""
+    print(f"[case] quasi-1D GPU alpha_on={alpha_on}", flush=True)
+    gam, L, p, Ne, A, T, CFL = 1.4, 1.0, 3, 12, 0.4, 0.08, 0.5
+    Ney, Ly = 4, 0.5
+    basis = igr.DGBasis(p)
+    c1 = igr.Context(igr.PeriodicMesh1D(Ne, L, basis))
+    c2 = igr.Context(igr.CartesianMesh2D(Ne, Ney, L, Ly, basis))
+    x1 = configs.coords1(p, Ne, L)
+    u1 = A * np.sin(2 * np.pi * x1 / L)
+    q1 = np.stack([u1, np.ones_like(u1), 1.0 / (gam - 1) + 0.5 * u1 ** 2])
+    x2, _ = configs.coords2(p, Ne, Ney, L, Ly)
+    u2 = A * np.sin(2 * np.pi * x2 / L)
+    q2 = np.stack([u2, np.zeros_like(u2), np.ones_like(u2), 1.0 / (gam - 1) + 0.5 * u2 ** 2])
+    dt = CFL * (L / Ne) / ((2 * p + 1) * configs.max_wavespeed1(q1, gam))
+    alpha = 3.0 * (L / Ne) ** 2 if alpha_on else 0.0
+    n_iter = 400 if alpha_on else 1
+    c1.set_state(q1)
+    c2.set_state(q2)
+    n1 = c1.run_forward(dt, T, gam, alpha, n_iter, "pcg")
+    n2 = c2.run_forward(dt, T, gam, alpha, n_iter, "pcg")
+    assert n1 == n2
+    f1, S1 = c1.get_state()
+    f2, S2 = c2.get_state()
+    pairs = [(f2[0], f1[0]), (f2[2], f1[1]), (f2[3], f1[2])] + ([(S2, S1)] if alpha_on else [])
+    for a2, a1 in pairs:   # a2 (Ney, Nex, j, i), a1 (Ne, i)
+        dev = np.max(np.abs(a2 - a1[None, :, None, :]))
+        ynu = np.max(np.abs(a2 - a2[:1, :, :1, :]))
+        assert dev < 1e-9 and ynu < 1e-11, (dev, ynu)
+    assert np.max(np.abs(f2[1])) < 1e-12
