@@ -1,5 +1,5 @@
 // Host driver of the 2D discrete adjoint: forward recompute of one step with
-// every PCG iterate stored, the reverse SSP-RK3 sweep, binomial
+// every solver iterate stored (PCG, damped Jacobi or Chebyshev), the reverse SSP-RK3 sweep, binomial
 // checkpointing of time steps, and the C-ABI gradient / VJP entry points.
 //
 // The reference differentiates its whole forward map with Enzyme reverse
@@ -33,11 +33,15 @@ int grid2_blocks(const Ctx& c);
 Geo2 geo2(const Ctx& c);
 SipArgs sip_base(const Ctx& c, double alpha);
 HypArgs hyp_base(const Ctx& c, double gamma);
+double cheb_lmax(int p);
 
 
 // Storage of one stage's solve, written by the recompute and read by the reverse.
 struct StageStore {
+  // PCG: P = p_k, AP = A p_k (k < n), R = r_k (k <= n).
+  // Jacobi: P = x_k, AP = A x_k (k < n).  Chebyshev: P = x_k at k-1 (1 <= k <= n), R = r_k (k < n).
   double *P = nullptr, *AP = nullptr, *R = nullptr;   // n, n, n+1 fields
+  double* b = nullptr;                                 // IGR source (Jacobi/Chebyshev)
   double *invR = nullptr, *invM = nullptr;
   double *sc = nullptr, *rs = nullptr;
 };
@@ -51,7 +55,8 @@ struct GradWork {
   double *QBn = nullptr, *QBa = nullptr, *QBb = nullptr, *QBnext = nullptr;
   double *Xbar = nullptr, *Scarry = nullptr;
   double *RB[2] = {nullptr, nullptr}, *PB[2] = {nullptr, nullptr};
-  double *irbar = nullptr, *imbar = nullptr;
+  double *irbar = nullptr, *imbar = nullptr, *bbar = nullptr;
+  double *XB[2] = {nullptr, nullptr};        // Chebyshev x-bar ping-pong
   double *gblk = nullptr, *ablk = nullptr;   // per-workgroup accumulators
   double* dsc = nullptr;                     // [0] J, [1] KE, [2] M, [3] gamma-bar, [4] alpha-bar
   std::vector<double*> ck;                   // checkpoint slots (5N each)
@@ -71,14 +76,15 @@ static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
       st.P = c.alloc(std::max<size_t>((size_t)n * N, 1));
       st.AP = c.alloc(std::max<size_t>((size_t)n * N, 1));
       st.R = c.alloc((size_t)(n + 1) * N);
-      st.invR = c.alloc(N); st.invM = c.alloc(N);
+      st.invR = c.alloc(N); st.invM = c.alloc(N); st.b = c.alloc(N);
       st.sc = c.alloc(4 * (size_t)std::max(n, 1) + 1);
       st.rs = c.alloc(5 * (size_t)std::max(n, 1) + 1);
-      if (!st.P || !st.AP || !st.R || !st.invR || !st.invM || !st.sc || !st.rs) return IGR_ENOMEM;
+      if (!st.P || !st.AP || !st.R || !st.invR || !st.invM || !st.b || !st.sc || !st.rs) return IGR_ENOMEM;
     }
     for (int s = 0; s < 3; ++s) if (!(w->Q[s] = c.alloc(4 * N))) return IGR_ENOMEM;
     for (int s = 0; s < 4; ++s) if (!(w->Sg[s] = c.alloc(N))) return IGR_ENOMEM;
-    double** one[] = {&w->Xbar, &w->Scarry, &w->RB[0], &w->RB[1], &w->PB[0], &w->PB[1], &w->irbar, &w->imbar};
+    double** one[] = {&w->Xbar, &w->Scarry, &w->RB[0], &w->RB[1], &w->PB[0], &w->PB[1], &w->irbar, &w->imbar,
+                      &w->bbar, &w->XB[0], &w->XB[1]};
     for (auto p : one) if (!(*p = c.alloc(N))) return IGR_ENOMEM;
     double** four[] = {&w->QBn, &w->QBa, &w->QBb, &w->QBnext};
     for (auto p : four) if (!(*p = c.alloc(4 * N))) return IGR_ENOMEM;
@@ -94,8 +100,24 @@ static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
   return IGR_OK;
 }
 
+// Chebyshev recurrence coefficients of iterations k = 1..n, as the forward
+// solve forms them (chebyshev_fixed!, IGR_adjoints_2d/elliptic.jl:370-397).
+static void cheb_coeffs(int p, int n, double& theta, std::vector<double>& c1, std::vector<double>& c2) {
+  const double lmin = 1.0, lmax = cheb_lmax(p);
+  theta = (lmax + lmin) / 2.0;
+  const double delta = (lmax - lmin) / 2.0;
+  const double sigma = theta / std::max(delta, 1e-10);
+  double rc = 1.0 / sigma;
+  c1.assign(n + 1, 0.0);
+  c2.assign(n + 1, 0.0);
+  for (int k = 2; k <= n; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rc);
+    c1[k] = rn * rc; c2[k] = 2.0 * rn / delta; rc = rn;
+  }
+}
+
 // PCG solve on stage state qs with warm start/result x, keeping every iterate.
-static void solve2_store(Ctx& c, const double* qs, double* x, double alpha, int n_iter, StageStore& st) {
+static void solve2_store_pcg(Ctx& c, const double* qs, double* x, double alpha, int n_iter, StageStore& st) {
   const size_t N = c.N;
   SipArgs a = sip_base(c, alpha);
   a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x;
@@ -121,16 +143,62 @@ static void solve2_store(Ctx& c, const double* qs, double* x, double alpha, int 
   launch_pcg_final(c, R0, grid2_blocks(c), st.sc, n_iter);
 }
 
+// Damped Jacobi (jacobi_fixed!, elliptic.jl:354-361) keeping x_k and A x_k.
+static void solve2_store_jac(Ctx& c, const double* qs, double* x, double alpha, int n_iter, StageStore& st) {
+  const size_t N = c.N;
+  SipArgs a = sip_base(c, alpha);
+  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x;
+  a.invR_out = st.invR; a.invM_out = st.invM; a.b_out = st.b;
+  launch2_sip(c, M_INIT_JAC, a);
+  if (n_iter > 0) launch_copy(c, st.P, x, N);
+  for (int k = 0; k < n_iter; ++k) {
+    SipArgs m = sip_base(c, alpha);
+    m.x = st.P + (size_t)k * N; m.xout = (k + 1 < n_iter) ? st.P + (size_t)(k + 1) * N : x;
+    m.y = st.AP + (size_t)k * N; m.b = st.b; m.invM = st.invM; m.invR = st.invR; m.omega = 2.0 / 3.0;
+    launch2_sip(c, M_JACOBI, m);
+  }
+}
+
+// Chebyshev-accelerated Jacobi (chebyshev_fixed!, elliptic.jl:370-397) keeping x_k and r_k.
+static void solve2_store_cheb(Ctx& c, const double* qs, double* x, double alpha, int n_iter, StageStore& st) {
+  const size_t N = c.N;
+  SipArgs a = sip_base(c, alpha);
+  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x;
+  a.invR_out = st.invR; a.invM_out = st.invM; a.b_out = st.b; a.rout = st.R;
+  launch2_sip(c, M_INIT_CHEB, a);
+  double theta;
+  std::vector<double> c1, c2;
+  cheb_coeffs(c.p, n_iter, theta, c1, c2);
+  double* Dd[2] = {c.dA, c.dB};
+  for (int k = 1; k <= n_iter; ++k) {
+    SipArgs m = sip_base(c, alpha);
+    m.first = (k == 1); m.theta = theta; m.c1 = c1[k]; m.c2 = c2[k];
+    m.x = (k == 1) ? x : st.P + (size_t)(k - 2) * N; m.d = Dd[(k - 1) % 2]; m.r = st.R + (size_t)(k - 1) * N;
+    m.xout = st.P + (size_t)(k - 1) * N; m.dout = Dd[k % 2]; m.rout = st.R + (size_t)k * N;
+    m.b = st.b; m.invM = st.invM; m.invR = st.invR;
+    launch2_sip(c, M_CHEB, m);
+  }
+  if (n_iter > 0) launch_copy(c, x, st.P + (size_t)(n_iter - 1) * N, N);
+}
+
+static void solve2_store(Ctx& c, const double* qs, double* x, double alpha, int n_iter, int solver,
+                         StageStore& st) {
+  if (solver == IGR_SOLVER_JACOBI) solve2_store_jac(c, qs, x, alpha, n_iter, st);
+  else if (solver == IGR_SOLVER_CHEBYSHEV) solve2_store_cheb(c, qs, x, alpha, n_iter, st);
+  else solve2_store_pcg(c, qs, x, alpha, n_iter, st);
+}
+
 // One forward step from (q^n in W.Q[0], Sigma_0 in W.Sg[0]) storing what the
 // reverse needs; q^{n+1} is written to out (4N), Sigma_3 stays in W.Sg[3].
-static void step_store(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, double* out) {
+static void step_store(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, int solver,
+                       double* out) {
   const size_t N = c.N;
   const double cdt[3] = {dt, 1.0 / 4.0 * dt, 2.0 / 3.0 * dt};
   for (int s = 0; s < 3; ++s) {
     const double* qs = W.Q[s];
     if (alpha > 0.0) {
       launch_copy(c, W.Sg[s + 1], W.Sg[s], N);
-      solve2_store(c, qs, W.Sg[s + 1], alpha, n_iter, W.st[s]);
+      solve2_store(c, qs, W.Sg[s + 1], alpha, n_iter, solver, W.st[s]);
     } else {
       launch_fill(c, W.Sg[s + 1], N, 0.0);
     }
@@ -144,8 +212,8 @@ static void step_store(Ctx& c, GradWork& W, double dt, double gamma, double alph
 // Reverse of one fixed-iteration PCG solve whose iterates are in st.
 // xbar: adjoint of the solve output; x0: warm start; qs: stage state.
 // Adds the state adjoint into qbar, writes the warm-start adjoint to x0bar.
-static void solve2_rev(Ctx& c, GradWork& W, const double* qs, const double* x0, const double* xbar, double alpha,
-                       int n, StageStore& st, double* qbar, double* x0bar) {
+static void solve2_rev_pcg(Ctx& c, GradWork& W, const double* qs, const double* x0, const double* xbar,
+                           double alpha, int n, StageStore& st, double* qbar, double* x0bar) {
   const size_t N = c.N;
   launch_fill(c, W.irbar, N, 0.0);
   launch_fill(c, W.imbar, N, 0.0);
@@ -176,10 +244,75 @@ static void solve2_rev(Ctx& c, GradWork& W, const double* qs, const double* x0, 
   launch2_rev(c, RV_PCG_FIN, f);
 }
 
+// Reverse of damped Jacobi: x-bar_k = x-bar_{k+1} - A (omega x-bar_{k+1} / M), plus
+// the adjoints of 1/M, b and 1/rho (and alpha) of every iteration.
+static void solve2_rev_jac(Ctx& c, GradWork& W, const double* qs, const double* x0, const double* xbar,
+                           double alpha, int n, StageStore& st, double* qbar, double* x0bar) {
+  const size_t N = c.N;
+  launch_fill(c, W.irbar, N, 0.0);
+  launch_fill(c, W.imbar, N, 0.0);
+  launch_fill(c, W.bbar, N, 0.0);
+  RevArgs a;
+  a.g = geo2(c); a.B = c.basis; a.alpha = alpha; a.n_iter = n;
+  a.invR = st.invR; a.invM = st.invM; a.X = st.P; a.AX = st.AP; a.b = st.b;
+  a.irbar = W.irbar; a.imbar = W.imbar; a.bbar = W.bbar; a.abar_blk = W.ablk; a.omega = 2.0 / 3.0;
+  const double* cur = xbar;
+  for (int k = n - 1; k >= 0; --k) {
+    const int m = n - 1 - k;
+    RevArgs r = a;
+    r.k = k; r.xbar = cur; r.xbar_out = W.RB[m % 2];
+    launch2_rev(c, RV_JAC_ITER, r);
+    cur = W.RB[m % 2];
+  }
+  RevArgs f = a;
+  f.xbar = cur; f.x0 = x0; f.q = qs; f.qbar = qbar; f.x0bar = x0bar;
+  launch2_rev(c, RV_JAC_FIN, f);
+}
+
+// Reverse of Chebyshev: per iteration k = n..1 the adjoints of x_k, d_k, r_{k-1};
+// the final kernel reverses r_0 = b - A x_0 and the set-up.
+static void solve2_rev_cheb(Ctx& c, GradWork& W, const double* qs, const double* x0, const double* xbar,
+                            double alpha, int n, StageStore& st, double* qbar, double* x0bar) {
+  const size_t N = c.N;
+  launch_fill(c, W.irbar, N, 0.0);
+  launch_fill(c, W.imbar, N, 0.0);
+  launch_fill(c, W.bbar, N, 0.0);
+  double theta;
+  std::vector<double> c1, c2;
+  cheb_coeffs(c.p, n, theta, c1, c2);
+  RevArgs a;
+  a.g = geo2(c); a.B = c.basis; a.alpha = alpha; a.n_iter = n;
+  a.invR = st.invR; a.invM = st.invM; a.X = st.P; a.R = st.R; a.b = st.b; a.theta = theta;
+  a.irbar = W.irbar; a.imbar = W.imbar; a.bbar = W.bbar; a.abar_blk = W.ablk;
+  const double* cur = xbar;
+  for (int k = n; k >= 1; --k) {
+    const int m = n - k;
+    RevArgs r = a;
+    r.k = k; r.first = (k == n); r.cc1 = c1[k]; r.cc2 = c2[k];
+    r.xbar = cur; r.xbar_out = W.XB[m % 2];
+    r.pbar_in = W.PB[(m + 1) % 2]; r.pbar_out = W.PB[m % 2];
+    r.rbar_in = W.RB[(m + 1) % 2]; r.rbar_out = W.RB[m % 2];
+    launch2_rev(c, RV_CHEB_ITER, r);
+    cur = W.XB[m % 2];
+  }
+  RevArgs f = a;
+  f.xbar = cur; f.rbar_in = (n > 0) ? W.RB[(n - 1) % 2] : nullptr;
+  f.x0 = x0; f.q = qs; f.qbar = qbar; f.x0bar = x0bar;
+  launch2_rev(c, RV_CHEB_FIN, f);
+}
+
+static void solve2_rev(Ctx& c, GradWork& W, const double* qs, const double* x0, const double* xbar, double alpha,
+                       int n, int solver, StageStore& st, double* qbar, double* x0bar) {
+  if (solver == IGR_SOLVER_JACOBI) solve2_rev_jac(c, W, qs, x0, xbar, alpha, n, st, qbar, x0bar);
+  else if (solver == IGR_SOLVER_CHEBYSHEV) solve2_rev_cheb(c, W, qs, x0, xbar, alpha, n, st, qbar, x0bar);
+  else solve2_rev_pcg(c, W, qs, x0, xbar, alpha, n, st, qbar, x0bar);
+}
+
 // Reverse of one step stored by step_store.  W.QBnext holds the adjoint of
 // q^{n+1}; sbar_next the adjoint of Sigma_3 (null if none).  On return W.QBn
 // holds the adjoint of q^n and W.Scarry the adjoint of Sigma_0.
-static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, bool has_sbar) {
+static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, int solver,
+                     bool has_sbar) {
   const size_t N = c.N;
   launch_fill(c, W.QBn, 4 * N, 0.0);
   const bool solve = alpha > 0.0;
@@ -201,7 +334,7 @@ static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha,
     h.gbar_blk = W.gblk;
     launch2_hyp_rev(c, h);
     if (solve) {
-      solve2_rev(c, W, W.Q[s], W.Sg[s], W.Xbar, alpha, n_iter, W.st[s], sg[s].qbar_stage, W.Scarry);
+      solve2_rev(c, W, W.Q[s], W.Sg[s], W.Xbar, alpha, n_iter, solver, W.st[s], sg[s].qbar_stage, W.Scarry);
       sbar_valid = true;
     }
   }
@@ -243,7 +376,7 @@ struct GradRun {
   Ctx& c;
   GradWork& W;
   double dt, gamma, alpha;
-  int n_iter;
+  int n_iter, solver;
   Schedule sch;
   int obj;
   double prm[4];
@@ -259,22 +392,22 @@ struct GradRun {
     launch_copy(c, c.S, W.ck[slot] + 4 * c.N, c.N);
   }
   void advance(int steps) {
-    for (int s = 0; s < steps; ++s) step2(c, dt, gamma, alpha, n_iter, IGR_SOLVER_PCG);
+    for (int s = 0; s < steps; ++s) step2(c, dt, gamma, alpha, n_iter, solver);
     fwd_steps += steps;
   }
   // recompute the step whose start state is in (c.q, c.S), storing its iterates, and reverse it
   void base_step() {
     launch_copy(c, W.Q[0], c.q, 4 * c.N);
     launch_copy(c, W.Sg[0], c.S, c.N);
-    step_store(c, W, dt, gamma, alpha, n_iter, c.q);
+    step_store(c, W, dt, gamma, alpha, n_iter, solver, c.q);
     ++fwd_steps;
     if (first_base) {
       // q(T) is now in c.q: the objective and its gradient seed the sweep
       objective_seed();
       first_base = false;
-      step_rev(c, W, dt, gamma, alpha, n_iter, false);
+      step_rev(c, W, dt, gamma, alpha, n_iter, solver, false);
     } else {
-      step_rev(c, W, dt, gamma, alpha, n_iter, true);
+      step_rev(c, W, dt, gamma, alpha, n_iter, solver, true);
     }
     std::swap(W.QBn, W.QBnext);
   }
@@ -341,8 +474,6 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
   if (!c || c->dim != 2 || n_steps < 1 || n_iter < 0) return c ? seterr(*c, IGR_EINVAL, "bad argument") : IGR_EINVAL;
   if (objective < 0 || objective > 6) return seterr(*c, IGR_EINVAL, "unknown objective");
   if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
-  if (solver != IGR_SOLVER_PCG && alpha > 0.0)
-    return seterr(*c, IGR_EINVAL, "the 2D adjoint supports the PCG solver (jacobi/chebyshev: use PCG)");
   int rc = begin_call(*c);
   if (rc) return rc;
   rc = ensure_scal(*c, n_iter);
@@ -364,7 +495,7 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
   launch_fill(*c, W->dsc, 16, 0.0);
   // state at t = 0: q0 from the context, Sigma = 0 (enzyme_adjoint.jl:51)
   launch_fill(*c, c->S, c->N, 0.0);
-  GradRun run{*c, *W, dt, gamma, alpha, n_iter};
+  GradRun run{*c, *W, dt, gamma, alpha, n_iter, solver};
   run.obj = objective;
   const double* pp = prm ? prm : KE_WINDOW_DEFAULT;
   for (int k = 0; k < 4; ++k) run.prm[k] = pp[k];
@@ -455,13 +586,13 @@ int igr_apply_sip_vjp(igr_ctx* c, double alpha, const double* x, const double* r
   return IGR_OK;
 }
 
-// VJP of one solve_elliptic! (PCG) on the context state q with warm start
+// VJP of one solve_elliptic! (any solver) on the context state q with warm start
 // sigma0: given sigma-bar (adjoint of the result) returns q-bar, the warm-start
 // adjoint and alpha-bar.
 int igr_solve_elliptic_vjp(igr_ctx* c, double alpha, int n_iter, int solver, const double* sigma0,
                            const double* sigmabar, double* qbar, double* sigma0bar, double* alphabar) {
   if (!c || c->dim != 2 || !sigma0 || !sigmabar || n_iter < 0) return IGR_EINVAL;
-  if (solver != IGR_SOLVER_PCG) return seterr(*c, IGR_EINVAL, "VJP implemented for the PCG solver");
+  if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
   int rc = begin_call(*c);
   if (rc) return rc;
   GradWork* W = nullptr;
@@ -472,11 +603,11 @@ int igr_solve_elliptic_vjp(igr_ctx* c, double alpha, int n_iter, int solver, con
   HIPCHK(*c, hipMemcpyAsync(W->Xbar, sigmabar, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   launch_copy(*c, W->Q[0], c->q, 4 * N);
   launch_copy(*c, W->Sg[1], W->Sg[0], N);
-  solve2_store(*c, W->Q[0], W->Sg[1], alpha, n_iter, W->st[0]);
+  solve2_store(*c, W->Q[0], W->Sg[1], alpha, n_iter, solver, W->st[0]);
   launch_fill(*c, W->QBn, 4 * N, 0.0);
   launch_fill(*c, W->ablk, c->partials_n, 0.0);
   launch_fill(*c, W->dsc, 16, 0.0);
-  solve2_rev(*c, *W, W->Q[0], W->Sg[0], W->Xbar, alpha, n_iter, W->st[0], W->QBn, W->Scarry);
+  solve2_rev(*c, *W, W->Q[0], W->Sg[0], W->Xbar, alpha, n_iter, solver, W->st[0], W->QBn, W->Scarry);
   launch_sum_blocks(*c, W->ablk, grid2_blocks(*c), W->dsc + 4);
   if (qbar) HIPCHK(*c, hipMemcpyAsync(qbar, W->QBn, 4 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (sigma0bar) HIPCHK(*c, hipMemcpyAsync(sigma0bar, W->Scarry, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
 template <int NP, int TX, int TY, int MODE>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
-  constexpr bool FIN = (MODE == RV_PCG_FIN);
+  constexpr bool FIN = (MODE == RV_PCG_FIN || MODE == RV_JAC_FIN || MODE == RV_CHEB_FIN);
   __shared__ double ws[NS * NP2];   // adjoint vector w (A is applied to it)
   __shared__ double vs[NS * NP2];   // vector v of the forward product A v
   __shared__ double irs[NS * NP2];
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   } else if constexpr (FIN) {
     rzb = a.n_iter > 0 ? a.rs[RS::carry(0)] : 0.0;
   }
-  const bool have_carry = (MODE == RV_PCG_ITER) ? !a.first : (a.n_iter > 0);
+  const bool have_carry = (MODE == RV_PCG_ITER || MODE == RV_CHEB_ITER) ? !a.first : (a.n_iter > 0);
 
   // own-node values kept for the outputs
   double o_rb = 0.0, o_zb = 0.0, o_R = 0.0, o_pb = 0.0;
@@ -232,6 +232,28 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     if (t.slot_global(sl, nn, gi)) {
       if constexpr (MODE == RV_VJP) {
         w = a.wbar[gi]; v = a.v[gi]; ir = a.invR[gi];
+      } else if constexpr (MODE == RV_JAC_ITER) {
+        // x_{k+1} = x_k + omega (b - A x_k) / M: A is applied to -omega x-bar_{k+1} / M
+        ir = a.invR[gi];
+        const double xb1 = a.xbar[gi];
+        w = -(a.omega * a.invM[gi]) * xb1;
+        v = a.X[(size_t)k * N + gi];
+        if (idx == tid) o_pb = xb1;
+      } else if constexpr (MODE == RV_CHEB_ITER) {
+        // r_k = b - A x_k: A is applied to -r-bar_k
+        ir = a.invR[gi];
+        const double rb = have_carry ? a.rbar_in[gi] : 0.0;
+        w = -rb;
+        v = a.X[(size_t)(k - 1) * N + gi];
+        if (idx == tid) o_rb = rb;
+      } else if constexpr (MODE == RV_JAC_FIN || MODE == RV_CHEB_FIN) {
+        ir = a.invR[gi];
+        const double iM = a.invM[gi];
+        const double rb = (MODE == RV_CHEB_FIN && have_carry) ? a.rbar_in[gi] : 0.0;
+        w = -rb;
+        v = a.x0[gi];
+        mb = -a.imbar[gi] * iM * iM;
+        if (sl < T) { us[idx] = a.q[gi] / a.q[2 * N + gi]; vvs[idx] = a.q[N + gi] / a.q[2 * N + gi]; }
       } else {
         ir = a.invR[gi];
         const double iM = a.invM[gi];
@@ -317,8 +339,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     if (j == 0) face(false, sB, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syb[s * NP + i]);
     if (j == NP - 1) face(true, sT, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syt[s * NP + i]);
     if constexpr (FIN) {
-      // IGR source b = c alpha [(uxx+uyy)^2 + uxx^2 + 2 uxy uyx + uyy^2], b-bar = r0-bar = -w
+      // IGR source b = c alpha [(uxx+uyy)^2 + uxx^2 + 2 uxy uyx + uyy^2]; b-bar is r0-bar = -w
+      // (PCG, Chebyshev) plus what the Jacobi/Chebyshev iterations accumulated
       bown = -wown;
+      if constexpr (MODE != RV_PCG_FIN) bown += a.bbar[gme];
       double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
@@ -391,6 +415,33 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     block_sum<1>(rv, red);
     if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
     block_partials_store<4>(dv, red, a.part_out, gridDim.x, blockIdx.x);
+  } else if constexpr (MODE == RV_JAC_ITER) {
+    if (valid) {
+      const double xb1 = o_pb, iM = a.invM[gme];
+      a.irbar[gme] += irb;
+      a.xbar_out[gme] = xb1 + y;
+      a.imbar[gme] += a.omega * (a.b[gme] - a.AX[(size_t)k * N + gme]) * xb1;
+      a.bbar[gme] += a.omega * iM * xb1;
+    }
+    double rv[1] = {ab};
+    block_sum<1>(rv, red);
+    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
+  } else if constexpr (MODE == RV_CHEB_ITER) {
+    if (valid) {
+      const double iM = a.invM[gme];
+      a.irbar[gme] += irb;
+      a.bbar[gme] += o_rb;
+      const double Xb = a.xbar[gme] + y;                       // adjoint of x_k
+      const double Db = (have_carry ? a.pbar_in[gme] : 0.0) + Xb;   // adjoint of d_k
+      const double zb = (k > 1) ? a.cc2 * Db : Db / a.theta;   // adjoint of z_k = r_{k-1} / M
+      a.pbar_out[gme] = (k > 1) ? a.cc1 * Db : 0.0;
+      a.rbar_out[gme] = zb * iM;
+      a.imbar[gme] += zb * a.R[(size_t)(k - 1) * N + gme];
+      a.xbar_out[gme] = Xb;
+    }
+    double rv[1] = {ab};
+    block_sum<1>(rv, red);
+    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
   } else if constexpr (FIN) {
     double ab_extra = 0.0;
     if (valid) {
@@ -583,6 +634,10 @@ struct LaunchRev {
       case RV_PCG_START: hipLaunchKernelGGL((k2_pcg_rev_start<NP, TX, TY>), G, Bk, 0, st, a); break;
       case RV_PCG_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_ITER>), G, Bk, 0, st, a); break;
       case RV_PCG_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_FIN>), G, Bk, 0, st, a); break;
+      case RV_JAC_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_JAC_ITER>), G, Bk, 0, st, a); break;
+      case RV_JAC_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_JAC_FIN>), G, Bk, 0, st, a); break;
+      case RV_CHEB_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_CHEB_ITER>), G, Bk, 0, st, a); break;
+      case RV_CHEB_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_CHEB_FIN>), G, Bk, 0, st, a); break;
     }
   }
   static void obj(const ObjArgs& a, bool grad, hipStream_t st) {

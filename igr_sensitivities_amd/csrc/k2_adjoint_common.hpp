@@ -42,14 +42,15 @@ struct RevArgs {
   const double* P = nullptr;      // PCG directions p_k, k < n       (stride N)
   const double* AP = nullptr;     // A p_k                            (stride N)
   const double* R = nullptr;      // residuals r_k, k <= n            (stride N)
-  const double* X = nullptr;      // Jacobi/Chebyshev iterates x_k    (stride N)
+  const double* X = nullptr;      // Jacobi x_k (k < n) / Chebyshev x_k at k-1 (k >= 1)   (stride N)
   const double* AX = nullptr;     // Jacobi A x_k                     (stride N)
   const double* b = nullptr;      // IGR source b (Jacobi/Chebyshev)
   const double* sc = nullptr;     // forward scalars rz | pAp | a | beta
   const double* x0 = nullptr;     // warm start
   const double* q = nullptr;      // stage state (4 fields): mx, my, rho, E
   // adjoint state
-  const double* xbar = nullptr;   // adjoint of the solve output
+  const double* xbar = nullptr;   // adjoint of the solve output (Jacobi/Chebyshev: of x_{k+1} / x_k)
+  double* xbar_out = nullptr;     // Jacobi/Chebyshev: adjoint of x_k / x_{k-1}
   const double* rbar_in = nullptr;
   double* rbar_out = nullptr;
   const double* pbar_in = nullptr;
@@ -62,7 +63,7 @@ struct RevArgs {
   double* qbar = nullptr;         // adjoint of the stage state (4 fields, +=)
   double* abar_blk = nullptr;     // per-workgroup alpha-bar accumulators
   double omega = 0.0;             // Jacobi
-  double theta = 1.0, c1n = 0.0, c2n = 0.0, c1k = 0.0;   // Chebyshev coefficients
+  double theta = 1.0, cc1 = 0.0, cc2 = 0.0;   // Chebyshev: d_k = cc1 d_{k-1} + cc2 z_k (k > 1), z_1/theta
   int first = 0;
   // standalone VJP
   const double* v = nullptr;      // vector the operator was applied to

@@ -276,7 +276,10 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     }
     block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
   } else if constexpr (MODE == M_JACOBI) {
-    if (valid) a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
+    if (valid) {
+      if (a.y) a.y[gme] = y;     // A x_k, kept by the adjoint's recompute
+      a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
+    }
   } else if constexpr (MODE == M_CHEB) {
     if (valid) {
       a.xout[gme] = vown;

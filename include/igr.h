@@ -162,7 +162,8 @@ int igr_objective(igr_ctx* ctx, int objective, const double* prm, double gamma, 
  * and dJ/dalpha.  ckpt_slots: number of (q, Sigma) checkpoints kept in HBM
  * (<= 0: as many as half the free HBM allows, up to n_steps); steps between
  * checkpoints are recomputed (optimal binomial schedule).  On return the
- * context state holds q(T).  Sigma solver: PCG (any solver when alpha <= 0). */
+ * context state holds q(T).  Any Sigma solver: PCG is reversed through its
+ * stored iterates and scalars, Jacobi and Chebyshev through their iterates. */
 int igr_gradient(igr_ctx* ctx, int objective, const double* prm, int n_steps, double dt, double gamma,
                  double alpha, int n_iter, int solver, int ckpt_slots, double* J, double* qbar0,
                  double* gammabar, double* alphabar);
@@ -174,7 +175,7 @@ int igr_hyperbolic_vjp(igr_ctx* ctx, double gamma, const double* q, const double
 /* apply_sip! (elliptic.jl:30): y-bar -> x-bar = A^T y-bar, rho-bar, alpha-bar. */
 int igr_apply_sip_vjp(igr_ctx* ctx, double alpha, const double* x, const double* rho, const double* ybar,
                       double* xbar, double* rhobar, double* alphabar);
-/* solve_elliptic! with PCG (elliptic.jl:413) on the context state q and warm
+/* solve_elliptic! (elliptic.jl:413, any solver) on the context state q and warm
  * start sigma0: Sigma-bar -> q-bar (4N), sigma0-bar, alpha-bar. */
 int igr_solve_elliptic_vjp(igr_ctx* ctx, double alpha, int n_iter, int solver, const double* sigma0,
                            const double* sigmabar, double* qbar, double* sigma0bar, double* alphabar);

@@ -54,8 +54,13 @@ def test_apply_sip_vjp_dot_product(orc, nx, ny):
     close(np.vdot(yb, Ju), np.vdot(xb, dx) + np.vdot(rb, drho) + ab * da, 1e-12)
 
 
-@pytest.mark.parametrize("n_iter", [0, 1, 5, 30])
-def test_solve_vjp_dot_product(orc, n_iter):
+SOLVE_CASES = [("pcg", 0), ("pcg", 1), ("pcg", 5), ("pcg", 30), ("jacobi", 0), ("jacobi", 1), ("jacobi", 7),
+               ("jacobi", 30), ("chebyshev", 0), ("chebyshev", 1), ("chebyshev", 2), ("chebyshev", 6),
+               ("chebyshev", 25)]
+
+
+@pytest.mark.parametrize("solver,n_iter", SOLVE_CASES)
+def test_solve_vjp_dot_product(orc, solver, n_iter):
     c = ctx2(3, 6, 5)
     g = orc.Grid2D(3, 6, 5)
     rng = np.random.default_rng(30 + n_iter)
@@ -64,9 +69,9 @@ def test_solve_vjp_dot_product(orc, n_iter):
     dq, dS0, da = rng.standard_normal(q.shape), rng.standard_normal(g.shape), 0.2
     Sb = rng.standard_normal(g.shape)
     a = 5e-3
-    _, dS = g.jvp_solve_elliptic(a, da * a, n_iter, orc.PCG, q, dq, S0, dS0)
+    _, dS = g.jvp_solve_elliptic(a, da * a, n_iter, igr.SOLVERS[solver], q, dq, S0, dS0)
     c.set_state(q)
-    qb, s0b, ab = c.solve_elliptic_vjp(a, n_iter, "pcg", S0, Sb)
+    qb, s0b, ab = c.solve_elliptic_vjp(a, n_iter, solver, S0, Sb)
     close(np.vdot(Sb, dS), np.vdot(qb, dq) + np.vdot(s0b, dS0) + ab * da * a, 1e-10)
 
 
@@ -94,6 +99,27 @@ def test_gradient_dot_product(orc, obj, prm):
         close(dJ, np.vdot(qb, u) + gb * ug + ab * ua * a, 1e-10)
 
 
+@pytest.mark.parametrize("solver,obj,prm", [("jacobi", 1, None), ("jacobi", 3, (0.55, 0.5, 0.15, 0.0)),
+                                            ("chebyshev", 0, None), ("chebyshev", 6, (0.5, 0.45, 0.3, 0.0))])
+def test_gradient_dot_product_jacobi_chebyshev(orc, solver, obj, prm):
+    nx, ny, ns = 6, 5, 3
+    c = ctx2(3, nx, ny)
+    g = orc.Grid2D(3, nx, ny)
+    q0 = configs.smooth_random2(3, nx, ny, seed=90 + obj, amp=0.3)
+    gam, a = 1.4, 4e-3
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, gam))
+    p4 = prm if prm is not None else (0.0, 0.0, 1.0, 0.0)
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(obj, ns, dt, gam, a, 12, solver, prm=p4)
+    rng = np.random.default_rng(95 + obj)
+    for trial in range(2):
+        u = rng.standard_normal(q0.shape)
+        ug, ua = rng.standard_normal(), rng.standard_normal()
+        Jr, dJ, _, _ = g.jvp_objective_run(gam, ug, a, ua * a, 12, igr.SOLVERS[solver], ns, dt, obj, q0, u, p4)
+        close(J, Jr, 1e-12)
+        close(dJ, np.vdot(qb, u) + gb * ug + ab * ua * a, 1e-10)
+
+
 def test_gradient_alpha_zero_and_deterministic(orc):
     nx, ny, ns = 5, 4, 4
     q0 = configs.smooth_random2(3, nx, ny, seed=60, amp=0.3)
@@ -109,7 +135,8 @@ def test_gradient_alpha_zero_and_deterministic(orc):
     assert ab == 0.0
 
 
-def test_checkpoint_schedules_bitwise_equal():
+@pytest.mark.parametrize("solver", ["pcg", "chebyshev"])
+def test_checkpoint_schedules_bitwise_equal(solver):
     nx, ny, ns = 6, 6, 9
     q0 = configs.smooth_random2(3, nx, ny, seed=70, amp=0.3)
     dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
@@ -117,7 +144,7 @@ def test_checkpoint_schedules_bitwise_equal():
     for slots in (2, 3, 5, 10):
         c = ctx2(3, nx, ny)
         c.set_state(q0)
-        res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 15, "pcg", prm=(0.55, 0.5, 0.15, 0.0), ckpt_slots=slots))
+        res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 15, solver, prm=(0.55, 0.5, 0.15, 0.0), ckpt_slots=slots))
     for r in res[1:]:
         assert r[0] == res[0][0]
         np.testing.assert_array_equal(r[1], res[0][1])
