@@ -387,9 +387,13 @@ OBJECTIVES = {"l2_density": 0, "l2_pressure": 1, "kinetic_energy": 2, "windowed_
 
 def enzyme_adjoint_ic(q0, n_steps, dt, basis, mesh, gamma, alpha, n_iter=10, solver="pcg",
                       objective="l2_density", prm=None, ckpt_slots=0):
-    """enzyme_adjoint_ic (IGR_adjoints_2d/enzyme_adjoint.jl:270-296) on the GPU
-    discrete adjoint: returns (J, dq0) with dq0 = dJ/dq0 stacked (mx, my, rho, E).
-    Like the reference, n_iter is clamped to >= 1 (enzyme_adjoint.jl:292)."""
+    """enzyme_adjoint_ic on the GPU discrete adjoint.
+
+    2D (IGR_adjoints_2d/enzyme_adjoint.jl:270-296): returns (J, dq0) with
+    dq0 = dJ/dq0 stacked (mx, my, rho, E).  1D (IGR_adjoints_1d/
+    enzyme_adjoint.jl:255-314): returns (J, adj_rho, adj_u, adj_p), the
+    conservative adjoints mapped to primitive ones as the reference does
+    (:306-311).  Like the reference, n_iter is clamped to >= 1."""
     key = objective.lstrip(":") if isinstance(objective, str) else objective
     if isinstance(key, str):
         if key not in OBJECTIVES:
@@ -399,6 +403,13 @@ def enzyme_adjoint_ic(q0, n_steps, dt, basis, mesh, gamma, alpha, n_iter=10, sol
     code = _solver_code(solver)
     c.set_state(q0)
     J, qb, _, _ = c.gradient(key, n_steps, dt, gamma, alpha, max(int(n_iter), 1), code, prm, ckpt_slots)
+    if mesh.dim == 1:
+        q0 = np.asarray(q0, dtype=np.float64)
+        u0 = q0[0] / q0[1]
+        adj_rho = qb[1] + u0 * qb[0] + (u0 ** 2 / 2) * qb[2]
+        adj_u = q0[1] * qb[0] + q0[1] * u0 * qb[2]
+        adj_p = qb[2] / (gamma - 1)
+        return J, adj_rho, adj_u, adj_p
     return J, qb
 
 

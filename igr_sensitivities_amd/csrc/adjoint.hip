@@ -34,6 +34,8 @@ Geo2 geo2(const Ctx& c);
 SipArgs sip_base(const Ctx& c, double alpha);
 HypArgs hyp_base(const Ctx& c, double gamma);
 double cheb_lmax(int p);
+void launch1_adjoint(const Ctx& c, const K1AdjArgs& a, int block);
+void launch1_run(const Ctx& c, const K1Args& a, int block);
 
 
 // Storage of one stage's solve, written by the recompute and read by the reverse.
@@ -93,6 +95,45 @@ static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
   }
   while ((int)w->ck.size() < slots) {
     double* p = c.alloc(5 * N);
+    if (!p) return IGR_ENOMEM;
+    w->ck.push_back(p);
+  }
+  W = w;
+  return IGR_OK;
+}
+
+// 1D workspace: the stored step, the adjoint vectors and checkpoint slots (4N each).
+struct GradWork1 {
+  int n_iter = -1;
+  K1AdjArgs base;
+  std::vector<double*> ck;
+};
+static std::map<const Ctx*, GradWork1*> g_work1;
+
+static int alloc_work1(Ctx& c, int n_iter, int slots, GradWork1*& W) {
+  GradWork1*& w = g_work1[&c];
+  const size_t N = c.N;
+  const int n = std::max(n_iter, 0);
+  if (!w || w->n_iter != n) {
+    w = new GradWork1();   // earlier buffers stay owned by the context until igr_destroy
+    w->n_iter = n;
+    K1AdjArgs& a = w->base;
+    for (int s = 0; s < 3; ++s) {
+      a.Q[s] = c.alloc(3 * N); a.invR[s] = c.alloc(N); a.invM[s] = c.alloc(N); a.b[s] = c.alloc(N);
+      a.P[s] = c.alloc(std::max<size_t>((size_t)n * N, 1)); a.AP[s] = c.alloc(std::max<size_t>((size_t)n * N, 1));
+      a.R[s] = c.alloc((size_t)(n + 1) * N); a.sc[s] = c.alloc(4 * (size_t)std::max(n, 1) + 1);
+      if (!a.Q[s] || !a.invR[s] || !a.invM[s] || !a.b[s] || !a.P[s] || !a.AP[s] || !a.R[s] || !a.sc[s])
+        return IGR_ENOMEM;
+    }
+    for (int s = 0; s < 4; ++s) if (!(a.Sg[s] = c.alloc(N))) return IGR_ENOMEM;
+    double** three[] = {&a.QBnext, &a.QBn, &a.QBa, &a.QBb, &a.Ab};
+    for (auto p : three) if (!(*p = c.alloc(3 * N))) return IGR_ENOMEM;
+    double** one[] = {&a.Xbar, &a.Scarry, &a.w, &a.y, &a.rb, &a.pb, &a.xb, &a.irbar, &a.imbar, &a.bbar};
+    for (auto p : one) if (!(*p = c.alloc(N))) return IGR_ENOMEM;
+    if (!(a.dsc = c.alloc(16))) return IGR_ENOMEM;
+  }
+  while ((int)w->ck.size() < slots) {
+    double* p = c.alloc(4 * N);
     if (!p) return IGR_ENOMEM;
     w->ck.push_back(p);
   }
@@ -374,7 +415,8 @@ struct Schedule {
 
 struct GradRun {
   Ctx& c;
-  GradWork& W;
+  GradWork* W = nullptr;     // 2D
+  GradWork1* W1 = nullptr;   // 1D
   double dt, gamma, alpha;
   int n_iter, solver;
   Schedule sch;
@@ -383,49 +425,78 @@ struct GradRun {
   bool first_base = true;
   long long fwd_steps = 0;
 
-  void save(int slot) {
-    launch_copy(c, W.ck[slot], c.q, 4 * c.N);
-    launch_copy(c, W.ck[slot] + 4 * c.N, c.S, c.N);
+  double* slot(int k) const { return c.dim == 1 ? W1->ck[k] : W->ck[k]; }
+  void save(int k) {
+    launch_copy(c, slot(k), c.q, c.nv * c.N);
+    launch_copy(c, slot(k) + c.nv * c.N, c.S, c.N);
   }
-  void restore(int slot) {
-    launch_copy(c, c.q, W.ck[slot], 4 * c.N);
-    launch_copy(c, c.S, W.ck[slot] + 4 * c.N, c.N);
+  void restore(int k) {
+    launch_copy(c, c.q, slot(k), c.nv * c.N);
+    launch_copy(c, c.S, slot(k) + c.nv * c.N, c.N);
   }
   void advance(int steps) {
-    for (int s = 0; s < steps; ++s) step2(c, dt, gamma, alpha, n_iter, solver);
+    if (steps <= 0) return;
+    if (c.dim == 1) {
+      K1Args a = k1_base(c, gamma, alpha, n_iter, solver);
+      a.op = OP_RUN; a.n_steps = steps; a.dt = dt; a.zero_sigma = 0; a.final_solve = 0;
+      launch1_run(c, a, k1_block(c));
+    } else {
+      for (int s = 0; s < steps; ++s) step2(c, dt, gamma, alpha, n_iter, solver);
+    }
     fwd_steps += steps;
   }
   // recompute the step whose start state is in (c.q, c.S), storing its iterates, and reverse it
   void base_step() {
-    launch_copy(c, W.Q[0], c.q, 4 * c.N);
-    launch_copy(c, W.Sg[0], c.S, c.N);
-    step_store(c, W, dt, gamma, alpha, n_iter, solver, c.q);
     ++fwd_steps;
+    if (c.dim == 1) {
+      K1AdjArgs& b = W1->base;
+      launch_copy(c, b.Q[0], c.q, 3 * c.N);
+      launch_copy(c, b.Sg[0], c.S, c.N);
+      K1AdjArgs a = b;
+      a.f = k1_base(c, gamma, alpha, n_iter, solver);
+      a.f.q = b.Q[0];
+      a.dt = dt; a.obj = obj; a.out = c.q;
+      a.op = OPA_STORE;
+      launch1_adjoint(c, a, k1_block(c));
+      if (first_base) {
+        a.op = OPA_OBJ; a.f.q = c.q;
+        launch1_adjoint(c, a, k1_block(c));
+        a.f.q = b.Q[0];
+      }
+      a.op = OPA_REV; a.has_sbar = first_base ? 0 : 1;
+      launch1_adjoint(c, a, k1_block(c));
+      first_base = false;
+      std::swap(b.QBn, b.QBnext);
+      return;
+    }
+    launch_copy(c, W->Q[0], c.q, 4 * c.N);
+    launch_copy(c, W->Sg[0], c.S, c.N);
+    step_store(c, *W, dt, gamma, alpha, n_iter, solver, c.q);
     if (first_base) {
       // q(T) is now in c.q: the objective and its gradient seed the sweep
       objective_seed();
       first_base = false;
-      step_rev(c, W, dt, gamma, alpha, n_iter, solver, false);
+      step_rev(c, *W, dt, gamma, alpha, n_iter, solver, false);
     } else {
-      step_rev(c, W, dt, gamma, alpha, n_iter, solver, true);
+      step_rev(c, *W, dt, gamma, alpha, n_iter, solver, true);
     }
-    std::swap(W.QBn, W.QBnext);
+    std::swap(W->QBn, W->QBnext);
   }
   // reverse steps [a, b): the state at a is in `slot`; the slots after it are free
-  void reverse_range(int a, int b, int slot, int free_slots) {
+  void reverse_range(int a, int b, int k, int free_slots) {
     const int n = b - a;
-    if (n == 1) { restore(slot); base_step(); return; }
+    if (n == 1) { restore(k); base_step(); return; }
     const int fs = std::min(free_slots, n - 1);
     if (fs == 0) {
-      for (int m = b - 1; m >= a; --m) { restore(slot); advance(m - a); base_step(); }
+      for (int m = b - 1; m >= a; --m) { restore(k); advance(m - a); base_step(); }
       return;
     }
     const int j = sch.split_of(n, fs);
-    restore(slot);
+    restore(k);
     advance(j);
-    save(slot + 1);
-    reverse_range(a + j, b, slot + 1, free_slots - 1);
-    reverse_range(a, a + j, slot, free_slots);
+    save(k + 1);
+    reverse_range(a + j, b, k + 1, free_slots - 1);
+    reverse_range(a, a + j, k, free_slots);
   }
   void objective_seed();
 };
@@ -435,7 +506,7 @@ void GradRun::objective_seed() {
   o.g = geo2(c); o.B = c.basis; o.obj = obj;
   for (int k = 0; k < 4; ++k) o.prm[k] = prm[k];
   o.Lx = c.Lx; o.Ly = c.Ly; o.gamma = gamma; o.q = c.q;
-  o.out = W.dsc; o.qbar = W.QBnext; o.gbar = W.dsc + 3;
+  o.out = W->dsc; o.qbar = W->QBnext; o.gbar = W->dsc + 3;
   o.partials = c.partials; o.counter = c.counter;
   launch2_obj(c, o, false);
   launch2_obj(c, o, true);
@@ -471,43 +542,60 @@ int igr_objective(igr_ctx* c, int objective, const double* prm, double gamma, do
 int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, double dt, double gamma, double alpha,
                  int n_iter, int solver, int ckpt_slots, double* J, double* qbar0, double* gammabar,
                  double* alphabar) {
-  if (!c || c->dim != 2 || n_steps < 1 || n_iter < 0) return c ? seterr(*c, IGR_EINVAL, "bad argument") : IGR_EINVAL;
+  if (!c || n_steps < 1 || n_iter < 0) return c ? seterr(*c, IGR_EINVAL, "bad argument") : IGR_EINVAL;
   if (objective < 0 || objective > 6) return seterr(*c, IGR_EINVAL, "unknown objective");
+  if (c->dim == 1 && !(objective == 0 || objective == 1 || objective == 2 || objective == 4))
+    return seterr(*c, IGR_EINVAL, "unknown 1D objective (l2_density, l2_pressure, kinetic_energy, weighted_momentum)");
   if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "unknown solver");
   int rc = begin_call(*c);
   if (rc) return rc;
   rc = ensure_scal(*c, n_iter);
   if (rc) return rc;
+  const size_t N = c->N, nv = c->nv;
   int slots = ckpt_slots;
   if (slots <= 0) {
     size_t fr = 0, tot = 0;
     HIPCHK(*c, hipMemGetInfo(&fr, &tot));
-    const size_t per = 5 * c->N * sizeof(double);
+    const size_t per = (nv + 1) * N * sizeof(double);
     slots = (int)std::min<size_t>((size_t)n_steps, std::max<size_t>(2, (fr / 2) / per));
   }
   slots = std::max(2, std::min(slots, n_steps + 1));
-  GradWork* W = nullptr;
-  rc = alloc_work(*c, n_iter, slots, W);
-  if (rc) return seterr(*c, rc, "device memory for the gradient workspace");
-  const int nb = grid2_blocks(*c);
-  launch_fill(*c, W->gblk, c->partials_n, 0.0);
-  launch_fill(*c, W->ablk, c->partials_n, 0.0);
-  launch_fill(*c, W->dsc, 16, 0.0);
+  GradRun run{*c};
+  run.dt = dt; run.gamma = gamma; run.alpha = alpha; run.n_iter = n_iter; run.solver = solver;
+  double* dsc = nullptr;
+  double* qb = nullptr;
+  if (c->dim == 1) {
+    rc = alloc_work1(*c, n_iter, slots, run.W1);
+    if (rc) return seterr(*c, rc, "device memory for the gradient workspace");
+    dsc = run.W1->base.dsc;
+  } else {
+    rc = alloc_work(*c, n_iter, slots, run.W);
+    if (rc) return seterr(*c, rc, "device memory for the gradient workspace");
+    launch_fill(*c, run.W->gblk, c->partials_n, 0.0);
+    launch_fill(*c, run.W->ablk, c->partials_n, 0.0);
+    dsc = run.W->dsc;
+  }
+  launch_fill(*c, dsc, 16, 0.0);
   // state at t = 0: q0 from the context, Sigma = 0 (enzyme_adjoint.jl:51)
-  launch_fill(*c, c->S, c->N, 0.0);
-  GradRun run{*c, *W, dt, gamma, alpha, n_iter, solver};
+  launch_fill(*c, c->S, N, 0.0);
   run.obj = objective;
   const double* pp = prm ? prm : KE_WINDOW_DEFAULT;
   for (int k = 0; k < 4; ++k) run.prm[k] = pp[k];
   run.sch.build(n_steps, slots - 1);
   run.save(0);
   run.reverse_range(0, n_steps, 0, slots - 1);
-  // W->QBnext now holds dJ/dq0
-  launch_sum_blocks(*c, W->gblk, nb, W->dsc + 3);
-  launch_sum_blocks(*c, W->ablk, nb, W->dsc + 4);
+  // the last reversed step left dJ/dq0 in QBnext
+  if (c->dim == 1) {
+    qb = run.W1->base.QBnext;
+  } else {
+    const int nb = grid2_blocks(*c);
+    launch_sum_blocks(*c, run.W->gblk, nb, dsc + 3);
+    launch_sum_blocks(*c, run.W->ablk, nb, dsc + 4);
+    qb = run.W->QBnext;
+  }
   double h[5];
-  HIPCHK(*c, hipMemcpyAsync(h, W->dsc, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (qbar0) HIPCHK(*c, hipMemcpyAsync(qbar0, W->QBnext, 4 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(*c, hipMemcpyAsync(h, dsc, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (qbar0) HIPCHK(*c, hipMemcpyAsync(qbar0, qb, nv * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   rc = end_call(*c);
   if (rc) return rc;
   if (J) *J = h[0];
@@ -624,5 +712,7 @@ namespace igr {
 void free_grad_work(const Ctx* c) {
   auto it = g_work.find(c);
   if (it != g_work.end()) { delete it->second; g_work.erase(it); }
+  auto it1 = g_work1.find(c);
+  if (it1 != g_work1.end()) { delete it1->second; g_work1.erase(it1); }
 }
 }  // namespace igr

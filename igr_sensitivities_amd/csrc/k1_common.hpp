@@ -50,9 +50,53 @@ struct K1Args {
   double* ts2[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
   double* tvs[K1_MAXTAN] = {nullptr, nullptr, nullptr, nullptr};
   double* bs = nullptr;    // tangent elliptic right-hand side (N)
+  // optional copies of every solver iterate (the adjoint's recompute):
+  // PCG p_k, A p_k, r_k and (rz, pAp, a, beta); Jacobi x_k, A x_k;
+  // Chebyshev x_k (at k-1), r_k
+  double* stP = nullptr;
+  double* stAP = nullptr;
+  double* stR = nullptr;
+  double* stsc = nullptr;
   const double* x_in = nullptr;   // OP_APPLY
   double* y_out = nullptr;
   int* flag = nullptr;
+};
+
+// 1D discrete adjoint (k1_adjoint.hip): one stored SSP-RK3 step and the
+// adjoint work of one instance.
+enum K1AdjOp { OPA_STORE = 0, OPA_REV = 1, OPA_OBJ = 2 };
+struct K1AdjArgs {
+  K1Args f;                    // forward arguments; f.q = q^n (Q[0])
+  int op = OPA_STORE;
+  double dt = 0.0;
+  double* Q[3] = {nullptr, nullptr, nullptr};            // q^n, q1, q2 (3N each)
+  double* Sg[4] = {nullptr, nullptr, nullptr, nullptr};  // Sigma_0 .. Sigma_3
+  double* invR[3] = {nullptr, nullptr, nullptr};         // per stage: 1/rho, 1/M, b
+  double* invM[3] = {nullptr, nullptr, nullptr};
+  double* b[3] = {nullptr, nullptr, nullptr};
+  double* P[3] = {nullptr, nullptr, nullptr};            // per stage: solver iterates
+  double* AP[3] = {nullptr, nullptr, nullptr};
+  double* R[3] = {nullptr, nullptr, nullptr};
+  double* sc[3] = {nullptr, nullptr, nullptr};
+  double* out = nullptr;       // q^{n+1} (OPA_STORE)
+  double* QBnext = nullptr;    // adjoint of q^{n+1} (3N; the objective gradient for OPA_OBJ)
+  double* QBn = nullptr;       // adjoint of q^n (3N)
+  double* QBa = nullptr;
+  double* QBb = nullptr;
+  double* Xbar = nullptr;      // N each
+  double* Scarry = nullptr;
+  double* w = nullptr;
+  double* y = nullptr;
+  double* rb = nullptr;
+  double* pb = nullptr;
+  double* xb = nullptr;
+  double* Ab = nullptr;        // 3N
+  double* irbar = nullptr;
+  double* imbar = nullptr;
+  double* bbar = nullptr;
+  double* dsc = nullptr;       // [0] J, [3] gamma-bar, [4] alpha-bar
+  int has_sbar = 0;
+  int obj = 0;
 };
 
 #pragma clang fp contract(off)

@@ -13,371 +13,9 @@
 // :307-430 (pcg/jacobi/chebyshev), forward.jl:50-87 (run_forward!).
 #include "igr_ctx.hpp"
 #include "k1_common.hpp"
+#include "k1_forward.hpp"
 
 namespace igr {
-
-template <int NP>
-struct K1 {
-  const K1Args& a;
-  const int N, Ne, tid, nt;
-  __device__ K1(const K1Args& args) : a(args), N(args.Ne * NP), Ne(args.Ne), tid(threadIdx.x), nt(blockDim.x) {}
-
-  __device__ double D(int i, int k) const { return a.B.D[i * NP + k]; }
-
-  // y = A x (apply_sip!, elliptic.jl:32-116) in gather form.  x must be
-  // complete (barrier) before the call; y is complete when it returns.
-  __device__ void apply(const double* x, double* y) {
-    const double al = a.alpha, invJ = a.invJ;
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      double dxi = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dxi += D(i, k) * x[e * NP + k];
-      a.g[n] = al * a.B.w[i] * invJ * dxi * a.invR[n];
-      if (i == 0) {   // face e: between element e-1 (L) and e (R)
-        const int eL = e == 0 ? Ne - 1 : e - 1;
-        double dL = 0.0;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) dL += D(NP - 1, k) * x[eL * NP + k];
-        const double irL = a.invR[eL * NP + NP - 1], irR = a.invR[n];
-        a.fjump[e] = x[eL * NP + NP - 1] - x[n];
-        a.favg[e] = invJ * (irL * dL + irR * dxi) / 2.0;
-        a.fsig[e] = a.eta * fmax(irL, irR) / a.dx;
-      }
-    }
-    __syncthreads();
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      const int eR = e == Ne - 1 ? 0 : e + 1;
-      double v = a.B.w[i] * a.J * x[n] * a.invR[n];
-      double st = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) st += D(k, i) * a.g[e * NP + k];
-      v += st;
-      // symmetry terms of the left face (this element is R) and right face (L)
-      v += -al * (a.invR[e * NP] / 2.0) * invJ * D(0, i) * a.fjump[e];
-      v += -al * (a.invR[e * NP + NP - 1] / 2.0) * invJ * D(NP - 1, i) * a.fjump[eR];
-      if (i == 0) v += al * a.favg[e] - al * a.fsig[e] * a.fjump[e];
-      if (i == NP - 1) v += -al * a.favg[eR] + al * a.fsig[eR] * a.fjump[eR];
-      y[n] = v;
-    }
-    __syncthreads();
-  }
-
-  __device__ double dot(const double* u, const double* v) {
-    __shared__ double sred[24];
-    double s[1] = {0.0};
-    for (int n = tid; n < N; n += nt) s[0] += u[n] * v[n];
-    block_sum<1>(s, sred);
-    if (tid == 0) sred[20] = s[0];
-    __syncthreads();
-    const double r = sred[20];
-    __syncthreads();
-    return r;
-  }
-
-  // 1/rho, IGR source b and diag(A) of stage state qs
-  // (compute_igr_rhs!, elliptic.jl:126-154; compute_sip_diagonal, :206-253)
-  __device__ void prepare(const double* qs) {
-    const double* m = qs;
-    const double* rho = qs + N;
-    const double al = a.alpha, invJ = a.invJ;
-    for (int n = tid; n < N; n += nt) {
-      a.invR[n] = 1.0 / rho[n];
-      a.u[n] = m[n] / rho[n];
-    }
-    __syncthreads();
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      double du = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) du += D(i, k) * a.u[e * NP + k];
-      du *= invJ;
-      a.b[n] = a.B.w[i] * a.J * (2.0 * al * (du * du));
-      const double ir = a.invR[n];
-      double Md = a.B.w[i] * a.J * ir;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) Md += al * a.B.w[k] * invJ * (D(k, i) * D(k, i)) * a.invR[e * NP + k];
-      if (i == NP - 1) {
-        const int eR = e == Ne - 1 ? 0 : e + 1;
-        const double sig = a.eta * fmax(ir, a.invR[eR * NP]) / a.dx;
-        Md += -al * invJ * D(NP - 1, NP - 1) * ir;
-        Md += al * sig;
-      }
-      if (i == 0) {
-        const int eL = e == 0 ? Ne - 1 : e - 1;
-        const double sig = a.eta * fmax(a.invR[eL * NP + NP - 1], ir) / a.dx;
-        Md += al * invJ * D(0, 0) * ir;
-        Md += al * sig;
-      }
-      a.invM[n] = 1.0 / Md;
-    }
-    __syncthreads();
-  }
-
-  // fixed-iteration solve A x = b with the prepared 1/rho and diag(A)
-  // (pcg_fixed! :307-339, jacobi_fixed! :359-368, chebyshev_fixed! :389-430)
-  __device__ void iterate(double* x, const double* bb, int trace) {
-    if (a.solver == 0) {
-      apply(x, a.Ap);
-      for (int n = tid; n < N; n += nt) {
-        const double r = bb[n] - a.Ap[n];
-        a.r[n] = r;
-        a.p[n] = r * a.invM[n];
-      }
-      __syncthreads();
-      double rz = dot(a.r, a.p);
-      for (int it = 0; it < a.n_iter; ++it) {
-        apply(a.p, a.Ap);
-        const double pAp = dot(a.p, a.Ap);
-        const double ak = rz / (pAp + 1e-30);
-        for (int n = tid; n < N; n += nt) {
-          x[n] = x[n] + ak * a.p[n];
-          a.r[n] = a.r[n] - ak * a.Ap[n];
-          a.z[n] = a.r[n] * a.invM[n];
-        }
-        __syncthreads();
-        const double rzn = dot(a.r, a.z);
-        const double bk = rzn / (rz + 1e-30);
-        if (trace && tid == 0 && a.trace) { a.trace[it] = ak; a.trace[a.n_iter + it] = bk; }
-        for (int n = tid; n < N; n += nt) a.p[n] = a.z[n] + bk * a.p[n];
-        __syncthreads();
-        rz = rzn;
-      }
-    } else if (a.solver == 1) {
-      for (int it = 0; it < a.n_iter; ++it) {
-        apply(x, a.Ap);
-        for (int n = tid; n < N; n += nt) x[n] = x[n] + (2.0 / 3.0) * (bb[n] - a.Ap[n]) * a.invM[n];
-        __syncthreads();
-      }
-    } else {
-      const double lmin = 1.0, lmax = a.cheb_lmax;
-      const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
-      const double sigma = theta / fmax(delta, 1e-10);
-      apply(x, a.Ap);
-      for (int n = tid; n < N; n += nt) a.r[n] = bb[n] - a.Ap[n];
-      __syncthreads();
-      double rc = 1.0 / sigma;
-      for (int k = 1; k <= a.n_iter; ++k) {
-        double c1 = 0.0, c2 = 0.0;
-        if (k > 1) { const double rn = 1.0 / (2.0 * sigma - rc); c1 = rn * rc; c2 = 2.0 * rn / delta; rc = rn; }
-        for (int n = tid; n < N; n += nt) {
-          const double z = a.r[n] * a.invM[n];
-          const double d = (k == 1) ? z / theta : c1 * a.p[n] + c2 * z;
-          a.p[n] = d;
-          x[n] = x[n] + d;
-        }
-        __syncthreads();
-        apply(x, a.Ap);
-        for (int n = tid; n < N; n += nt) a.r[n] = bb[n] - a.Ap[n];
-        __syncthreads();
-      }
-    }
-  }
-
-  // solve_elliptic! (elliptic.jl:451-467) on state qs, warm start a.S
-  __device__ void solve(const double* qs, int trace) {
-    prepare(qs);
-    iterate(a.S, a.b, trace);
-  }
-
-  // right-hand side of the tangent elliptic equation into a.bs
-  // (form 1: sensitivity.jl:168-256, SIP-like faces; form 2: :272-314, pointwise)
-  __device__ void sens_rhs(const double* qs, const double* ss) {
-    const double al = a.alpha, invJ = a.invJ, J = a.J;
-    const double* m = qs;
-    const double* r = qs + N;
-    const double* sm = ss;
-    const double* sr = ss + N;
-    const double* S = a.S;
-    if (a.form == 2) {
-      for (int n = tid; n < N; n += nt) {
-        const int i = n % NP, e = n / NP;
-        double ux = 0.0, sux = 0.0, dso = 0.0, dS = 0.0;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          const int nk = e * NP + k;
-          const double uj = m[nk] / r[nk];
-          const double suj = (sm[nk] - uj * sr[nk]) / r[nk];
-          ux += D(i, k) * uj; sux += D(i, k) * suj;
-          dso += D(i, k) * (sr[nk] / r[nk]);
-          dS += D(i, k) * S[nk];
-        }
-        ux *= invJ; sux *= invJ; dso *= invJ; dS *= invJ;
-        const double so = sr[n] / r[n];
-        const double R = al * (2.0 * ux * (2.0 * sux + so * ux) - dso * dS / r[n]);
-        a.bs[n] = a.B.w[i] * J * R;
-      }
-      __syncthreads();
-      return;
-    }
-    // form 1: faces first (kappa = s_rho / rho^2 diffusion of Sigma)
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      if (i != 0) continue;
-      const int eL = e == 0 ? Ne - 1 : e - 1;
-      const int nl = eL * NP + NP - 1;
-      const double kL = sr[nl] / (r[nl] * r[nl]), kR = sr[n] / (r[n] * r[n]);
-      double dL = 0.0, dR = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) { dL += D(NP - 1, k) * S[eL * NP + k]; dR += D(0, k) * S[e * NP + k]; }
-      a.fjump[e] = S[nl] - S[n];
-      a.favg[e] = invJ * (kL * dL + kR * dR) / 2.0;
-      a.fsig[e] = a.eta * fmax_first(kL, kR) / a.dx;
-    }
-    // kappa * dSigma at every node (volume stiffness-like term)
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      double dS = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dS += D(i, k) * S[e * NP + k];
-      a.g[n] = al * a.B.w[i] * invJ * (sr[n] / (r[n] * r[n])) * dS;
-    }
-    __syncthreads();
-    for (int n = tid; n < N; n += nt) {
-      const int i = n % NP, e = n / NP;
-      const int eR = e == Ne - 1 ? 0 : e + 1;
-      double ux = 0.0, sux = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const int nk = e * NP + k;
-        const double uj = m[nk] / r[nk];
-        const double suj = (sm[nk] - uj * sr[nk]) / r[nk];
-        ux += D(i, k) * uj; sux += D(i, k) * suj;
-      }
-      ux *= invJ; sux *= invJ;
-      const double src = 4.0 * al * ux * sux + sr[n] * S[n] / (r[n] * r[n]);
-      double v = a.B.w[i] * J * src;
-      double st = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) st += a.g[e * NP + k] * D(k, i);
-      v += st;
-      const int n0 = e * NP, n1 = e * NP + NP - 1;
-      const double k0 = sr[n0] / (r[n0] * r[n0]), k1 = sr[n1] / (r[n1] * r[n1]);
-      v += -al * (k0 / 2.0) * invJ * D(0, i) * a.fjump[e];
-      v += -al * (k1 / 2.0) * invJ * D(NP - 1, i) * a.fjump[eR];
-      if (i == 0) v += al * a.favg[e] - al * a.fsig[e] * a.fjump[e];
-      if (i == NP - 1) v += -al * a.favg[eR] + al * a.fsig[eR] * a.fjump[eR];
-      a.bs[n] = v;
-    }
-    __syncthreads();
-  }
-
-  // sensitivity hyperbolic RHS (sensitivity.jl:91-150) + SSP-RK3 stage of a tangent
-  __device__ void sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn, int mode,
-                                 double cdt, double* out);
-
-  // hyperbolic RHS (hyperbolic.jl:23-83) fused with the SSP-RK3 stage update
-  __device__ void hyp_stage(const double* qs, int mode, double cdt, double* out);
-};
-
-#pragma clang fp contract(off)
-template <int NP>
-__device__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double* out) {
-  const double g = a.gamma, invJ = a.invJ;
-  for (int n = tid; n < N; n += nt) {
-    double f[3];
-    flux1(g, qs[n], qs[N + n], qs[2 * N + n], a.S[n], f);
-    a.F[n] = f[0]; a.F[N + n] = f[1]; a.F[2 * N + n] = f[2];
-    const int i = n % NP, e = n / NP;
-    if (i == 0) {     // LLF at face e (between e-1 and e), llf_flux euler.jl:64-78
-      const int eL = e == 0 ? Ne - 1 : e - 1;
-      const int nl = eL * NP + NP - 1;
-      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], a.S[nl]};
-      const double qR[4] = {qs[n], qs[N + n], qs[2 * N + n], a.S[n]};
-      const double lam = fmax_first(wave1(g, qL, a.flag), wave1(g, qR, a.flag));
-      double fL[3], fR[3];
-      flux1(g, qL[0], qL[1], qL[2], qL[3], fL);
-      flux1(g, qR[0], qR[1], qR[2], qR[3], fR);
-#pragma unroll
-      for (int v = 0; v < 3; ++v) a.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
-    }
-  }
-  __syncthreads();
-  for (int n = tid; n < N; n += nt) {
-    const int i = n % NP, e = n / NP;
-    const int eR = e == Ne - 1 ? 0 : e + 1;
-    double acc[3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      acc[v] = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * a.F[v * N + e * NP + k];
-    }
-    if (i == NP - 1)
-#pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (a.fstar[v * Ne + eR] - a.F[v * N + n]) / a.B.w[NP - 1];
-    if (i == 0)
-#pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] += invJ * (a.fstar[v * Ne + e] - a.F[v * N + n]) / a.B.w[0];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      const size_t gi = (size_t)v * N + n;
-      double o;
-      if (mode == 0) o = acc[v];
-      else if (mode == 1) o = qs[gi] + cdt * acc[v];
-      else if (mode == 2) o = 3.0 / 4.0 * a.q[gi] + 1.0 / 4.0 * qs[gi] + cdt * acc[v];
-      else o = 1.0 / 3.0 * a.q[gi] + 2.0 / 3.0 * qs[gi] + cdt * acc[v];
-      out[gi] = o;
-    }
-  }
-  __syncthreads();
-}
-template <int NP>
-__device__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
-                                       int mode, double cdt, double* out) {
-  const double g = a.gamma, invJ = a.invJ;
-  for (int n = tid; n < N; n += nt) {
-    const double qq[4] = {qs[n], qs[N + n], qs[2 * N + n], a.S[n]};
-    const double sv[4] = {ss[n], ss[N + n], ss[2 * N + n], svs[n]};
-    double f[3];
-    sflux1(g, qq, sv, a.sgam, f);
-    a.F[n] = f[0]; a.F[N + n] = f[1]; a.F[2 * N + n] = f[2];
-    const int i = n % NP, e = n / NP;
-    if (i == 0) {     // sensitivity_llf_flux at face e, primal wave speed (sensitivity.jl:58-78)
-      const int eL = e == 0 ? Ne - 1 : e - 1;
-      const int nl = eL * NP + NP - 1;
-      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], a.S[nl]};
-      const double sL[4] = {ss[nl], ss[N + nl], ss[2 * N + nl], svs[nl]};
-      const double lam = fmax_first(wave1(g, qL, a.flag), wave1(g, qq, a.flag));
-      double fL[3], fR[3];
-      sflux1(g, qL, sL, a.sgam, fL);
-      sflux1(g, qq, sv, a.sgam, fR);
-#pragma unroll
-      for (int v = 0; v < 3; ++v) a.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sv[v]) / 2.0;
-    }
-  }
-  __syncthreads();
-  for (int n = tid; n < N; n += nt) {
-    const int i = n % NP, e = n / NP;
-    const int eR = e == Ne - 1 ? 0 : e + 1;
-    double acc[3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      acc[v] = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * a.F[v * N + e * NP + k];
-    }
-    if (i == NP - 1)
-#pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (a.fstar[v * Ne + eR] - a.F[v * N + n]) / a.B.w[NP - 1];
-    if (i == 0)
-#pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] += invJ * (a.fstar[v * Ne + e] - a.F[v * N + n]) / a.B.w[0];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      const size_t gi = (size_t)v * N + n;
-      double o;
-      if (mode == 0) o = acc[v];
-      else if (mode == 1) o = ss[gi] + cdt * acc[v];
-      else if (mode == 2) o = 3.0 / 4.0 * sn[gi] + 1.0 / 4.0 * ss[gi] + cdt * acc[v];
-      else o = 1.0 / 3.0 * sn[gi] + 2.0 / 3.0 * ss[gi] + cdt * acc[v];
-      out[gi] = o;
-    }
-  }
-  __syncthreads();
-}
-#pragma clang fp contract(fast)
 
 // Whole-run kernel: `op` selects what one launch does.
 //  OP_RUN       Sigma <- 0 (if zero_sigma), n_steps SSP-RK3 steps, optional final solve

@@ -154,11 +154,15 @@ enum igr_objective {
 /* Objective value on the context state (2D). */
 int igr_objective(igr_ctx* ctx, int objective, const double* prm, double gamma, double* J);
 
-/* Discrete adjoint (2D): enzyme_adjoint_ic(q0..., n_steps, dt, basis, mesh,
+/* Discrete adjoint: enzyme_adjoint_ic(q0..., n_steps, dt, basis, mesh,
  * gamma, alpha; n_iter, solver, objective) (IGR_adjoints_2d/enzyme_adjoint.jl:
- * 270-296) extended to gamma and alpha.  q0 is the context state; the map is
- * the Enzyme objectives' loop (Sigma <- 0, n_steps fixed steps of dt).
- * Returns J, dJ/dq0 (4N, conservative variables; may be NULL), dJ/dgamma
+ * 270-296; IGR_adjoints_1d/enzyme_adjoint.jl:255-314) extended to gamma and
+ * alpha.  q0 is the context state; the map is the Enzyme objectives' loop
+ * (Sigma <- 0, n_steps fixed steps of dt).  1D objectives: L2_DENSITY,
+ * L2_PRESSURE, KINETIC_ENERGY, WEIGHTED_MOMENTUM (1D enzyme_adjoint.jl:65-71,
+ * 110-117); the 1D gradient is returned in conservative variables (the
+ * reference's primitive form, :306-311, is a pointwise map of it).
+ * Returns J, dJ/dq0 (nv*N, conservative variables; may be NULL), dJ/dgamma
  * and dJ/dalpha.  ckpt_slots: number of (q, Sigma) checkpoints kept in HBM
  * (<= 0: as many as half the free HBM allows, up to n_steps); steps between
  * checkpoints are recomputed (optimal binomial schedule).  On return the
