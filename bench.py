@@ -91,6 +91,41 @@ def ensemble_leg(igr, configs):
     }
 
 
+def c5_leg(igr, configs, budget_slots=40):
+    """C5: 256x256 blast beside the Oseen vortex pair, PCG-60, 400 steps: the
+    forward, then the gradient of the disc kinetic energy w.r.t. the initial
+    state, gamma and alpha, once with every step checkpointed and once under
+    a budget of `budget_slots` checkpoints (binomial recomputation)."""
+    cfg = configs.config5()
+    N1, ns = cfg["N"], cfg["n_steps"]
+    ctx = igr.Context(igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
+    gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
+    obj = igr.OBJECTIVES["disc_kinetic_energy"]
+    out = {"workload": "2D 256x256 p=3, Gaussian blast (peak 100, sigma 0.03) beside a counter-rotating Oseen "
+                       "pair (core 0.05, vortex Mach 0.5), PCG-60, 400 SSP-RK3 steps; J = kinetic energy in a disc",
+           "steps": ns}
+    try:
+        ctx.set_state(cfg["q"])
+        ctx.run_steps(2, dt, gam, al, n_iter, "pcg")
+        ctx.set_state(cfg["q"])
+        t0 = time.perf_counter()
+        ctx.run_steps(ns, dt, gam, al, n_iter, "pcg")
+        out["forward_s"] = time.perf_counter() - t0
+        out["dof_stage_per_s"] = 4 * 16 * N1 * N1 * 3 * ns / out["forward_s"]
+        for tag, slots in (("all_steps", 0), (f"{budget_slots}_slots", budget_slots)):
+            ctx.set_state(cfg["q"])
+            t0 = time.perf_counter()
+            J, _, dg, da = ctx.gradient(obj, ns, dt, gam, al, n_iter, "pcg", prm=cfg["disc"], ckpt_slots=slots)
+            t = time.perf_counter() - t0
+            out[f"gradient_{tag}_s"] = t
+            out[f"gradient_{tag}_over_forward"] = t / out["forward_s"]
+            out.update({"J": J, "dJ_dgamma": dg, "dJ_dalpha": da})
+    except (igr.DomainError, igr.IGRError, ValueError) as e:
+        out["error"] = f"{type(e).__name__}: {e}"
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +134,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8, help="SSP-RK3 steps of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 256x256 blast-vortex forward + gradient leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by 2D block "
@@ -153,6 +189,7 @@ def main():
     t_grad = g1 - g0
 
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
+    c5 = None if args.no_c5 else c5_leg(igr, configs)
 
     cpu = None
     if not args.no_cpu:
@@ -205,6 +242,7 @@ def main():
         "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,
                      "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},
         "ensemble": ens,
+        "blast_vortex_256": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

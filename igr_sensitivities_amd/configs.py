@@ -15,7 +15,8 @@ here are the ones this library is measured on:
   C3  C2 with J = sum w w Jx Jy rho(T) exp(-|x - c_w|^2 / 0.1^2), c_w seeded.
   C4  64 instances, 1D and y-uniform 2D of mixed sizes, shifted C1 data,
       4 tangent directions each, run as one batch.
-  C5  2D blast + Lamb-Oseen vortex, 256x256, PCG-60, 400 steps.
+  C5  2D blast beside the reference's Oseen vortex pair, 256x256, PCG-60,
+      400 steps, gradient of the kinetic energy in a disc.
 
 The time step follows the reference scripts' formula
 dt = CFL * dx / ((2p+1) * max(|u| + c)) (IGR_adjoints_1d/enzyme_adjoint.jl:272-273,
@@ -251,38 +252,74 @@ def config3_objective(seed=3):
     return (float(c[0]), float(c[1]), 0.1, 0.0)
 
 
+def oseen_pair_blast(x, y, gamma, A=100.0, sigma=0.03, a_core=0.05, M_star=0.5,
+                     centers=((0.45, 0.60, 1.0), (0.45, 0.40, -1.0)), blast=(0.25, 0.50), L=1.0):
+    """Counter-rotating pair of Oseen vortices plus a Gaussian over-pressure
+    blast, as build_oseen_pair_blast_ic! (IGR_adjoints_2d/initial_conditions.jl:
+    152-232) sets it up: circulation from the vortex Mach number (v_max =
+    0.6383 Gamma / (2 pi a)), velocity summed over the 3x3 periodic images,
+    homentropic radial equilibrium through the tabulated enthalpy defect
+    D(r) = (Gamma/2pi)^2 int_r^inf (1 - exp(-s^2/a^2))^2 / s^3 ds (4000 points
+    on [0, 2L], trapezoid rule from the outer end, linear interpolation).
+    Returns q = (mx, my, rho, E)."""
+    Gam = M_star * np.sqrt(gamma) * 2 * np.pi * a_core / 0.6383
+    r_max, n_tab = 2.0 * L, 4000
+    r_tab = np.linspace(0.0, r_max, n_tab)
+    f = np.zeros(n_tab)
+    nz = r_tab >= 1e-12
+    f[nz] = (1.0 - np.exp(-r_tab[nz] ** 2 / a_core ** 2)) ** 2 / r_tab[nz] ** 3
+    seg = 0.5 * (f[:-1] + f[1:]) * np.diff(r_tab)
+    D_tab = np.zeros(n_tab)
+    D_tab[:-1] = (Gam / (2 * np.pi)) ** 2 * np.cumsum(seg[::-1])[::-1]
+
+    def defect(r):
+        t = r / r_max * (n_tab - 1)
+        i = np.clip(np.floor(t).astype(int), 0, n_tab - 2)
+        w = t - i
+        return np.where(r >= r_max, 0.0, (1 - w) * D_tab[i] + w * D_tab[i + 1])
+
+    ux = np.zeros_like(x)
+    uy = np.zeros_like(x)
+    h = np.full_like(x, gamma / (gamma - 1.0))
+    for xc, yc, sgn in centers:
+        for sx in (-1, 0, 1):
+            for sy in (-1, 0, 1):
+                dx = x - (xc + sx * L)
+                dy = y - (yc + sy * L)
+                r2 = dx * dx + dy * dy
+                vr = np.where(r2 < 1e-24, 0.0, sgn * Gam / (2 * np.pi * np.maximum(r2, 1e-300))
+                              * (1.0 - np.exp(-r2 / a_core ** 2)))
+                ux += -vr * dy
+                uy += vr * dx
+        dx = x - xc
+        dy = y - yc
+        dx -= L * np.round(dx / L)
+        dy -= L * np.round(dy / L)
+        h -= defect(np.sqrt(dx * dx + dy * dy))
+    p_vort = ((gamma - 1) * h / gamma) ** (gamma / (gamma - 1))
+    rho = p_vort ** (1.0 / gamma)
+    pr = p_vort + A * np.exp(-((x - blast[0]) ** 2 + (y - blast[1]) ** 2) / sigma ** 2)
+    return np.stack([rho * ux, rho * uy, rho, pr / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy)])
+
+
 def config5(N=256, p=3, seed=5, gamma=1.4, n_steps=400, cfl=0.4):
-    """C5 Gaussian pressure blast (peak 100, sigma 0.03) beside a Lamb-Oseen vortex
-    (circulation 1, core radius 0.05) on rho=1, p=1.  The vortex pressure comes
-    from radial equilibrium dp/dr = rho v^2/r integrated on a fine radial grid."""
+    """C5 blast beside vortices at 256x256: the Gaussian blast of peak 100 and
+    sigma 0.03 next to the reference's counter-rotating Oseen pair (core radius
+    0.05, vortex Mach number 0.5; oseen_pair_blast), PCG-60, 400 steps, and
+    the kinetic energy in a seeded disc (radius 0.15 around the pair) as the
+    objective of the gradient with respect to gamma and alpha.
+
+    A single Lamb-Oseen vortex of circulation 1 on rho = p = 1 is not an
+    admissible state: its core pressure from radial equilibrium is negative,
+    and a net circulation is not periodic.  The pair of the reference's own
+    blast-vortex builder keeps zero net circulation and a positive pressure."""
     rng = np.random.default_rng(seed)
     x, y = coords2(p, N, N)
-    bx, by = rng.uniform(0.25, 0.35), rng.uniform(0.45, 0.55)
-    vx, vy = rng.uniform(0.65, 0.75), rng.uniform(0.45, 0.55)
-    Gam, a = 1.0, 0.05
-    r2 = periodic_d2(x, y, vx, vy)
-    r = np.sqrt(r2)
-    rs = np.maximum(r, 1e-12)
-    vth = Gam / (2 * np.pi * rs) * (1.0 - np.exp(-r2 / a ** 2))
-    dxv = (x - vx + 0.5) % 1.0 - 0.5
-    dyv = (y - vy + 0.5) % 1.0 - 0.5
-    u = -vth * dyv / rs
-    v = vth * dxv / rs
-    rr = np.linspace(0.0, 1.0, 20001)
-    vr = Gam / (2 * np.pi * np.maximum(rr, 1e-12)) * (1.0 - np.exp(-rr ** 2 / a ** 2))
-    integrand = np.where(rr > 0, vr ** 2 / np.maximum(rr, 1e-12), 0.0)
-    cum = np.concatenate([[0.0], np.cumsum(0.5 * (integrand[1:] + integrand[:-1]) * np.diff(rr))])
-    dp = np.interp(r, rr, cum) - cum[-1]
-    pr = 1.0 + dp + 100.0 * np.exp(-periodic_d2(x, y, bx, by) / 0.03 ** 2)
-    q = np.zeros((4,) + x.shape)
-    q[2] = 1.0
-    q[0] = u
-    q[1] = v
-    q[3] = pr / (gamma - 1.0) + 0.5 * (u * u + v * v)
+    q = oseen_pair_blast(x, y, gamma)
     h = 1.0 / N
     lam = max_wavespeed2(q, gamma)
     dt = cfl * h / ((2 * p + 1) * lam)
-    disc = (float(vx), float(vy), 0.15, 0.0)
+    disc = (float(rng.uniform(0.42, 0.48)), float(rng.uniform(0.47, 0.53)), 0.15, 0.0)
     return dict(q=q, gamma=gamma, alpha=alpha_of(h, p), dt=dt, n_steps=n_steps, n_iter=60, solver="pcg",
                 p=p, N=N, disc=disc)
 

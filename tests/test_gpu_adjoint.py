@@ -176,3 +176,23 @@ def test_gradient_c3_32_vs_oracle_and_fd(orc):
     fd_a = (Jat(gam, a * (1 + h)) - Jat(gam, a * (1 - h))) / (2 * h * a)
     close(fd_g, gb, 1e-6)
     close(fd_a, ab, 1e-5)
+
+
+def test_blast_vortex_32_forward_and_gradient(orc):
+    """The blast-beside-Oseen-pair workload at 32x32 (3 steps): forward state
+    vs the oracle, and the disc-kinetic-energy gradient vs the oracle's JVP."""
+    cfg = configs.config5(N=32)
+    q0, gam, a, dt = cfg["q"], cfg["gamma"], cfg["alpha"], cfg["dt"]
+    c = ctx2(3, 32, 32)
+    g = orc.Grid2D(3, 32, 32)
+    c.set_state(q0)
+    c.run_steps(3, dt, gam, a, 60, "pcg")
+    q3 = c.get_state()[0]
+    q3r, _ = g.run_steps(gam, a, 60, orc.PCG, 3, dt, q0)
+    assert np.max(np.abs(q3 - q3r)) <= 1e-11 * np.max(np.abs(q3r))
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(6, 3, dt, gam, a, 60, "pcg", prm=cfg["disc"])
+    u = np.random.default_rng(81).standard_normal(q0.shape) * (q0.std(axis=(1, 2, 3, 4), keepdims=True) + 1e-2)
+    Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.3, a, 0.2 * a, 60, orc.PCG, 3, dt, 6, q0, u, cfg["disc"])
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.3 * gb + 0.2 * a * ab, 1e-9)
