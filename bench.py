@@ -187,6 +187,7 @@ def main():
     J, _, dgam, dal = ctx.gradient(igr.OBJECTIVES["windowed_density"], args.steps, dt, gam, al, n_iter, "pcg", prm=prm)
     g1 = time.perf_counter()
     t_grad = g1 - g0
+    ctx.close()        # release this context's HBM (the gradient keeps whole steps) before the other legs
 
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)

@@ -31,6 +31,7 @@ PROTOTYPES = {
     "igr_last_error": (C.c_char_p, [_vp]),
     "igr_sizes": (_i, [_vp, C.POINTER(_ll), _pi]),
     "igr_set_option": (_i, [_vp, _i, _i]),
+    "igr_get_option": (_i, [_vp, _i, _pi]),
     "igr_set_state": (_i, [_vp, _pd, _pd]),
     "igr_get_state": (_i, [_vp, _pd, _pd]),
     "igr_hyperbolic_rhs": (_i, [_vp, _d, _pd, _pd, _pd]),

@@ -22,6 +22,7 @@ import numpy as np
 from . import _lib
 
 SOLVERS = {"pcg": 0, "jacobi": 1, "chebyshev": 2}
+OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE = 0, 1, 2, 3
 
 
 class DomainError(ArithmeticError):
@@ -129,6 +130,17 @@ class Context:
 
     def set_option(self, option, value):
         self.check(self._L.igr_set_option(self.h, int(option), int(value)))
+
+    def get_option(self, option):
+        v = C.c_int(0)
+        self.check(self._L.igr_get_option(self.h, int(option), C.byref(v)))
+        return v.value
+
+    def autotune(self):
+        """Time the tile shapes on the current state and keep the fastest (2D, p = 3);
+        returns (SIP tile, hyperbolic tile) as tile_x*100 + tile_y."""
+        self.set_option(OPT_AUTOTUNE, 0)
+        return self.get_option(OPT_TILE), self.get_option(OPT_HTILE)
 
     # state transfer
     def set_state(self, q, sigma=None):

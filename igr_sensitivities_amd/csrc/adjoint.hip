@@ -30,6 +30,7 @@ void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
 int grid2_blocks(const Ctx& c);
+int grid2_hblocks(const Ctx& c);
 Geo2 geo2(const Ctx& c);
 SipArgs sip_base(const Ctx& c, double alpha);
 HypArgs hyp_base(const Ctx& c, double gamma);
@@ -48,6 +49,19 @@ struct StageStore {
   double *sc = nullptr, *rs = nullptr;
 };
 
+// The stored stage states, Sigmas and solver data of one whole step.
+struct StepView {
+  StageStore* st;
+  double** Q;
+  double** Sg;
+};
+struct StepSet {
+  StageStore st[3];
+  double* Q[3] = {nullptr, nullptr, nullptr};
+  double* Sg[4] = {nullptr, nullptr, nullptr, nullptr};
+  StepView view() { return {st, Q, Sg}; }
+};
+
 // Workspace of the gradient (allocated per (n_iter) on demand, kept in the context).
 struct GradWork {
   int n_iter = -1;
@@ -62,7 +76,50 @@ struct GradWork {
   double *gblk = nullptr, *ablk = nullptr;   // per-workgroup accumulators
   double* dsc = nullptr;                     // [0] J, [1] KE, [2] M, [3] gamma-bar, [4] alpha-bar
   std::vector<double*> ck;                   // checkpoint slots (5N each)
+  // whole steps kept from the forward sweep (no recompute when reversed):
+  // one device block holding `sets`
+  std::vector<StepSet> sets;
+  void* set_block = nullptr;
+  size_t set_block_bytes = 0;
+  StepView view() { return {st, Q, Sg}; }
+  ~GradWork() {
+    if (set_block) (void)hipFree(set_block);
+  }
 };
+
+// Doubles of one StepSet for n solver iterations on N nodes.
+static size_t step_set_doubles(size_t N, int n) {
+  const size_t per_stage = (size_t)std::max(n, 1) * N * 2 + (size_t)(n + 1) * N + 3 * N +
+                           (4 * (size_t)std::max(n, 1) + 1) + (5 * (size_t)std::max(n, 1) + 1);
+  return 3 * per_stage + 3 * 4 * N + 4 * N;
+}
+
+// Carve K step sets out of one allocation (kept until the context goes).
+static int alloc_sets(Ctx& c, GradWork& W, int K) {
+  if ((int)W.sets.size() >= K) return IGR_OK;
+  const size_t N = c.N;
+  const int n = W.n_iter;
+  const size_t per = step_set_doubles(N, n) + 64;
+  if (W.set_block) { (void)hipFree(W.set_block); W.set_block = nullptr; W.sets.clear(); }
+  const size_t bytes = per * sizeof(double) * (size_t)K;
+  if (hipMalloc(&W.set_block, bytes) != hipSuccess) { W.set_block = nullptr; return IGR_ENOMEM; }
+  W.set_block_bytes = bytes;
+  double* p = (double*)W.set_block;
+  auto take = [&](size_t cnt) { double* r = p; p += (cnt + 31) / 32 * 32; return r; };
+  W.sets.resize(K);
+  for (int k = 0; k < K; ++k) {
+    StepSet& S = W.sets[k];
+    for (int st = 0; st < 3; ++st) {
+      StageStore& t = S.st[st];
+      t.P = take((size_t)std::max(n, 1) * N); t.AP = take((size_t)std::max(n, 1) * N); t.R = take((size_t)(n + 1) * N);
+      t.invR = take(N); t.invM = take(N); t.b = take(N);
+      t.sc = take(4 * (size_t)std::max(n, 1) + 1); t.rs = take(5 * (size_t)std::max(n, 1) + 1);
+    }
+    for (int q = 0; q < 3; ++q) S.Q[q] = take(4 * N);
+    for (int q = 0; q < 4; ++q) S.Sg[q] = take(N);
+  }
+  return IGR_OK;
+}
 
 static std::map<const Ctx*, GradWork*> g_work;
 
@@ -71,7 +128,8 @@ static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
   const size_t N = c.N;
   const int n = std::max(n_iter, 0);
   if (!w || w->n_iter != n) {
-    w = new GradWork();   // earlier buffers stay owned by the context until igr_destroy
+    delete w;             // its c.alloc buffers stay owned by the context until igr_destroy
+    w = new GradWork();
     w->n_iter = n;
     for (int s = 0; s < 3; ++s) {
       StageStore& st = w->st[s];
@@ -231,7 +289,7 @@ static void solve2_store(Ctx& c, const double* qs, double* x, double alpha, int 
 
 // One forward step from (q^n in W.Q[0], Sigma_0 in W.Sg[0]) storing what the
 // reverse needs; q^{n+1} is written to out (4N), Sigma_3 stays in W.Sg[3].
-static void step_store(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, int solver,
+static void step_store(Ctx& c, StepView W, double dt, double gamma, double alpha, int n_iter, int solver,
                        double* out) {
   const size_t N = c.N;
   const double cdt[3] = {dt, 1.0 / 4.0 * dt, 2.0 / 3.0 * dt};
@@ -352,7 +410,7 @@ static void solve2_rev(Ctx& c, GradWork& W, const double* qs, const double* x0, 
 // Reverse of one step stored by step_store.  W.QBnext holds the adjoint of
 // q^{n+1}; sbar_next the adjoint of Sigma_3 (null if none).  On return W.QBn
 // holds the adjoint of q^n and W.Scarry the adjoint of Sigma_0.
-static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha, int n_iter, int solver,
+static void step_rev(Ctx& c, GradWork& W, StepView V, double dt, double gamma, double alpha, int n_iter, int solver,
                      bool has_sbar) {
   const size_t N = c.N;
   launch_fill(c, W.QBn, 4 * N, 0.0);
@@ -367,7 +425,7 @@ static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha,
   for (int s = 2; s >= 0; --s) {
     HypRevArgs h;
     h.g = geo2(c); h.B = c.basis; h.gamma = gamma;
-    h.q = W.Q[s]; h.S = W.Sg[s + 1];
+    h.q = V.Q[s]; h.S = V.Sg[s + 1];
     h.qbar_out = sg[s].qbar_out; h.qbar_stage = sg[s].qbar_stage; h.accumulate = sg[s].acc;
     h.qbar_n = (s > 0) ? W.QBn : nullptr;
     h.sbar_in = sbar_valid ? W.Scarry : nullptr; h.sbar_out = W.Xbar;
@@ -375,7 +433,7 @@ static void step_rev(Ctx& c, GradWork& W, double dt, double gamma, double alpha,
     h.gbar_blk = W.gblk;
     launch2_hyp_rev(c, h);
     if (solve) {
-      solve2_rev(c, W, W.Q[s], W.Sg[s], W.Xbar, alpha, n_iter, solver, W.st[s], sg[s].qbar_stage, W.Scarry);
+      solve2_rev(c, W, V.Q[s], V.Sg[s], W.Xbar, alpha, n_iter, solver, V.st[s], sg[s].qbar_stage, W.Scarry);
       sbar_valid = true;
     }
   }
@@ -424,7 +482,10 @@ struct GradRun {
   double prm[4];
   bool first_base = true;
   long long fwd_steps = 0;
+  int n_total = 0;
+  int kfirst = 1 << 30;   // 2D: steps kfirst .. n_total-2 keep their iterates from the forward sweep
 
+  bool kept(int m) const { return c.dim == 2 && m >= kfirst && m < n_total - 1; }
   double* slot(int k) const { return c.dim == 1 ? W1->ck[k] : W->ck[k]; }
   void save(int k) {
     launch_copy(c, slot(k), c.q, c.nv * c.N);
@@ -434,8 +495,24 @@ struct GradRun {
     launch_copy(c, c.q, slot(k), c.nv * c.N);
     launch_copy(c, c.S, slot(k) + c.nv * c.N, c.N);
   }
-  void advance(int steps) {
+  // advance from step index `at` by `steps` steps (the state is in c.q, c.S)
+  void advance(int at, int steps) {
     if (steps <= 0) return;
+    if (c.dim == 2) {
+      for (int m = at; m < at + steps; ++m) {
+        if (kept(m)) {
+          StepSet& S = W->sets[m - kfirst];
+          launch_copy(c, S.Q[0], c.q, 4 * c.N);
+          launch_copy(c, S.Sg[0], c.S, c.N);
+          step_store(c, S.view(), dt, gamma, alpha, n_iter, solver, c.q);
+          launch_copy(c, c.S, S.Sg[3], c.N);
+        } else {
+          step2(c, dt, gamma, alpha, n_iter, solver);
+        }
+      }
+      fwd_steps += steps;
+      return;
+    }
     if (c.dim == 1) {
       K1Args a = k1_base(c, gamma, alpha, n_iter, solver);
       a.op = OP_RUN; a.n_steps = steps; a.dt = dt; a.zero_sigma = 0; a.final_solve = 0;
@@ -445,8 +522,15 @@ struct GradRun {
     }
     fwd_steps += steps;
   }
-  // recompute the step whose start state is in (c.q, c.S), storing its iterates, and reverse it
-  void base_step() {
+  // reverse step m: from its kept iterates, or recomputed (with storage) from
+  // its start state in (c.q, c.S)
+  void base_step(int m) {
+    if (kept(m)) {
+      step_rev(c, *W, W->sets[m - kfirst].view(), dt, gamma, alpha, n_iter, solver, !first_base);
+      first_base = false;
+      std::swap(W->QBn, W->QBnext);
+      return;
+    }
     ++fwd_steps;
     if (c.dim == 1) {
       K1AdjArgs& b = W1->base;
@@ -471,29 +555,33 @@ struct GradRun {
     }
     launch_copy(c, W->Q[0], c.q, 4 * c.N);
     launch_copy(c, W->Sg[0], c.S, c.N);
-    step_store(c, *W, dt, gamma, alpha, n_iter, solver, c.q);
+    step_store(c, W->view(), dt, gamma, alpha, n_iter, solver, c.q);
     if (first_base) {
       // q(T) is now in c.q: the objective and its gradient seed the sweep
       objective_seed();
       first_base = false;
-      step_rev(c, *W, dt, gamma, alpha, n_iter, solver, false);
+      step_rev(c, *W, W->view(), dt, gamma, alpha, n_iter, solver, false);
     } else {
-      step_rev(c, *W, dt, gamma, alpha, n_iter, solver, true);
+      step_rev(c, *W, W->view(), dt, gamma, alpha, n_iter, solver, true);
     }
     std::swap(W->QBn, W->QBnext);
   }
   // reverse steps [a, b): the state at a is in `slot`; the slots after it are free
   void reverse_range(int a, int b, int k, int free_slots) {
     const int n = b - a;
-    if (n == 1) { restore(k); base_step(); return; }
+    if (n == 1) {
+      if (!kept(a)) restore(k);
+      base_step(a);
+      return;
+    }
     const int fs = std::min(free_slots, n - 1);
     if (fs == 0) {
-      for (int m = b - 1; m >= a; --m) { restore(k); advance(m - a); base_step(); }
+      for (int m = b - 1; m >= a; --m) { restore(k); advance(a, m - a); base_step(m); }
       return;
     }
     const int j = sch.split_of(n, fs);
     restore(k);
-    advance(j);
+    advance(a, j);
     save(k + 1);
     reverse_range(a + j, b, k + 1, free_slots - 1);
     reverse_range(a, a + j, k, free_slots);
@@ -579,9 +667,20 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
   // state at t = 0: q0 from the context, Sigma = 0 (enzyme_adjoint.jl:51)
   launch_fill(*c, c->S, N, 0.0);
   run.obj = objective;
+  run.n_total = n_steps;
   const double* pp = prm ? prm : KE_WINDOW_DEFAULT;
   for (int k = 0; k < 4; ++k) run.prm[k] = pp[k];
   run.sch.build(n_steps, slots - 1);
+  if (c->dim == 2 && run.sch.store_all && n_steps >= 2) {
+    // every step is checkpointed: keep the iterates of as many of the last
+    // steps as 85% of the free HBM holds, so their reverse needs no recompute
+    size_t fr = 0, tot = 0;
+    HIPCHK(*c, hipMemGetInfo(&fr, &tot));
+    const size_t per = (step_set_doubles(N, n_iter) + 64) * sizeof(double);
+    const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes));
+    const int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
+    if (K >= 1 && alloc_sets(*c, *run.W, K) == IGR_OK) run.kfirst = n_steps - 1 - K;
+  }
   run.save(0);
   run.reverse_range(0, n_steps, 0, slots - 1);
   // the last reversed step left dJ/dq0 in QBnext
@@ -589,7 +688,7 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
     qb = run.W1->base.QBnext;
   } else {
     const int nb = grid2_blocks(*c);
-    launch_sum_blocks(*c, run.W->gblk, nb, dsc + 3);
+    launch_sum_blocks(*c, run.W->gblk, grid2_hblocks(*c), dsc + 3);
     launch_sum_blocks(*c, run.W->ablk, nb, dsc + 4);
     qb = run.W->QBnext;
   }
@@ -624,7 +723,7 @@ int igr_hyperbolic_vjp(igr_ctx* c, double gamma, const double* q, const double* 
   h.qbar_out = W->QBnext; h.qbar_stage = W->QBn; h.accumulate = 0; h.qbar_n = nullptr;
   h.sbar_in = nullptr; h.sbar_out = W->Xbar; h.c_dt = 1.0; h.c_prev = 0.0; h.c_n = 0.0; h.gbar_blk = W->gblk;
   launch2_hyp_rev(*c, h);
-  launch_sum_blocks(*c, W->gblk, grid2_blocks(*c), W->dsc + 3);
+  launch_sum_blocks(*c, W->gblk, grid2_hblocks(*c), W->dsc + 3);
   if (qbar) HIPCHK(*c, hipMemcpyAsync(qbar, W->QBn, 4 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (sigmabar) HIPCHK(*c, hipMemcpyAsync(sigmabar, W->Xbar, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   double gb = 0.0;

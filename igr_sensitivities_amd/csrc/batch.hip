@@ -31,8 +31,8 @@ void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int 
 
 namespace {
 
-int tile_x_of(const Ctx& c) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 2 : c.tile_x; }
-int tile_y_of(const Ctx& c) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 4 : c.tile_y; }
+int tile_x_of(const Ctx& c, bool h) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 2 : (h ? c.htile_x : c.tile_x); }
+int tile_y_of(const Ctx& c, bool h) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 4 : (h ? c.htile_y : c.tile_y); }
 
 struct DevBufs {
   std::vector<void*> p;
@@ -59,7 +59,7 @@ struct Program {
     int cnt;
     size_t maxn;
   };
-  int np = 4, tx = 4, ty = 4;
+  int np = 4, tx = 4, ty = 4, htx = 4, hty = 4;   // SIP-family and hyperbolic-family tiles
   std::vector<L> ls;
   SipArgs* sip = nullptr;
   HypArgs* hyp = nullptr;
@@ -67,16 +67,18 @@ struct Program {
   FinArgs* fin = nullptr;
   FillArgs* fill = nullptr;
   CopyArgs* copy = nullptr;
-  int2* work = nullptr;
+  int2* work = nullptr;      // (instance, workgroup) items of a SIP-family launch
   int nblocks = 0;
+  int2* hwork = nullptr;     // and of a hyperbolic-family launch
+  int hnblocks = 0;
 
   void run(hipStream_t st) const {
     for (const L& l : ls) {
       switch (l.kind) {
         case R_SIP: launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, work, nblocks, st); break;
-        case R_HYP: launch2_hyp_batch(np, tx, ty, hyp + l.off, work, nblocks, st); break;
-        case R_SENS_RHS: launch2_sens_rhs_batch(np, tx, ty, sens + l.off, work, nblocks, st); break;
-        case R_SENS_HYP: launch2_sens_hyp_batch(np, tx, ty, sens + l.off, work, nblocks, st); break;
+        case R_HYP: launch2_hyp_batch(np, htx, hty, hyp + l.off, hwork, hnblocks, st); break;
+        case R_SENS_RHS: launch2_sens_rhs_batch(np, htx, hty, sens + l.off, hwork, hnblocks, st); break;
+        case R_SENS_HYP: launch2_sens_hyp_batch(np, htx, hty, sens + l.off, hwork, hnblocks, st); break;
         case R_FINAL: launch_pcg_final_batch(fin + l.off, l.cnt, st); break;
         case R_FILL: launch_fill_batch(fill + l.off, l.cnt, l.maxn, st); break;
         case R_COPY: launch_copy_batch(copy + l.off, l.cnt, l.maxn, st); break;
@@ -86,7 +88,7 @@ struct Program {
 };
 
 std::vector<int> signature(const Recorder& r, const Ctx& c) {
-  std::vector<int> s = {c.np, tile_x_of(c), tile_y_of(c)};
+  std::vector<int> s = {c.np, tile_x_of(c, false), tile_y_of(c, false), tile_x_of(c, true), tile_y_of(c, true)};
   s.reserve(3 + 2 * r.recs.size());
   for (const Rec& x : r.recs) { s.push_back(x.kind); s.push_back(x.mode); }
   return s;
@@ -101,7 +103,8 @@ int build_programs(const std::vector<Ctx*>& ctxs, const std::vector<int>& ids, c
     const std::vector<int>& mem = kv.second;
     const Ctx& c0 = *ctxs[ids[mem[0]]];
     Program P;
-    P.np = c0.np; P.tx = tile_x_of(c0); P.ty = tile_y_of(c0);
+    P.np = c0.np; P.tx = tile_x_of(c0, false); P.ty = tile_y_of(c0, false);
+    P.htx = tile_x_of(c0, true); P.hty = tile_y_of(c0, true);
     std::vector<SipArgs> hs;
     std::vector<HypArgs> hh;
     std::vector<SensArgs> hn;
@@ -135,15 +138,19 @@ int build_programs(const std::vector<Ctx*>& ctxs, const std::vector<int>& ids, c
       }
       P.ls.push_back(l);
     }
-    std::vector<int2> work;
+    std::vector<int2> work, hwork;
     for (size_t g = 0; g < mem.size(); ++g) {
       const Ctx& c = *ctxs[ids[mem[g]]];
       const int nb = ((c.Nex + P.tx - 1) / P.tx) * ((c.Ney + P.ty - 1) / P.ty);
       for (int b = 0; b < nb; ++b) work.push_back(make_int2((int)g, b));
+      const int hb = ((c.Nex + P.htx - 1) / P.htx) * ((c.Ney + P.hty - 1) / P.hty);
+      for (int b = 0; b < hb; ++b) hwork.push_back(make_int2((int)g, b));
     }
     P.nblocks = (int)work.size();
+    P.hnblocks = (int)hwork.size();
     P.work = dev.put(work);
-    if (!P.work) return IGR_ENOMEM;
+    P.hwork = dev.put(hwork);
+    if (!P.work || !P.hwork) return IGR_ENOMEM;
     if ((!hs.empty() && !(P.sip = dev.put(hs))) || (!hh.empty() && !(P.hyp = dev.put(hh))) ||
         (!hn.empty() && !(P.sens = dev.put(hn))) || (!hf.empty() && !(P.fin = dev.put(hf))) ||
         (!hl.empty() && !(P.fill = dev.put(hl))) || (!hc.empty() && !(P.copy = dev.put(hc))))

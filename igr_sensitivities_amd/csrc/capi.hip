@@ -209,6 +209,82 @@ int k1_block(const Ctx& c) {  // threads of the 1D whole-instance workgroup
   const size_t N = c.N;
   return N >= 1024 ? 1024 : (int)(((N + 63) / 64) * 64);
 }
+// ---- tile shapes (p = 3) -------------------------------------------------
+bool tile_ok(int tx, int ty, bool hyp) {
+  return (tx == 4 && ty == 4) || (tx == 8 && ty == 4) || (tx == 2 && ty == 2) || (!hyp && tx == 8 && ty == 8);
+}
+void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
+  if (tx == c.tile_x && ty == c.tile_y && htx == c.htile_x && hty == c.htile_y) return;
+  (void)hipStreamSynchronize(c.stream);
+  destroy_graphs(c);   // captured steps hold launches of the old shapes
+  c.tile_x = tx; c.tile_y = ty; c.htile_x = htx; c.htile_y = hty;
+}
+
+// Default shapes by mesh size (elements E = Nex * Ney), from timing every
+// shape on MI355X at 16^2 .. 256^2: up to ~64^2 elements the launches are
+// latency-bound and 4x4 tiles are fastest; from ~128^2 the 8x4 tiles win
+// (fewer partial sums for every workgroup to read, less halo per node); from
+// ~256^2 the SIP family prefers 8x8.  The shape of a given mesh is fixed, so
+// repeated runs give bit-identical results; the autotuner may choose others.
+void default_tiles(Ctx& c) {
+  if (c.np != 4) return;
+  const long long E = (long long)c.Nex * c.Ney;
+  if (E < 96 * 96) { c.tile_x = 4; c.tile_y = 4; c.htile_x = 4; c.htile_y = 4; }
+  else if (E < 192 * 192) { c.tile_x = 8; c.tile_y = 4; c.htile_x = 8; c.htile_y = 4; }
+  else { c.tile_x = 8; c.tile_y = 8; c.htile_x = 8; c.htile_y = 4; }
+}
+
+// The autotuner: on the current state, time a 30-iteration PCG solve for
+// every SIP-family shape and 20 hyperbolic stage launches for every
+// hyperbolic-family shape (HIP events, best of 3), and keep the fastest.
+// Results change only in the order of the solver's partial sums.
+int autotune(Ctx& c) {
+  const int sip[3][2] = {{4, 4}, {8, 4}, {8, 8}}, hyp[2][2] = {{4, 4}, {8, 4}};
+  const double h = c.Lx / c.Nex;
+  const double alpha = 10.0 * (h / 4.0) * (h / 4.0), gamma = 1.4;
+  const int n_it = 30;
+  int rc = ensure_scal(c, n_it);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  auto timed = [&](auto&& body) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      (void)hipEventRecord(e0, c.stream);
+      body();
+      (void)hipEventRecord(e1, c.stream);
+      (void)hipEventSynchronize(e1);
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      if (rep > 0) best = std::min(best, t);    // the first run loads code objects
+    }
+    return best;
+  };
+  int bt = 0, bh = 0;
+  float tb = 1e30f, th = 1e30f;
+  const int htx0 = c.htile_x, hty0 = c.htile_y;
+  for (int k = 0; k < 3; ++k) {
+    set_tiles(c, sip[k][0], sip[k][1], htx0, hty0);
+    const float t = timed([&]() {
+      launch_copy(c, c.xA, c.S, c.N);
+      solve2(c, c.q, alpha, n_it, IGR_SOLVER_PCG, c.scal, c.xA);
+    });
+    if (t < tb) { tb = t; bt = k; }
+  }
+  for (int k = 0; k < 2; ++k) {
+    set_tiles(c, sip[bt][0], sip[bt][1], hyp[k][0], hyp[k][1]);
+    HypArgs a = hyp_base(c, gamma);
+    a.q = c.q; a.mode = 2; a.cdt = 1e-6; a.out = c.q2; a.qn = c.q;
+    const float t = timed([&]() { for (int r = 0; r < 20; ++r) launch2_hyp(c, a); });
+    if (t < th) { th = t; bh = k; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_tiles(c, sip[bt][0], sip[bt][1], hyp[bh][0], hyp[bh][1]);
+  return IGR_OK;
+}
+
 int run1(Ctx& c, const K1Args& a) {
   launch1_run(c, a, k1_block(c));
   HIPCHK(c, hipGetLastError());
@@ -248,6 +324,7 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   if (dim == 1) { A(c->k1F, 3 * N); A(c->k1face, 6 * (size_t)nex); }
   // two regions of 4 partial sums per workgroup, sized for the smallest tile
   // (2x2 elements) so any tile choice fits
+  default_tiles(*c);
   c->pstride = 4 * (size_t)(((nex + 1) / 2) * ((c->Ney + 1) / 2)) + 64;
   c->partials_n = 2 * c->pstride;
   A(c->partials, c->partials_n);
@@ -287,18 +364,31 @@ int igr_sizes(const igr_ctx* c, long long* n_nodes, int* n_vars) {
 int igr_set_option(igr_ctx* c, int option, int value) {
   if (!c) return IGR_EINVAL;
   if (option == IGR_OPT_GRAPHS) { c->use_graphs = value ? 1 : 0; return IGR_OK; }
-  if (option == IGR_OPT_TILE) {
+  if (option == IGR_OPT_TILE || option == IGR_OPT_HTILE) {
     const int tx = value / 100, ty = value % 100;
-    const bool ok = (tx == 4 && ty == 4) || (tx == 8 && ty == 4) || (tx == 2 && ty == 2);
-    if (!ok) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
-    if (c->np == 4 && (tx != c->tile_x || ty != c->tile_y)) {
-      (void)hipStreamSynchronize(c->stream);
-      destroy_graphs(*c);
-      c->tile_x = tx; c->tile_y = ty;
-    }
+    const bool h = option == IGR_OPT_HTILE;
+    if (!tile_ok(tx, ty, h)) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
+    if (c->np == 4) set_tiles(*c, h ? c->tile_x : tx, h ? c->tile_y : ty, h ? tx : c->htile_x, h ? ty : c->htile_y);
     return IGR_OK;
   }
+  if (option == IGR_OPT_AUTOTUNE) {
+    if (c->dim != 2 || c->np != 4) return IGR_OK;   // one tile shape exists for the other cases
+    int rc = begin_call(*c);
+    if (rc) return rc;
+    rc = autotune(*c);
+    if (rc) return rc;
+    return end_call(*c);
+  }
   return seterr(*c, IGR_EINVAL, "unknown option");
+}
+
+int igr_get_option(const igr_ctx* c, int option, int* value) {
+  if (!c || !value) return IGR_EINVAL;
+  if (option == IGR_OPT_GRAPHS) *value = c->use_graphs;
+  else if (option == IGR_OPT_TILE) *value = 100 * c->tile_x + c->tile_y;
+  else if (option == IGR_OPT_HTILE) *value = 100 * c->htile_x + c->htile_y;
+  else return IGR_EINVAL;
+  return IGR_OK;
 }
 
 int igr_set_state(igr_ctx* c, const double* q, const double* sigma) {

@@ -61,8 +61,12 @@ struct Ctx {
   double* k1F = nullptr;      // 3N fluxes
   double* k1face = nullptr;   // 6*Ne face values
 
-  // tile shape chosen by the autotuner (2D): elements per workgroup
+  // 2D elements per workgroup (p = 3): tile_* for the SIP/solver kernels and
+  // their reverse (the kernels with grid reductions), htile_* for the
+  // hyperbolic, tangent and objective kernels; chosen by a size table or by
+  // the autotuner (igr_set_option IGR_OPT_AUTOTUNE)
   int tile_x = 4, tile_y = 4;
+  int htile_x = 4, htile_y = 4;
   int use_graphs = 1;
 
   // captured SSP-RK3 steps, keyed by their scalar parameters

@@ -199,6 +199,25 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   // reverse scalars of this iteration, from the dot products the previous
   // kernel left as partial sums (every workgroup forms the same totals)
   double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
+  // PCG_ITER: load the tile + halo fields first so they are in flight with
+  // the partial sums (the staging loop below then only combines them)
+  constexpr int NL = (NS * NP2 + NT - 1) / NT;
+  double l_ir[NL], l_iM[NL], l_rc[NL], l_pb[NL], l_R[NL], l_p[NL];
+  if constexpr (MODE == RV_PCG_ITER) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int idx = tid + l * NT;
+      l_ir[l] = 0.0; l_iM[l] = 0.0; l_rc[l] = 0.0; l_pb[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0;
+      size_t gi;
+      if (idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi)) {
+        l_ir[l] = a.invR[gi];
+        l_iM[l] = a.invM[gi];
+        if (!a.first) { l_rc[l] = a.rbar_in[gi]; l_pb[l] = a.pbar_in[gi]; }
+        l_R[l] = a.R[(size_t)(k + 1) * N + gi];
+        l_p[l] = a.P[(size_t)k * N + gi];
+      }
+    }
+  }
   if constexpr (MODE == RV_PCG_ITER) {
     double d[4];
     grid_total<4>(a.part_in, gridDim.x, d, red);
@@ -225,7 +244,22 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
 
   // own-node values kept for the outputs
   double o_rb = 0.0, o_zb = 0.0, o_R = 0.0, o_pb = 0.0;
-  for (int idx = tid; idx < NS * NP2; idx += NT) {
+  if constexpr (MODE == RV_PCG_ITER) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int idx = tid + l * NT;
+      if (idx < NS * NP2) {
+        const double z = l_R[l] * l_iM[l];
+        const double zb = l_pb[l] + rzb * l_R[l];
+        const double rb = l_rc[l] + rzb * z + zb * l_iM[l];
+        ws[idx] = -ak * rb + papb * l_p[l];
+        vs[idx] = l_p[l];
+        irs[idx] = l_ir[l];
+        if (l == 0) { o_rb = rb; o_zb = zb; o_R = l_R[l]; o_pb = l_pb[l]; }
+      }
+    }
+  }
+  for (int idx = tid; MODE != RV_PCG_ITER && idx < NS * NP2; idx += NT) {
     const int sl = idx / NP2, nn = idx % NP2;
     double w = 0.0, v = 0.0, ir = 0.0, mb = 0.0;
     size_t gi;
@@ -646,21 +680,39 @@ struct LaunchRev {
   }
 };
 
-#define IGR_DISPATCH_TILE(c, CALL)                                              \
-  switch ((c).np) {                                                             \
+// SIP / solver reverse: the SIP tile (4x4, 8x4, 8x8, 2x2 at p = 3); the
+// hyperbolic reverse and the objectives: the hyperbolic tile (no 8x8).
+#define IGR_REV_DISPATCH(np, tx, ty, CALL)                                      \
+  switch (np) {                                                                 \
     case 2: LaunchRev<2, 8, 8>::CALL; break;                                    \
     case 3: LaunchRev<3, 4, 4>::CALL; break;                                    \
     case 4:                                                                     \
-      if ((c).tile_x == 8 && (c).tile_y == 4) LaunchRev<4, 8, 4>::CALL;        \
-      else if ((c).tile_x == 2 && (c).tile_y == 2) LaunchRev<4, 2, 2>::CALL;   \
+      if ((tx) == 8 && (ty) == 4) LaunchRev<4, 8, 4>::CALL;                     \
+      else if ((tx) == 8 && (ty) == 8) LaunchRev<4, 8, 8>::CALL;                \
+      else if ((tx) == 2 && (ty) == 2) LaunchRev<4, 2, 2>::CALL;                \
+      else LaunchRev<4, 4, 4>::CALL;                                            \
+      break;                                                                    \
+    case 5: LaunchRev<5, 2, 4>::CALL; break;                                    \
+  }
+#define IGR_HREV_DISPATCH(np, tx, ty, CALL)                                     \
+  switch (np) {                                                                 \
+    case 2: LaunchRev<2, 8, 8>::CALL; break;                                    \
+    case 3: LaunchRev<3, 4, 4>::CALL; break;                                    \
+    case 4:                                                                     \
+      if ((tx) == 8 && (ty) == 4) LaunchRev<4, 8, 4>::CALL;                     \
+      else if ((tx) == 2 && (ty) == 2) LaunchRev<4, 2, 2>::CALL;                \
       else LaunchRev<4, 4, 4>::CALL;                                            \
       break;                                                                    \
     case 5: LaunchRev<5, 2, 4>::CALL; break;                                    \
   }
 
-void launch2_hyp_rev(const Ctx& c, const HypRevArgs& a) { IGR_DISPATCH_TILE(c, hyp(a, c.stream)); }
-void launch2_rev(const Ctx& c, int mode, const RevArgs& a) { IGR_DISPATCH_TILE(c, rev(mode, a, c.stream)); }
-void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad) { IGR_DISPATCH_TILE(c, obj(a, grad, c.stream)); }
+void launch2_hyp_rev(const Ctx& c, const HypRevArgs& a) { IGR_HREV_DISPATCH(c.np, c.htile_x, c.htile_y, hyp(a, c.stream)); }
+void launch2_rev(const Ctx& c, int mode, const RevArgs& a) {
+  IGR_REV_DISPATCH(c.np, c.tile_x, c.tile_y, rev(mode, a, c.stream));
+}
+void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad) {
+  IGR_HREV_DISPATCH(c.np, c.htile_x, c.htile_y, obj(a, grad, c.stream));
+}
 void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out) {
   hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(1024), 0, c.stream, blk, nb, out);
 }

@@ -632,25 +632,39 @@ struct Launch2 {
   }
 };
 
-// (np, tile) -> the instantiated tile shape; p = 3 (np = 4) has three tile shapes
-#define IGR_TILE_DISPATCH(np, tx, ty, CALL)                             \
+// (np, tile) -> the instantiated tile shape.  p = 3 (np = 4): the SIP /
+// solver family has 4x4, 8x4, 8x8 and 2x2 tiles, the hyperbolic family
+// 4x4, 8x4 and 2x2 (an 8x8 hyperbolic tile would not fit the LDS).
+#define IGR_SIP_TILE_DISPATCH(L, np, tx, ty, CALL)                      \
   switch (np) {                                                         \
-    case 2: Launch2<2, 8, 8>::CALL; break;                              \
-    case 3: Launch2<3, 4, 4>::CALL; break;                              \
+    case 2: L<2, 8, 8>::CALL; break;                                    \
+    case 3: L<3, 4, 4>::CALL; break;                                    \
     case 4:                                                             \
-      if ((tx) == 8 && (ty) == 4) Launch2<4, 8, 4>::CALL;               \
-      else if ((tx) == 2 && (ty) == 2) Launch2<4, 2, 2>::CALL;          \
-      else Launch2<4, 4, 4>::CALL;                                      \
+      if ((tx) == 8 && (ty) == 4) L<4, 8, 4>::CALL;                     \
+      else if ((tx) == 8 && (ty) == 8) L<4, 8, 8>::CALL;                \
+      else if ((tx) == 2 && (ty) == 2) L<4, 2, 2>::CALL;                \
+      else L<4, 4, 4>::CALL;                                            \
       break;                                                            \
-    case 5: Launch2<5, 2, 4>::CALL; break;                              \
+    case 5: L<5, 2, 4>::CALL; break;                                    \
+  }
+#define IGR_HYP_TILE_DISPATCH(L, np, tx, ty, CALL)                      \
+  switch (np) {                                                         \
+    case 2: L<2, 8, 8>::CALL; break;                                    \
+    case 3: L<3, 4, 4>::CALL; break;                                    \
+    case 4:                                                             \
+      if ((tx) == 8 && (ty) == 4) L<4, 8, 4>::CALL;                     \
+      else if ((tx) == 2 && (ty) == 2) L<4, 2, 2>::CALL;                \
+      else L<4, 4, 4>::CALL;                                            \
+      break;                                                            \
+    case 5: L<5, 2, 4>::CALL; break;                                    \
   }
 
 void launch2_sip_batch(int np, int tx, int ty, int mode, const SipArgs* tab, const int2* work, int nblocks,
                        hipStream_t st) {
-  IGR_TILE_DISPATCH(np, tx, ty, sip_batch(mode, tab, work, nblocks, st));
+  IGR_SIP_TILE_DISPATCH(Launch2, np, tx, ty, sip_batch(mode, tab, work, nblocks, st));
 }
 void launch2_hyp_batch(int np, int tx, int ty, const HypArgs* tab, const int2* work, int nblocks, hipStream_t st) {
-  IGR_TILE_DISPATCH(np, tx, ty, hyp_batch(tab, work, nblocks, st));
+  IGR_HYP_TILE_DISPATCH(Launch2, np, tx, ty, hyp_batch(tab, work, nblocks, st));
 }
 void launch_pcg_final_batch(const FinArgs* tab, int n, hipStream_t st) {
   hipLaunchKernelGGL(k2_pcg_final_batch, dim3(n), dim3(256), 0, st, tab);
@@ -666,30 +680,21 @@ void launch_copy_batch(const CopyArgs* tab, int n, size_t maxn, hipStream_t st) 
 
 void launch2_sip(const Ctx& c, int mode, const SipArgs& a) {
   if (g_rec) { g_rec->recs.push_back({R_SIP, mode, g_rec->sip.size()}); g_rec->sip.push_back(a); return; }
-  switch (c.np) {
-    case 2: Launch2<2, 8, 8>::sip(mode, a, c.stream); break;
-    case 3: Launch2<3, 4, 4>::sip(mode, a, c.stream); break;
-    case 4:
-      if (c.tile_x == 8 && c.tile_y == 4) Launch2<4, 8, 4>::sip(mode, a, c.stream);
-      else if (c.tile_x == 2 && c.tile_y == 2) Launch2<4, 2, 2>::sip(mode, a, c.stream);
-      else Launch2<4, 4, 4>::sip(mode, a, c.stream);
-      break;
-    case 5: Launch2<5, 2, 4>::sip(mode, a, c.stream); break;
-  }
+  IGR_SIP_TILE_DISPATCH(Launch2, c.np, c.tile_x, c.tile_y, sip(mode, a, c.stream));
 }
 void launch2_hyp(const Ctx& c, const HypArgs& a) {
   if (g_rec) { g_rec->recs.push_back({R_HYP, 0, g_rec->hyp.size()}); g_rec->hyp.push_back(a); return; }
-  switch (c.np) {
-    case 2: Launch2<2, 8, 8>::hyp(a, c.stream); break;
-    case 3: Launch2<3, 4, 4>::hyp(a, c.stream); break;
-    case 4:
-      if (c.tile_x == 8 && c.tile_y == 4) Launch2<4, 8, 4>::hyp(a, c.stream);
-      else if (c.tile_x == 2 && c.tile_y == 2) Launch2<4, 2, 2>::hyp(a, c.stream);
-      else Launch2<4, 4, 4>::hyp(a, c.stream);
-      break;
-    case 5: Launch2<5, 2, 4>::hyp(a, c.stream); break;
-  }
+  IGR_HYP_TILE_DISPATCH(Launch2, c.np, c.htile_x, c.htile_y, hyp(a, c.stream));
 }
+// workgroups of one hyperbolic-family launch
+int grid2_hblocks(const Ctx& c) {
+  int tx = 4, ty = 4;
+  if (c.np == 2) { tx = 8; ty = 8; }
+  else if (c.np == 5) { tx = 2; ty = 4; }
+  else if (c.np == 4) { tx = c.htile_x; ty = c.htile_y; }
+  return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
+}
+// workgroups of one SIP-family launch (the number of partial sums)
 int grid2_blocks(const Ctx& c) {
   int tx = 4, ty = 4;
   if (c.np == 2) { tx = 8; ty = 8; }

@@ -289,11 +289,11 @@ struct LaunchSens {
   }
 void launch2_sens_rhs(const Ctx& c, const SensArgs& a) {
   if (g_rec) { g_rec->recs.push_back({R_SENS_RHS, 0, g_rec->sens.size()}); g_rec->sens.push_back(a); return; }
-  IGR_SENS_DISPATCH(c.np, c.tile_x, c.tile_y, rhs(a, c.stream));
+  IGR_SENS_DISPATCH(c.np, c.htile_x, c.htile_y, rhs(a, c.stream));
 }
 void launch2_sens_hyp(const Ctx& c, const SensArgs& a) {
   if (g_rec) { g_rec->recs.push_back({R_SENS_HYP, 0, g_rec->sens.size()}); g_rec->sens.push_back(a); return; }
-  IGR_SENS_DISPATCH(c.np, c.tile_x, c.tile_y, hyp(a, c.stream));
+  IGR_SENS_DISPATCH(c.np, c.htile_x, c.htile_y, hyp(a, c.stream));
 }
 void launch2_sens_rhs_batch(int np, int tx, int ty, const SensArgs* tab, const int2* work, int nblocks,
                             hipStream_t st) {

@@ -135,6 +135,25 @@ def test_gradient_alpha_zero_and_deterministic(orc):
     assert ab == 0.0
 
 
+@pytest.mark.parametrize("tile,htile", [(808, 804), (404, 404), (202, 202)])
+def test_gradient_dot_product_tiles(orc, tile, htile):
+    nx, ny, ns = 12, 10, 2
+    c = ctx2(3, nx, ny)
+    c.set_option(igr.core.OPT_TILE, tile)
+    c.set_option(igr.core.OPT_HTILE, htile)
+    g = orc.Grid2D(3, nx, ny)
+    q0 = configs.smooth_random2(3, nx, ny, seed=120, amp=0.3)
+    gam, a = 1.4, 4e-3
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, gam))
+    p4 = (0.5, 0.5, 0.2, 0.0)
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(3, ns, dt, gam, a, 15, "pcg", prm=p4)
+    u = np.random.default_rng(121).standard_normal(q0.shape)
+    Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.4, a, 0.3 * a, 15, orc.PCG, ns, dt, 3, q0, u, p4)
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.4 * gb + 0.3 * a * ab, 1e-10)
+
+
 @pytest.mark.parametrize("solver", ["pcg", "chebyshev"])
 def test_checkpoint_schedules_bitwise_equal(solver):
     nx, ny, ns = 6, 6, 9

@@ -154,20 +154,39 @@ def test_graph_replay_matches_eager_and_is_deterministic():
         np.testing.assert_array_equal(outs[k], outs[6 + k])
 
 
-def test_tile_shapes_agree(orc):
-    q0 = configs.taylor_green2(3, 24, 24, A=0.3)
-    dt = 0.4 * (1 / 24) / (7 * configs.max_wavespeed2(q0, 1.4))
-    res = []
-    for tile in (404, 804, 202):
-        c = ctx2(3, 24, 24)
-        c.set_option(1, tile)
+@pytest.mark.parametrize("nx,ny", [(24, 24), (20, 12)])
+def test_tile_shapes_agree(orc, nx, ny):
+    q0 = configs.taylor_green2(3, nx, ny, A=0.3)
+    dt = 0.4 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
+    g = orc.Grid2D(3, nx, ny)
+    ref, _ = g.run_steps(1.4, 2e-3, 60, orc.PCG, 2, dt, q0)
+    for tile, htile in ((404, 404), (804, 804), (808, 804), (808, 404), (202, 202), (404, 804)):
+        c = ctx2(3, nx, ny)
+        c.set_option(igr.core.OPT_TILE, tile)
+        c.set_option(igr.core.OPT_HTILE, htile)
+        assert (c.get_option(igr.core.OPT_TILE), c.get_option(igr.core.OPT_HTILE)) == (tile, htile)
         c.set_state(q0)
         c.run_steps(2, dt, 1.4, 2e-3, 60, "pcg")
-        res.append(c.get_state()[0])
-    g = orc.Grid2D(3, 24, 24)
-    ref, _ = g.run_steps(1.4, 2e-3, 60, orc.PCG, 2, dt, q0)
-    for r in res:
-        assert rel(r, ref) < 1e-11
+        assert rel(c.get_state()[0], ref) < 1e-11, (tile, htile)
+    c = ctx2(3, nx, ny)
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_HTILE, 808)     # no 8x8 hyperbolic tile
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_TILE, 303)
+
+
+def test_autotune_keeps_results(orc):
+    q0 = configs.taylor_green2(3, 32, 32, A=0.3)
+    dt = 0.4 * (1 / 32) / (7 * configs.max_wavespeed2(q0, 1.4))
+    c = ctx2(3, 32, 32)
+    c.set_state(q0)
+    tile, htile = c.autotune()
+    print(f"[autotune] 32x32 -> SIP tile {tile}, hyperbolic tile {htile}", flush=True)
+    assert tile in (404, 804, 808) and htile in (404, 804)
+    c.set_state(q0)
+    c.run_steps(2, dt, 1.4, 2e-3, 60, "pcg")
+    ref, _ = orc.Grid2D(3, 32, 32).run_steps(1.4, 2e-3, 60, orc.PCG, 2, dt, q0)
+    assert rel(c.get_state()[0], ref) < 1e-11
 
 
 def test_c2_full_size_one_stage(orc):
