@@ -150,6 +150,7 @@ def main():
     n_nodes = 16 * N1 * N1
     gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
 
+    tiles = (ctx.get_option(igr.core.OPT_TILE), ctx.get_option(igr.core.OPT_HTILE))
     ctx.set_state(cfg["q"])
     ctx.run_steps(max(args.warmup, 1), dt, gam, al, n_iter, "pcg")   # also captures the step graph
     ctx.set_state(cfg["q"])
@@ -226,10 +227,11 @@ def main():
             "workload": "2D IGR Euler forward, 128x128 elements, p=3, PCG-60 Sigma solve per stage, "
                         "SSP-RK3, triple energy deposit",
             "nodes": n_nodes, "n_iter": n_iter, "solver": "pcg", "dt": dt, "alpha": al, "gamma": gam,
+            "tiles": {"sip": tiles[0], "hyperbolic": tiles[1]},
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k2_sip<PCG_A> (SIP apply fused with p = z + beta p)",
+            "kernel": f"k2_sip<4,{tiles[0] // 100},{tiles[0] % 100},PCG_A> (SIP apply fused with p = z + beta p)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
