@@ -10,7 +10,9 @@ const libigr = joinpath(@__DIR__, "..", "lib", "libigr_hip.so")
 const IGR_OK, IGR_EINVAL, IGR_EDOMAIN = 0, 1, 2
 const SOLVERS = Dict(:pcg => 0, :jacobi => 1, :chebyshev => 2)
 const OBJECTIVES = Dict(:l2_density => 0, :l2_pressure => 1, :kinetic_energy => 2,
-                        :windowed_kinetic_energy => 3, :weighted_momentum => 4)
+                        :windowed_kinetic_energy => 3, :weighted_momentum => 4,
+                        :windowed_density => 5, :disc_kinetic_energy => 6)
+const OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE = 0, 1, 2, 3
 
 mutable struct Context
     h::Ptr{Cvoid}
@@ -92,6 +94,82 @@ function enzyme_adjoint_ic(c::Context, μx0, μy0, ρ0, E0, n_steps, Δt, γ, α
     unpack!(qb, d...)
     return J[], d..., gb[], ab[]
 end
+
+# 1D enzyme_adjoint_ic (1D/enzyme_adjoint.jl:255-314): the gradient of the
+# fixed-step objective, mapped to primitive adjoints as the reference does
+# (:306-311).  Returns (J, adj_ρ, adj_u, adj_p), plus dJ/dγ, dJ/dα.
+function enzyme_adjoint_ic_1d(c::Context, μ0, ρ0, E0, n_steps, Δt, γ, α;
+                              n_iter::Int=10, solver::Symbol=:pcg, objective::Symbol=:l2_density,
+                              ckpt_slots::Int=0)
+    haskey(OBJECTIVES, objective) || error("Unknown objective: $objective (use :l2_density or :l2_pressure)")
+    q = pack(μ0, ρ0, E0)
+    set_state!(c, q)
+    J = Ref{Cdouble}(0); gb = Ref{Cdouble}(0); ab = Ref{Cdouble}(0)
+    qb = similar(q)
+    check(c, ccall((:igr_gradient, libigr), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint, Cdouble, Cdouble, Cdouble, Cint, Cint, Cint,
+                    Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+                   c.h, OBJECTIVES[objective], C_NULL, n_steps, Δt, γ, α, max(n_iter, 1),
+                   solver_code(solver), ckpt_slots, J, qb, gb, ab))
+    dμ, dρ, dE = similar(μ0), similar(ρ0), similar(E0)
+    unpack!(qb, dμ, dρ, dE)
+    u0 = μ0 ./ ρ0
+    adj_ρ = dρ .+ u0 .* dμ .+ (u0 .^ 2 ./ 2) .* dE
+    adj_u = ρ0 .* dμ .+ ρ0 .* u0 .* dE
+    adj_p = dE ./ (γ - 1)
+    return J[], adj_ρ, adj_u, adj_p, gb[], ab[]
+end
+
+# run_sensitivity(q0..., s0..., Δt, T, basis, mesh, γ, α; n_iter, solver[, elliptic_rhs, sγ])
+# (2D/sensitivity.jl:353; 1D/sensitivity.jl:475): q0 and s0 are the packed
+# conserved fields; returns (q, Σ, s, ς) at T.
+function run_sensitivity(c::Context, q0::Vector{Float64}, s0::Vector{Float64}, Δt, T, γ, α;
+                         n_iter::Int=10, solver::Symbol=:pcg, elliptic_rhs::Symbol=:form2, sγ=0.0)
+    form = elliptic_rhs == :form1 ? 1 : elliptic_rhs == :form2 ? 2 :
+           error("Unknown elliptic_rhs: $elliptic_rhs (use :form1 or :form2)")
+    set_state!(c, q0)
+    s = similar(s0); ς = zeros(c.N); ns = Ref{Cint}(0)
+    check(c, ccall((:igr_run_sensitivity, libigr), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cdouble, Cdouble, Cdouble, Cdouble, Cint, Cint, Cint, Cdouble,
+                    Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cint}),
+                   c.h, 1, s0, Δt, T, γ, α, n_iter, solver_code(solver), form, sγ, s, ς, ns))
+    q = similar(q0); Σ = zeros(c.N)
+    get_state!(c, q, Σ)
+    return q, Σ, s, ς
+end
+
+# Batched ensembles: one igr_batch_run_sensitivity call for many contexts
+# (struct igr_batch_item of include/igr.h, field for field).
+struct BatchItem
+    ctx::Ptr{Cvoid}
+    dt::Cdouble; t_end::Cdouble; gamma::Cdouble; alpha::Cdouble; sgam::Cdouble
+    n_tan::Cint
+    s0::Ptr{Cdouble}; s_out::Ptr{Cdouble}; vs_out::Ptr{Cdouble}
+    n_steps::Cint; status::Cint
+end
+
+# runs: vector of (ctx, q0, s0 (n_tan*nv*N, may be empty), Δt, T, γ, α, sγ)
+function run_sensitivity_batch(runs; n_iter::Int=10, solver::Symbol=:pcg, elliptic_rhs::Symbol=:form2)
+    form = elliptic_rhs == :form1 ? 1 : 2
+    outs = [(similar(r[3]), zeros(length(r[3]) == 0 ? 0 : div(length(r[3]), r[1].nv))) for r in runs]
+    for r in runs
+        set_state!(r[1], r[2])
+    end
+    items = [BatchItem(r[1].h, r[4], r[5], r[6], r[7], r[8], div(length(r[3]), r[1].nv * r[1].N),
+                       length(r[3]) == 0 ? C_NULL : pointer(r[3]),
+                       length(r[3]) == 0 ? C_NULL : pointer(o[1]), length(r[3]) == 0 ? C_NULL : pointer(o[2]),
+                       0, 0) for (r, o) in zip(runs, outs)]
+    rc = GC.@preserve runs outs ccall((:igr_batch_run_sensitivity, libigr), Cint,
+                                      (Ptr{BatchItem}, Cint, Cint, Cint, Cint),
+                                      items, length(items), n_iter, solver_code(solver), form)
+    rc == IGR_OK || check(runs[1][1], rc)
+    return outs
+end
+
+# Tile shapes (2D, p = 3): set_option!(c, OPT_TILE, 804), or autotune!(c) on the current state.
+set_option!(c::Context, opt, value) =
+    check(c, ccall((:igr_set_option, libigr), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, opt, value))
+autotune!(c::Context) = set_option!(c, OPT_AUTOTUNE, 0)
 
 # ChainRulesCore rrule for the loss J(q0) (ChainRulesCore is not a dependency of
 # the reference; load it in the caller's environment to enable this rule):
