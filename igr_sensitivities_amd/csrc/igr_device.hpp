@@ -81,37 +81,62 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* red, double* p
   }
 }
 
+constexpr int MAX_WAVES = 16;   // waves of a 1024-thread workgroup
+
 // Store this workgroup's NV partial sums at part[k*nb + bid] (no atomics,
-// no fences: the consumer is the next kernel on the stream).
+// no fences: the consumer is the next kernel on the stream).  One barrier;
+// the per-wave sums go through this helper's own LDS, so it must run once
+// per kernel, after the kernel's last use of its other LDS reductions.
 template <int NV>
-__device__ __forceinline__ void block_partials_store(double (&v)[NV], double* red, double* part, int nb, int bid) {
-  block_sum<NV>(v, red);
+__device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*red*/, double* part, int nb,
+                                                     int bid) {
+  __shared__ double sp[NV * MAX_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sp[k * nw + wid] = v[k];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[(size_t)k * nb + bid] = v[k];
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += sp[k * nw + w];
+      part[(size_t)k * nb + bid] = s;
+    }
   }
 }
 
 // Every workgroup sums the nb partials of the previous kernel in the same
-// fixed order, so all workgroups hold bit-identical totals.
+// fixed order (per thread, then per wave, then over the waves), so all
+// workgroups hold bit-identical totals.  One barrier; own LDS, used once
+// per kernel.
 template <int NV>
-__device__ __forceinline__ void grid_total(const double* part, int nb, double (&out)[NV], double* red) {
+__device__ __forceinline__ void grid_total(const double* part, int nb, double (&out)[NV], double* /*red*/) {
+  __shared__ double sg[NV * MAX_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)k * nb + b];
-    acc[k] = s;
+    acc[k] = wave_sum(s);
   }
-  block_sum<NV>(acc, red);
-  __shared__ double bc[NV];
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) bc[k] = acc[k];
+    for (int k = 0; k < NV; ++k) sg[k * nw + wid] = acc[k];
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = bc[k];
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += sg[k * nw + w];
+    out[k] = s;
+  }
 }
 
 // Julia max(x, y): returns x on ties.
