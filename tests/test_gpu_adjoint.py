@@ -215,3 +215,20 @@ def test_blast_vortex_32_forward_and_gradient(orc):
     Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.3, a, 0.2 * a, 60, orc.PCG, 3, dt, 6, q0, u, cfg["disc"])
     close(J, Jr, 1e-12)
     close(dJ, np.vdot(qb, u) + 0.3 * gb + 0.2 * a * ab, 1e-9)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_gradient_other_degrees(orc, p):
+    """The 2D adjoint at p = 2 and p = 4 against the oracle's JVP."""
+    nx, ny, ns = 5, 4, 2
+    c = ctx2(p, nx, ny)
+    g = orc.Grid2D(p, nx, ny)
+    q0 = configs.smooth_random2(p, nx, ny, seed=130 + p, amp=0.3)
+    gam, a = 1.4, 4e-3
+    dt = 0.3 * (1 / nx) / ((2 * p + 1) * configs.max_wavespeed2(q0, gam))
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(1, ns, dt, gam, a, 12, "pcg")
+    u = np.random.default_rng(131).standard_normal(q0.shape)
+    Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.4, a, 0.3 * a, 12, orc.PCG, ns, dt, 1, q0, u)
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.4 * gb + 0.3 * a * ab, 1e-10)

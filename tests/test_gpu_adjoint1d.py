@@ -83,3 +83,19 @@ def test_enzyme_adjoint_ic_1d_primitive_form():
     np.testing.assert_allclose(ap, qb[2] / (gam - 1), rtol=1e-14, atol=1e-300)
     with pytest.raises(ValueError):
         c.gradient(3, ns, dt, gam, 2e-3, 6, "pcg")     # the windowed objectives are 2D only
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_gradient_1d_other_degrees(orc, p):
+    ne, ns = 8, 2
+    q0 = configs.smooth_random1(p, ne, seed=350 + p, amp=0.3)
+    gam = 1.4
+    dt = 0.3 / ne / ((2 * p + 1) * configs.max_wavespeed1(q0, gam))
+    c = igr.Context(igr.PeriodicMesh1D(ne, 1.0, igr.DGBasis(p)))
+    c.set_state(q0)
+    J, qb, gb, ab = c.gradient(0, ns, dt, gam, 3e-3, 9, "chebyshev")
+    u = np.random.default_rng(360 + p).standard_normal(q0.shape)
+    Jr, dJ, _, _ = orc.Grid1D(p, ne).jvp_objective_run(gam, 0.2, 3e-3, 0.5 * 3e-3, 9, orc.CHEBYSHEV, ns, dt, 0,
+                                                       q0, u)
+    close(J, Jr, 1e-12)
+    close(dJ, np.vdot(qb, u) + 0.2 * gb + 0.5 * 3e-3 * ab, 1e-10)

@@ -331,3 +331,53 @@ def test_quasi1d_gpu_2d_equals_gpu_1d(alpha_on):
         ynu = np.max(np.abs(a2 - a2[:1, :, :1, :]))
         assert dev < 1e-9 and ynu < 1e-11, (dev, ynu)
     assert np.max(np.abs(f2[1])) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_other_degrees_step_run_2d(orc, p):
+    """p = 1, 2, 4 (the reference's 2D heat-map runs use p = 4): one step to
+    1e-12 and a short run to 1e-9 against the oracle, every solver."""
+    nx, ny = 6, 5
+    q0 = configs.smooth_random2(p, nx, ny, seed=40 + p, amp=0.3)
+    dt = 0.3 * (1 / nx) / ((2 * p + 1) * configs.max_wavespeed2(q0, 1.4))
+    g = orc.Grid2D(p, nx, ny)
+    for solver in ("pcg", "jacobi", "chebyshev"):
+        print(f"[case] 2D p={p} {solver}", flush=True)
+        c = ctx2(p, nx, ny)
+        code = igr.SOLVERS[solver]
+        c.set_state(q0)
+        c.run_steps(1, dt, 1.4, 3e-3, 20, solver)
+        q1r, _ = g.run_steps(1.4, 3e-3, 20, code, 1, dt, q0)
+        assert rel(c.get_state()[0], q1r) < 1e-12
+        c.set_state(q0)
+        ns = c.run_forward(dt, 12 * dt, 1.4, 3e-3, 20, solver)
+        qT, ST = c.get_state()
+        qTr, STr, nsr = g.run_forward(1.4, 3e-3, 20, code, dt, 12 * dt, q0)
+        assert ns == nsr and rel(qT, qTr) < 1e-9 and rel(ST, STr) < 1e-9
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_other_degrees_run_1d(orc, p):
+    """Short 1D runs at p = 1, 2, 4 against the oracle.  The bound is 1e-9 or
+    ten times the oracle's own response to a 1e-14 relative change of rho,
+    whichever is larger: the 20-iteration PCG at p = 4 is that sensitive
+    (a 1e-14 perturbation moves Sigma by ~1e-7), so no summation order can
+    match it to 1e-9."""
+    ne = 12
+    q0 = configs.smooth_random1(p, ne, seed=50 + p, amp=0.3)
+    dt = 0.2 / ne / ((2 * p + 1) * configs.max_wavespeed1(q0, 1.4))
+    g = orc.Grid1D(p, ne)
+    q1 = q0.copy()
+    q1[1] *= 1.0 + 1e-14
+    for solver in ("pcg", "jacobi", "chebyshev"):
+        print(f"[case] 1D p={p} {solver}", flush=True)
+        c = ctx1(p, ne)
+        code = igr.SOLVERS[solver]
+        c.set_state(q0)
+        ns = c.run_forward(dt, 15 * dt, 1.4, 5e-3, 20, solver)
+        qT, ST = c.get_state()
+        qTr, STr, nsr = g.run_forward(1.4, 5e-3, 20, code, dt, 15 * dt, q0)
+        qTp, STp, _ = g.run_forward(1.4, 5e-3, 20, code, dt, 15 * dt, q1)
+        tq = max(1e-9, 10 * rel(qTp, qTr))
+        tS = max(1e-9, 10 * rel(STp, STr))
+        assert ns == nsr and rel(qT, qTr) < tq and rel(ST, STr) < tS, (solver, rel(qT, qTr), tq, rel(ST, STr), tS)
