@@ -126,6 +126,26 @@ def c5_leg(igr, configs, budget_slots=40):
     return out
 
 
+def c1_leg(igr, configs, cpu_steps=200):
+    """C1: the 1D blast to t = 0.038 (N_e = 128, p = 3, Jacobi-50, CFL 0.1,
+    about 12,700 SSP-RK3 steps), one persistent workgroup; the CPU oracle
+    runs the first `cpu_steps` steps of the same run for the baseline."""
+    cfg = configs.config1()
+    ctx = igr.Context(igr.PeriodicMesh1D(cfg["Ne"], 1.0, igr.DGBasis(3)))
+    ctx.set_state(cfg["q"])
+    ctx.run_steps(2, cfg["dt"], cfg["gamma"], cfg["alpha"], cfg["n_iter"], cfg["solver"])
+    ctx.set_state(cfg["q"])
+    t0 = time.perf_counter()
+    ns = ctx.run_forward(cfg["dt"], cfg["T"], cfg["gamma"], cfg["alpha"], cfg["n_iter"], cfg["solver"])
+    t = time.perf_counter() - t0
+    n_nodes = cfg["Ne"] * 4
+    out = {"workload": "1D blast (1000 / 0.01 / 100 with tanh edges), N_e = 128, p = 3, damped Jacobi-50, "
+                       "CFL 0.1 to t = 0.038", "steps": ns, "seconds": t,
+           "dof_stage_per_s": 3 * n_nodes * 3 * ns / t}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 256x256 blast-vortex forward + gradient leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 1D blast run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by 2D block "
@@ -192,6 +213,7 @@ def main():
 
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)
+    blast1 = None if args.no_c1 else c1_leg(igr, configs)
 
     cpu = None
     if not args.no_cpu:
@@ -246,6 +268,7 @@ def main():
                      "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},
         "ensemble": ens,
         "blast_vortex_256": c5,
+        "blast_1d": blast1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
