@@ -374,3 +374,43 @@ def smooth_random1(p, Ne, seed, gamma=1.4, amp=0.1):
     pr = 1.0 + 0.5 * amp * fld()
     q[2] = pr / (gamma - 1.0) + 0.5 * q[0] ** 2 / q[1]
     return q
+
+
+# 1D single-mode pulses of IGR_adjoints_1d/initial_conditions.jl:94-204
+# (IC_BUILDERS, :218-224): (rho, u, p) = (1, 0, 1) plus A sin(2 pi x / L) on
+# the primitives each kind perturbs, E = p/(gamma-1) + rho u^2 / 2.
+PULSE_KINDS = {"acoustic": (1, 1, 1), "entropy": (1, None, 0), "density": (1, 0, 0), "pressure": (0, 0, 1),
+               "velocity": (0, 1, 0)}
+
+
+def pulse1(kind, p, Ne, A, gamma=1.4, L=1.0):
+    """build_<kind>_pulse_ic! (entropy: build_entropy_wave_ic!, u = 1)."""
+    wr, wu, wp = PULSE_KINDS[kind]
+    x = coords1(p, Ne, L)
+    s = np.sin(2 * np.pi * x / L)
+    rho = 1 + A * s if wr else np.ones_like(x)
+    u = np.ones_like(x) if wu is None else (A * s if wu else np.zeros_like(x))
+    pr = 1 + A * s if wp else np.ones_like(x)
+    q = np.zeros((3,) + x.shape)
+    q[0] = rho * u
+    q[1] = rho
+    q[2] = pr / (gamma - 1) + 0.5 * rho * u ** 2
+    return q
+
+
+def pulse1_dA(kind, p, Ne, A, gamma=1.4, L=1.0):
+    """d q0 / d A of pulse1 in conservative variables (the chain rule of
+    dJ_dA_from_adjoint, IGR_adjoints_1d/tests/run_adjoint_conservative_test.jl:22-47)."""
+    wr, wu, wp = PULSE_KINDS[kind]
+    x = coords1(p, Ne, L)
+    s = np.sin(2 * np.pi * x / L)
+    q = pulse1(kind, p, Ne, A, gamma, L)
+    rho, u = q[1], q[0] / q[1]
+    drho = (wr or 0) * s
+    du = (wu or 0) * s
+    dp = (wp or 0) * s
+    d = np.zeros_like(q)
+    d[0] = rho * du + u * drho
+    d[1] = drho
+    d[2] = dp / (gamma - 1) + 0.5 * (drho * u ** 2 + 2 * rho * u * du)
+    return d

@@ -8,8 +8,12 @@
 // Return codes: 0 ok, 1 negative pressure under sqrt (Julia DomainError),
 // -1 unknown solver/objective code (Julia ErrorException).
 #include "igr_oracle.hpp"
+#include "igr_oracle_adjpde.hpp"
 
-namespace orc { thread_local int g_domain_error = 0; }
+namespace orc {
+thread_local int g_domain_error = 0;
+thread_local int g_adj_mutation = 0;
+}
 using namespace orc;
 
 namespace {
@@ -288,6 +292,83 @@ int orc1_jvp_sensitivity_run(G1ARGS, double g, double alpha, int n_iter, int sol
   int rc = run_steps1<Dual>(G, Dual(g), Dual(alpha), n_iter, solver, n_steps, dt, qd.data(), Sd.data());
   from_dual(qd, q, dq); from_dual(Sd, S, dS);
   return finish(rc);
+}
+
+// ------------------------------------------- continuous adjoint (PDE) ----
+// Snapshot arrays hold (n_steps + 1) records of (state, Sigma): 5N doubles in
+// 2D, 4N in 1D; `cap` is the number of records the caller allocated.
+void orc_set_adj_mutation(int m) { g_adj_mutation = m; }
+
+int orc2_adj_hyp_rhs_local(G2ARGS, double g, const double* q, const double* S, const double* a, double* da) {
+  MAKE_G2; adj_hyp_rhs_local2(G, g, q, S, a, da); return finish(0);
+}
+int orc2_adj_elliptic_rhs(G2ARGS, const double* a, const double* q, double* b) {
+  MAKE_G2; adj_elliptic_rhs2(G, a, q, b); return finish(0);
+}
+int orc2_nonlocal_source(G2ARGS, const double* Pi, const double* q, const double* S, double* Sn) {
+  MAKE_G2; nonlocal_source2(G, Pi, q, S, Sn); return finish(0);
+}
+int orc2_adj_rhs_conservative(G2ARGS, double g, double alpha, int n_iter, int solver, const double* q,
+                              const double* S, const double* a, double* Pi, double* da) {
+  MAKE_G2; return finish(adj_rhs_conservative2(G, g, alpha, n_iter, solver, q, S, a, Pi, da));
+}
+int orc2_run_forward_store(G2ARGS, double g, double alpha, int n_iter, int solver, double dt, double T,
+                           const double* q0, int cap, double* snaps, int* n_steps) {
+  MAKE_G2;
+  std::vector<double> sv;
+  int rc = run_forward_store2(G, g, alpha, n_iter, solver, dt, T, q0, sv, n_steps);
+  if (rc) return finish(rc);
+  if (sv.size() > (size_t)cap * 5 * (size_t)G.N) return -1;
+  std::copy(sv.begin(), sv.end(), snaps);
+  return finish(0);
+}
+int orc2_run_adjoint_conservative(G2ARGS, const double* snaps, int n_steps, double dt, double T, double g,
+                                  double alpha, int n_iter, int solver, int obj, const double* prm, double* a) {
+  MAKE_G2;
+  return finish(run_adjoint_conservative2(G, snaps, n_steps, dt, T, g, alpha, n_iter, solver, obj, prm, a));
+}
+int orc2_adj_terminal(G2ARGS, int obj, const double* prm, double g, const double* qT, double* a) {
+  MAKE_G2; return finish(adj_terminal2(G, obj, prm, g, qT, a));
+}
+int orc1_adj_hyp_rhs_local(G1ARGS, double g, const double* q, const double* S, const double* a, double* da) {
+  MAKE_G1; adj_hyp_rhs_local1(G, g, q, S, a, da); return finish(0);
+}
+int orc1_adj_elliptic_rhs(G1ARGS, const double* a, const double* q, double* b) {
+  MAKE_G1; adj_elliptic_rhs1(G, a, q, b); return finish(0);
+}
+int orc1_nonlocal_source(G1ARGS, const double* Pi, const double* q, const double* S, double* Sn) {
+  MAKE_G1; nonlocal_source1(G, Pi, q, S, Sn); return finish(0);
+}
+int orc1_adj_rhs_conservative(G1ARGS, double g, double alpha, int n_iter, int solver, const double* q,
+                              const double* S, const double* a, double* Pi, double* da) {
+  MAKE_G1; return finish(adj_rhs_conservative1(G, g, alpha, n_iter, solver, q, S, a, Pi, da));
+}
+int orc1_run_forward_store(G1ARGS, double g, double alpha, int n_iter, int solver, double dt, double T,
+                           const double* q0, int cap, double* snaps, int* n_steps) {
+  MAKE_G1;
+  std::vector<double> sv;
+  int rc = run_forward_store1(G, g, alpha, n_iter, solver, dt, T, q0, sv, n_steps);
+  if (rc) return finish(rc);
+  if (sv.size() > (size_t)cap * 4 * (size_t)G.N) return -1;
+  std::copy(sv.begin(), sv.end(), snaps);
+  return finish(0);
+}
+int orc1_run_adjoint_conservative(G1ARGS, const double* snaps, int n_steps, double dt, double T, double g,
+                                  double alpha, int n_iter, int solver, int obj, double* a) {
+  MAKE_G1;
+  return finish(run_adjoint_conservative1(G, snaps, n_steps, dt, T, g, alpha, n_iter, solver, obj, a));
+}
+int orc1_jvp_solve_elliptic(G1ARGS, double alpha, double dalpha, int n_iter, int solver, const double* q,
+                            const double* dq, double* S, double* dS) {
+  MAKE_G1;
+  auto qd = to_dual(q, dq, 3 * (size_t)G.N);
+  auto Sd = to_dual(S, dS, G.N);
+  int rc = solve_elliptic1<Dual>(G, Dual(alpha, dalpha), n_iter, solver, qd.data(), Sd.data());
+  from_dual(Sd, S, dS);
+  return finish(rc);
+}
+int orc1_adj_terminal(G1ARGS, int obj, double g, const double* qT, double* a) {
+  MAKE_G1; return finish(adj_terminal1(G, obj, g, qT, a));
 }
 
 }  // extern "C"

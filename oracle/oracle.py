@@ -44,6 +44,23 @@ class OracleError(RuntimeError):
     pass
 
 
+class adj_mutation:
+    """Context manager that plants one deliberate mistake in the oracle's
+    continuous adjoint (see igr_oracle_adjpde.hpp), for tests that show their
+    checks reject it."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().orc_set_adj_mutation(C.c_int(self.m))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_adj_mutation(C.c_int(0))
+        return False
+
+
 class DomainError(OracleError):
     """Negative pressure under sqrt (Julia DomainError)."""
 
@@ -245,6 +262,54 @@ class Grid2D:
                                       C.byref(J), C.byref(dJ)))
         return J.value, dJ.value
 
+    # --- continuous adjoint (IGR_adjoints_2d/adjoint_pde.jl) ---
+    def adj_hyp_rhs_local(self, gamma, q, S, a):
+        da = np.zeros_like(a)
+        _chk(lib().orc2_adj_hyp_rhs_local(*self._g(), _d(gamma), _p(q), _p(S), _p(a), _p(da)))
+        return da
+
+    def adj_elliptic_rhs(self, a, q):
+        b = self.field()
+        _chk(lib().orc2_adj_elliptic_rhs(*self._g(), _p(a), _p(q), _p(b)))
+        return b
+
+    def nonlocal_source(self, Pi, q, S):
+        Sn = self.state()
+        _chk(lib().orc2_nonlocal_source(*self._g(), _p(Pi), _p(q), _p(S), _p(Sn)))
+        return Sn
+
+    def adj_rhs_conservative(self, gamma, alpha, n_iter, solver, q, S, a, Pi0):
+        """(da, Pi): the full adjoint operator; Pi0 is the warm start of the Pi solve."""
+        da, Pi = np.zeros_like(a), np.array(Pi0, dtype=np.float64)
+        _chk(lib().orc2_adj_rhs_conservative(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _p(q),
+                                             _p(S), _p(a), _p(Pi), _p(da)))
+        return da, Pi
+
+    def adj_terminal(self, obj, gamma, qT, prm=KE_WINDOW):
+        a = self.state()
+        prm = np.ascontiguousarray(prm, dtype=np.float64)
+        _chk(lib().orc2_adj_terminal(*self._g(), _i(obj), _p(prm), _d(gamma), _p(qT), _p(a)))
+        return a
+
+    def run_forward_store(self, gamma, alpha, n_iter, solver, dt, T, q0):
+        """run_forward_store (IGR_adjoints_2d/forward.jl:111): (snaps, n_steps), snaps of shape
+        (n_steps + 1, 5, ...) holding (mx, my, rho, E, Sigma) after every step."""
+        cap = int(np.ceil(T / dt)) + 1
+        snaps = np.zeros((cap, 5) + self.shape)
+        ns = C.c_int(0)
+        _chk(lib().orc2_run_forward_store(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _d(dt),
+                                          _d(T), _p(np.ascontiguousarray(q0)), _i(cap), _p(snaps), C.byref(ns)))
+        return snaps[:ns.value + 1], ns.value
+
+    def run_adjoint_conservative(self, snaps, n_steps, dt, T, gamma, alpha, n_iter, solver, obj, prm=KE_WINDOW):
+        """run_adjoint_conservative (IGR_adjoints_2d/adjoint_pde.jl:522): adjoint fields at t = 0."""
+        a = self.state()
+        prm = np.ascontiguousarray(prm, dtype=np.float64)
+        _chk(lib().orc2_run_adjoint_conservative(*self._g(), _p(np.ascontiguousarray(snaps)), _i(n_steps), _d(dt),
+                                                 _d(T), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _i(obj),
+                                                 _p(prm), _p(a)))
+        return a
+
 
 class Grid1D:
     """Geometry handle; mirrors PeriodicMesh1D (IGR_adjoints_1d/mesh.jl:21)."""
@@ -373,3 +438,53 @@ class Grid1D:
         _chk(lib().orc1_jvp_sensitivity_run(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver),
                                             _i(n_steps), _d(dt), _p(q), _p(dq), _p(S), _p(dS)))
         return q, dq, S, dS
+
+    def jvp_solve_elliptic(self, alpha, dalpha, n_iter, solver, q, dq, S0, dS0):
+        S, dS = S0.copy(), dS0.copy()
+        _chk(lib().orc1_jvp_solve_elliptic(*self._g(), _d(alpha), _d(dalpha), _i(n_iter), _i(solver), _p(q),
+                                           _p(dq), _p(S), _p(dS)))
+        return S, dS
+
+    # --- continuous adjoint (IGR_adjoints_1d/adjoint_pde.jl) ---
+    def adj_hyp_rhs_local(self, gamma, q, S, a):
+        da = np.zeros_like(a)
+        _chk(lib().orc1_adj_hyp_rhs_local(*self._g(), _d(gamma), _p(q), _p(S), _p(a), _p(da)))
+        return da
+
+    def adj_elliptic_rhs(self, a, q):
+        b = self.field()
+        _chk(lib().orc1_adj_elliptic_rhs(*self._g(), _p(a), _p(q), _p(b)))
+        return b
+
+    def nonlocal_source(self, Pi, q, S):
+        Sn = self.state()
+        _chk(lib().orc1_nonlocal_source(*self._g(), _p(Pi), _p(q), _p(S), _p(Sn)))
+        return Sn
+
+    def adj_rhs_conservative(self, gamma, alpha, n_iter, solver, q, S, a, Pi0):
+        da, Pi = np.zeros_like(a), np.array(Pi0, dtype=np.float64)
+        _chk(lib().orc1_adj_rhs_conservative(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _p(q),
+                                             _p(S), _p(a), _p(Pi), _p(da)))
+        return da, Pi
+
+    def adj_terminal(self, obj, gamma, qT):
+        a = self.state()
+        _chk(lib().orc1_adj_terminal(*self._g(), _i(obj), _d(gamma), _p(qT), _p(a)))
+        return a
+
+    def run_forward_store(self, gamma, alpha, n_iter, solver, dt, T, q0):
+        """run_forward_store (IGR_adjoints_1d/forward.jl:137): (snaps, n_steps), snaps of shape
+        (n_steps + 1, 4, Ne, np) holding (mu, rho, E, Sigma)."""
+        cap = int(np.ceil(T / dt)) + 1
+        snaps = np.zeros((cap, 4) + self.shape)
+        ns = C.c_int(0)
+        _chk(lib().orc1_run_forward_store(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _d(dt),
+                                          _d(T), _p(np.ascontiguousarray(q0)), _i(cap), _p(snaps), C.byref(ns)))
+        return snaps[:ns.value + 1], ns.value
+
+    def run_adjoint_conservative(self, snaps, n_steps, dt, T, gamma, alpha, n_iter, solver, obj):
+        a = self.state()
+        _chk(lib().orc1_run_adjoint_conservative(*self._g(), _p(np.ascontiguousarray(snaps)), _i(n_steps), _d(dt),
+                                                 _d(T), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _i(obj),
+                                                 _p(a)))
+        return a
