@@ -51,6 +51,11 @@ PROTOTYPES = {
     "igr_apply_sip_vjp": (_i, [_vp, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
     "igr_solve_elliptic_vjp": (_i, [_vp, _d, _i, _i, _pd, _pd, _pd, _pd, _pd]),
     "igr_batch_run_sensitivity": (_i, [C.POINTER(BatchItem), _i, _i, _i, _i]),
+    "igr_run_forward_store": (_i, [_vp, _d, _d, _d, _d, _i, _i, _pi]),
+    "igr_snapshot_count": (_i, [_vp, _pi]),
+    "igr_get_snapshot": (_i, [_vp, _i, _pd, _pd]),
+    "igr_set_snapshots": (_i, [_vp, _i, _pd]),
+    "igr_run_adjoint_conservative": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _d, _i, _i, _pd]),
 }
 
 _lib = None

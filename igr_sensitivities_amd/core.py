@@ -257,6 +257,41 @@ class Context:
                                                   _p(sb), _p(qb), _p(s0b), C.byref(ab)))
         return qb, s0b, ab.value
 
+    # continuous adjoint
+    def run_forward_store(self, dt, T, gamma, alpha, n_iter, solver="pcg"):
+        """run_forward_store from the context state; the snapshots stay on the device. Returns n_steps."""
+        ns = C.c_int(0)
+        self.check(self._L.igr_run_forward_store(self.h, float(dt), float(T), float(gamma), float(alpha),
+                                                 int(n_iter), _solver_code(solver), C.byref(ns)))
+        return ns.value
+
+    def snapshot_count(self):
+        n = C.c_int(0)
+        self.check(self._L.igr_snapshot_count(self.h, C.byref(n)))
+        return n.value
+
+    def get_snapshot(self, k):
+        """(q, Sigma) of record k (0: after the initial solve, k: after step k)."""
+        q, S = np.empty((self.nv,) + self.shape), np.empty(self.shape)
+        self.check(self._L.igr_get_snapshot(self.h, int(k), _p(q), _p(S)))
+        return q, S
+
+    def set_snapshots(self, records):
+        """records: array (n_steps + 1, nv + 1, ...) of (state, Sigma)."""
+        r = _f64(records)
+        self.check(self._L.igr_set_snapshots(self.h, r.shape[0] - 1, _p(r)))
+
+    def run_adjoint_conservative(self, obj, dt, T, gamma, alpha, n_iter, solver="pcg", n_steps=None, prm=None):
+        """Adjoint fields at t = 0 (nv, ...) from the stored snapshots."""
+        if n_steps is None:
+            n_steps = self.snapshot_count()
+        a = np.empty((self.nv,) + self.shape)
+        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        self.check(self._L.igr_run_adjoint_conservative(self.h, int(obj), _p(p), int(n_steps), float(dt), float(T),
+                                                        float(gamma), float(alpha), int(n_iter),
+                                                        _solver_code(solver), _p(a)))
+        return a
+
     def time_kernel(self, kernel, gamma, alpha, reps):
         ms = C.c_double(0)
         self.check(self._L.igr_time_kernel(self.h, int(kernel), float(gamma), float(alpha), int(reps),
@@ -431,3 +466,60 @@ def run_forward(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
     S = np.zeros(mesh.shape)
     run_forward_(q, S, dt, T, basis, mesh, gamma, alpha, n_iter, solver)
     return q, S
+
+
+class Snapshots:
+    """The `snapshots` vector of run_forward_store, kept on the device of the
+    context that made it: snaps[k] is (q, Sigma) after step k (k = 0: the
+    initial state and its Sigma), copied to the host on access."""
+
+    def __init__(self, ctx, n_steps):
+        self.ctx, self.n_steps = ctx, n_steps
+
+    def __len__(self):
+        return self.n_steps + 1
+
+    def __getitem__(self, k):
+        k = range(len(self))[k]
+        return self.ctx.get_snapshot(k)
+
+    def array(self):
+        """All records as one array (n_steps + 1, nv + 1, ...)."""
+        out = np.empty((len(self), self.ctx.nv + 1) + self.ctx.shape)
+        for k in range(len(self)):
+            q, S = self[k]
+            out[k, :-1], out[k, -1] = q, S
+        return out
+
+
+def _objective_code(objective):
+    key = objective.lstrip(":") if isinstance(objective, str) else objective
+    if isinstance(key, str):
+        if key not in OBJECTIVES:
+            raise ValueError(f"Unknown objective: {objective}")
+        key = OBJECTIVES[key]
+    return int(key)
+
+
+def run_forward_store(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
+    """run_forward_store (IGR_adjoints_2d/forward.jl:111; 1D forward.jl:137):
+    returns (snapshots, n_steps); the snapshots stay in HBM (Snapshots)."""
+    c = Context(mesh)
+    c.set_state(q0)
+    n = c.run_forward_store(dt, T, gamma, alpha, n_iter, solver)
+    return Snapshots(c, n), n
+
+
+def run_adjoint_conservative(snapshots, n_steps, dt, T, basis, mesh, gamma, alpha, n_iter=10,
+                             objective="l2_density", solver="pcg", prm=None):
+    """run_adjoint_conservative (IGR_adjoints_2d/adjoint_pde.jl:522; 1D
+    adjoint_pde.jl:466): the adjoint fields (stacked conservative variables)
+    at t = 0.  `snapshots` is what run_forward_store returned, or an array
+    (n_steps + 1, nv + 1, ...) of (state, Sigma) records."""
+    code = _objective_code(objective)
+    if isinstance(snapshots, Snapshots):
+        c = snapshots.ctx
+    else:
+        c = Context(mesh)
+        c.set_snapshots(np.asarray(snapshots, dtype=np.float64))
+    return c.run_adjoint_conservative(code, dt, T, gamma, alpha, n_iter, solver, n_steps, prm)

@@ -346,6 +346,7 @@ int igr_destroy(igr_ctx* c) {
   destroy_graphs(*c);
   free_grad_work(c);
   free_tan_bufs(c);
+  if (c->snaps) (void)hipFree(c->snaps);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

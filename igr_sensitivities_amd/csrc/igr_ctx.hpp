@@ -69,10 +69,19 @@ struct Ctx {
   int htile_x = 4, htile_y = 4;
   int use_graphs = 1;
 
+  // run_forward_store snapshots: records of (state, Sigma), (nv + 1) N
+  // doubles each, for the continuous adjoint (adjpde.hip)
+  double* snaps = nullptr;
+  size_t snaps_cap = 0;       // records allocated
+  int snaps_n = -1;           // steps stored (records 0 .. snaps_n); -1: none
+  double* adjw = nullptr;     // continuous-adjoint workspace (adjpde.hip)
+
   // captured SSP-RK3 steps, keyed by their scalar parameters
   struct GraphKey {
     double dt, gamma, alpha; int n_iter, solver;
+    int kind = 0;   // 0: SSP-RK3 step; 1: step + Sigma solve (store run); 2: adjoint step
     bool operator<(const GraphKey& o) const {
+      if (kind != o.kind) return kind < o.kind;
       if (dt != o.dt) return dt < o.dt;
       if (gamma != o.gamma) return gamma < o.gamma;
       if (alpha != o.alpha) return alpha < o.alpha;

@@ -6,7 +6,7 @@ namespace igr {
 
 constexpr int K1_MAXTAN = 4;
 
-enum K1Op { OP_RUN = 0, OP_SOLVE = 1, OP_RHS = 2, OP_HYP = 3, OP_APPLY = 4 };
+enum K1Op { OP_RUN = 0, OP_SOLVE = 1, OP_RHS = 2, OP_HYP = 3, OP_APPLY = 4, OP_STORE = 5 };
 
 // One 1D instance (one workgroup).  All pointers are device memory.
 struct K1Args {
@@ -57,6 +57,7 @@ struct K1Args {
   double* stAP = nullptr;
   double* stR = nullptr;
   double* stsc = nullptr;
+  double* snaps = nullptr;        // OP_STORE: records (mu, rho, E, Sigma), 4N doubles each
   const double* x_in = nullptr;   // OP_APPLY
   double* y_out = nullptr;
   int* flag = nullptr;

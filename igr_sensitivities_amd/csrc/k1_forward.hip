@@ -23,10 +23,38 @@ namespace igr {
 //  OP_RHS       compute_igr_euler_rhs! into a.dq
 //  OP_HYP       hyperbolic RHS only (Sigma given) into a.dq
 //  OP_APPLY     y = A(rho) x with rho = state rho
+//  OP_STORE     run_forward_store (forward.jl:137-187): Sigma <- 0, a solve
+//               on the initial state, then n_steps steps each followed by a
+//               solve; (state, Sigma) stored after the initial solve and
+//               after every step
 template <int NP>
 __device__ void k1_body(const K1Args& a) {
   K1<NP> k(a);
   const int N = k.N;
+  if (a.op == OP_STORE) {
+    auto store = [&](int rec) {
+      double* d = a.snaps + (size_t)rec * 4 * N;
+      for (int n = k.tid; n < 3 * N; n += k.nt) d[n] = a.q[n];
+      for (int n = k.tid; n < N; n += k.nt) d[3 * N + n] = a.S[n];
+    };
+    for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0;
+    __syncthreads();
+    if (a.alpha > 0.0) k.solve(a.q, 0);
+    store(0);
+    for (int step = 0; step < a.n_steps; ++step) {
+      const double* st[3] = {a.q, a.q1, a.q2};
+      double* ou[3] = {a.q1, a.q2, a.q};
+      const double cd[3] = {a.dt, 1.0 / 4.0 * a.dt, 2.0 / 3.0 * a.dt};
+      for (int s = 0; s < 3; ++s) {
+        if (a.alpha > 0.0) k.solve(st[s], 0);
+        else { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
+        k.hyp_stage(st[s], s + 1, cd[s], ou[s]);
+      }
+      if (a.alpha > 0.0) k.solve(a.q, 0);
+      store(step + 1);
+    }
+    return;
+  }
   if (a.op == OP_APPLY) {
     for (int n = k.tid; n < N; n += k.nt) a.invR[n] = 1.0 / a.q[N + n];
     __syncthreads();

@@ -194,6 +194,37 @@ int igr_apply_sip_vjp(igr_ctx* ctx, double alpha, const double* x, const double*
 int igr_solve_elliptic_vjp(igr_ctx* ctx, double alpha, int n_iter, int solver, const double* sigma0,
                            const double* sigmabar, double* qbar, double* sigma0bar, double* alphabar);
 
+/* Continuous ("optimise-then-discretise") adjoint.
+ * run_forward_store(q0..., dt, T, basis, mesh, gamma, alpha; n_iter, solver)
+ * (IGR_adjoints_2d/forward.jl:111-158; IGR_adjoints_1d/forward.jl:137-187):
+ * from the context state, Sigma <- 0, a Sigma solve (alpha > 0), then
+ * n_steps = ceil(T/dt) SSP-RK3 steps of dt = T/n_steps, each followed by a
+ * Sigma solve; the (state, Sigma) after the initial solve and after every
+ * step are kept in HBM as records 0..n_steps, which replace the returned
+ * `snapshots` vector.  On return the context state holds q(T). */
+int igr_run_forward_store(igr_ctx* ctx, double dt, double t_end, double gamma, double alpha, int n_iter,
+                          int solver, int* n_steps);
+/* Number of steps stored (records 0..n_steps), -1 if none. */
+int igr_snapshot_count(const igr_ctx* ctx, int* n_steps);
+/* snapshots[k + 1] of the reference: state (nv*N) and Sigma (N) of record k. */
+int igr_get_snapshot(igr_ctx* ctx, int k, double* q, double* sigma);
+/* Replace the stored snapshots by n_steps + 1 host records of (nv + 1)*N
+ * doubles each (state then Sigma). */
+int igr_set_snapshots(igr_ctx* ctx, int n_steps, const double* records);
+/* run_adjoint_conservative(snapshots, n_steps, dt, T, basis, mesh, gamma,
+ * alpha; n_iter, objective, solver) (IGR_adjoints_2d/adjoint_pde.jl:522-611;
+ * IGR_adjoints_1d/adjoint_pde.jl:466-552) over the stored snapshots: the
+ * terminal condition from the last record, then n_steps backward SSP-RK3
+ * steps of min(dt, T), step n using record n; writes the adjoint fields at
+ * t = 0 (nv*N, conservative variables).  Objectives (enum igr_objective): in
+ * 2D all of them (prm as for igr_objective; WINDOWED_DENSITY and DISC_KE have
+ * no counterpart in the reference), in 1D L2_DENSITY, L2_PRESSURE,
+ * KINETIC_ENERGY and WEIGHTED_MOMENTUM; anything else is IGR_EINVAL (the
+ * reference's ErrorException). */
+int igr_run_adjoint_conservative(igr_ctx* ctx, int objective, const double* prm, int n_steps, double dt,
+                                 double t_end, double gamma, double alpha, int n_iter, int solver,
+                                 double* a_out);
+
 /* Mean device time (ms) per launch of one hot kernel on the current state,
  * each launch stamped with start/end events by hipExtLaunchKernelGGL on the
  * context stream.  kernel 0 (fused SIP apply + PCG direction update) and 1
