@@ -26,6 +26,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GRAPHS = True      # replay captured hipGraphs (off: direct launches, for profilers)
+
+
+def new_ctx(igr, mesh):
+    c = igr.Context(mesh)
+    if not GRAPHS:
+        c.set_option(igr.core.OPT_GRAPHS, 0)
+    return c
 
 
 def cpu_model():
@@ -48,7 +56,7 @@ def ensemble_leg(igr, configs):
     for r in runs:
         mesh = (igr.PeriodicMesh1D(r["N"], 1.0, igr.DGBasis(3)) if r["dim"] == 1
                 else igr.CartesianMesh2D(r["N"], r["N"], 1.0, 1.0, igr.DGBasis(3)))
-        ctxs.append(igr.Context(mesh))
+        ctxs.append(new_ctx(igr, mesh))
     items = [dict(r, ctx=c) for r, c in zip(runs, ctxs)]
 
     def batched():
@@ -98,7 +106,7 @@ def c5_leg(igr, configs, budget_slots=40):
     a budget of `budget_slots` checkpoints (binomial recomputation)."""
     cfg = configs.config5()
     N1, ns = cfg["N"], cfg["n_steps"]
-    ctx = igr.Context(igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
+    ctx = new_ctx(igr, igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
     gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
     obj = igr.OBJECTIVES["disc_kinetic_energy"]
     out = {"workload": "2D 256x256 p=3, Gaussian blast (peak 100, sigma 0.03) beside a counter-rotating Oseen "
@@ -131,7 +139,7 @@ def c1_leg(igr, configs, cpu_steps=200):
     about 12,700 SSP-RK3 steps), one persistent workgroup; the CPU oracle
     runs the first `cpu_steps` steps of the same run for the baseline."""
     cfg = configs.config1()
-    ctx = igr.Context(igr.PeriodicMesh1D(cfg["Ne"], 1.0, igr.DGBasis(3)))
+    ctx = new_ctx(igr, igr.PeriodicMesh1D(cfg["Ne"], 1.0, igr.DGBasis(3)))
     ctx.set_state(cfg["q"])
     ctx.run_steps(2, cfg["dt"], cfg["gamma"], cfg["alpha"], cfg["n_iter"], cfg["solver"])
     ctx.set_state(cfg["q"])
@@ -146,6 +154,37 @@ def c1_leg(igr, configs, cpu_steps=200):
     return out
 
 
+def adjoint_pde_leg(igr, configs, cfg, steps, prm, qbar, t_forward):
+    """Continuous adjoint of the windowed-density objective on the headline
+    workload: run_forward_store (a Sigma solve after every step, snapshots in
+    HBM) and run_adjoint_conservative (one Pi solve per stage), timed apart;
+    the optimise-then-discretise gap is the combined relative L2 difference
+    between the adjoint fields and the discrete gradient divided by the nodal
+    mass (IGR_adjoints_2d/tests/run_adjoint_test.jl:43-76)."""
+    N = cfg["N"]
+    gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
+    obj = igr.OBJECTIVES["windowed_density"]
+    c = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(3)))
+    c.set_state(cfg["q"])          # warm-up: allocation and graph capture (same dt)
+    c.run_forward_store(dt, 2 * dt, gam, al, n_iter, "pcg")
+    c.run_adjoint_conservative(obj, dt, 2 * dt, gam, al, n_iter, "pcg", prm=prm)
+    T = steps * dt
+    c.set_state(cfg["q"])
+    t0 = time.perf_counter()
+    n = c.run_forward_store(dt, T, gam, al, n_iter, "pcg")
+    t1 = time.perf_counter()
+    a = c.run_adjoint_conservative(obj, dt, T, gam, al, n_iter, "pcg", prm=prm)
+    t2 = time.perf_counter()
+    c.close()
+    m = configs.quad2(3, N, N)
+    ref = qbar / m
+    gap = float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+    n_nodes = 16 * N * N
+    return {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": n, "forward_store_s": t1 - t0,
+            "adjoint_s": t2 - t1, "adjoint_dof_stage_per_s": 4 * n_nodes * 3 * n / (t2 - t1),
+            "total_over_forward": (t2 - t0) / t_forward, "otd_gap_vs_discrete_gradient": gap}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +195,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 256x256 blast-vortex forward + gradient leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1D blast run")
+    ap.add_argument("--no-adjpde", action="store_true", help="skip the continuous-adjoint leg")
+    ap.add_argument("--graphs-off", action="store_true",
+                    help="launch every kernel directly instead of replaying captured hipGraphs (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by 2D block "
@@ -164,10 +206,12 @@ def main():
 
     import igr_sensitivities_amd as igr
     from igr_sensitivities_amd import configs
+    global GRAPHS
+    GRAPHS = not args.graphs_off
 
     cfg = configs.config2(N=128)
     N1 = cfg["N"]
-    ctx = igr.Context(igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
+    ctx = new_ctx(igr, igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
     n_nodes = 16 * N1 * N1
     gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
 
@@ -206,10 +250,13 @@ def main():
     ctx.gradient(igr.OBJECTIVES["windowed_density"], 2, dt, gam, al, n_iter, "pcg", prm=prm)   # allocates work buffers
     ctx.set_state(cfg["q"])
     g0 = time.perf_counter()
-    J, _, dgam, dal = ctx.gradient(igr.OBJECTIVES["windowed_density"], args.steps, dt, gam, al, n_iter, "pcg", prm=prm)
+    J, qbar, dgam, dal = ctx.gradient(igr.OBJECTIVES["windowed_density"], args.steps, dt, gam, al, n_iter, "pcg",
+                                      prm=prm)
     g1 = time.perf_counter()
     t_grad = g1 - g0
     ctx.close()        # release this context's HBM (the gradient keeps whole steps) before the other legs
+
+    pde = None if args.no_adjpde else adjoint_pde_leg(igr, configs, cfg, args.steps, prm, qbar, elapsed)
 
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)
@@ -266,6 +313,7 @@ def main():
         "gradient_over_forward": t_grad / elapsed,
         "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,
                      "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},
+        "adjoint_pde": pde,
         "ensemble": ens,
         "blast_vortex_256": c5,
         "blast_1d": blast1,

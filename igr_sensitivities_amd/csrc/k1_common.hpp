@@ -11,6 +11,7 @@ enum K1Op { OP_RUN = 0, OP_SOLVE = 1, OP_RHS = 2, OP_HYP = 3, OP_APPLY = 4, OP_S
 // One 1D instance (one workgroup).  All pointers are device memory.
 struct K1Args {
   int op = OP_RUN;
+  int lds = 0;             // OP_RUN / OP_STORE: state and scratch in dynamic LDS (k1_lds_doubles of them)
   int Ne = 1;
   double J = 0.5, invJ = 2.0, dx = 1.0, eta = 16.0;
   BasisK B;
@@ -99,6 +100,14 @@ struct K1AdjArgs {
   int has_sbar = 0;
   int obj = 0;
 };
+
+// Doubles of dynamic LDS an LDS-resident single-instance run needs: the state
+// and both stage buffers (9N), Sigma and ten scratch vectors (11N), the
+// fluxes (3N) and six face values per element.
+__host__ __device__ inline size_t k1_lds_doubles(int Ne, int np) {
+  const size_t N = (size_t)Ne * np;
+  return 23 * N + 6 * (size_t)Ne;
+}
 
 #pragma clang fp contract(off)
 // polytropic_flux, IGR_adjoints_1d/euler.jl:26-35

@@ -28,7 +28,7 @@ namespace igr {
 //               solve; (state, Sigma) stored after the initial solve and
 //               after every step
 template <int NP>
-__device__ void k1_body(const K1Args& a) {
+__device__ void k1_ops(const K1Args& a) {
   K1<NP> k(a);
   const int N = k.N;
   if (a.op == OP_STORE) {
@@ -102,6 +102,51 @@ __device__ void k1_body(const K1Args& a) {
   }
 }
 
+// a.lds: the run keeps its state, stage buffers and scratch in dynamic LDS.
+// A 1D instance of the reference's sizes (C1: 512 nodes, 100 KB) fits, and
+// every solver phase then waits on LDS instead of L2 between its workgroup
+// barriers.  The state and Sigma are copied in first and back at the end;
+// tangents, snapshots and traces stay in global memory.
+template <int NP>
+__device__ void k1_body(const K1Args& a) {
+  if (!a.lds) { k1_ops<NP>(a); return; }
+  extern __shared__ double k1_lds[];
+  __shared__ __align__(16) unsigned char raw[sizeof(K1Args)];
+  K1Args* la = reinterpret_cast<K1Args*>(raw);
+  const int N = a.Ne * NP, Ne = a.Ne;
+  if (threadIdx.x == 0) {
+    __builtin_memcpy(raw, &a, sizeof(K1Args));
+    double* p = k1_lds;
+    la->q = p; p += 3 * N;
+    la->q1 = p; p += 3 * N;
+    la->q2 = p; p += 3 * N;
+    la->S = p; p += N;
+    la->invR = p; p += N;
+    la->invM = p; p += N;
+    la->b = p; p += N;
+    la->r = p; p += N;
+    la->p = p; p += N;
+    la->z = p; p += N;
+    la->Ap = p; p += N;
+    la->u = p; p += N;
+    la->g = p; p += N;
+    la->bs = p; p += N;
+    la->F = p; p += 3 * N;
+    la->fstar = p; p += 3 * Ne;
+    la->fjump = p; p += Ne;
+    la->favg = p; p += Ne;
+    la->fsig = p;
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < 3 * N; n += blockDim.x) la->q[n] = a.q[n];
+  for (int n = threadIdx.x; n < N; n += blockDim.x) la->S[n] = a.S[n];
+  __syncthreads();
+  k1_ops<NP>(*la);
+  __syncthreads();
+  for (int n = threadIdx.x; n < 3 * N; n += blockDim.x) a.q[n] = la->q[n];
+  for (int n = threadIdx.x; n < N; n += blockDim.x) a.S[n] = la->S[n];
+}
+
 // Single instance: arguments by value in the kernel-argument segment.
 template <int NP>
 __global__ __launch_bounds__(1024) void k1_run(K1Args a) { k1_body<NP>(a); }
@@ -115,12 +160,31 @@ __global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ 
   }
 }
 
+// Dynamic LDS of an LDS-resident run, kept below the 160 KB of a CU with room
+// for the kernel's static LDS.
+constexpr size_t K1_LDS_MAX_BYTES = 150 * 1024;
+
+template <int NP>
+static void launch1_run_np(const Ctx& c, K1Args a, int block) {
+  const size_t bytes = k1_lds_doubles(a.Ne, NP) * sizeof(double);
+  size_t smem = 0;
+  if ((a.op == OP_RUN || a.op == OP_STORE) && bytes <= K1_LDS_MAX_BYTES) {
+    static bool attr = false;
+    if (!attr) {
+      attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k1_run<NP>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)K1_LDS_MAX_BYTES) == hipSuccess;
+      (void)hipGetLastError();
+    }
+    if (attr) { a.lds = 1; smem = bytes; }
+  }
+  hipLaunchKernelGGL(k1_run<NP>, dim3(1), dim3(block), smem, c.stream, a);
+}
 void launch1_run(const Ctx& c, const K1Args& a, int block) {
   switch (c.np) {
-    case 2: hipLaunchKernelGGL(k1_run<2>, dim3(1), dim3(block), 0, c.stream, a); break;
-    case 3: hipLaunchKernelGGL(k1_run<3>, dim3(1), dim3(block), 0, c.stream, a); break;
-    case 4: hipLaunchKernelGGL(k1_run<4>, dim3(1), dim3(block), 0, c.stream, a); break;
-    case 5: hipLaunchKernelGGL(k1_run<5>, dim3(1), dim3(block), 0, c.stream, a); break;
+    case 2: launch1_run_np<2>(c, a, block); break;
+    case 3: launch1_run_np<3>(c, a, block); break;
+    case 4: launch1_run_np<4>(c, a, block); break;
+    case 5: launch1_run_np<5>(c, a, block); break;
   }
 }
 void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int n_wg, int block) {

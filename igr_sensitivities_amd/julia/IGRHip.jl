@@ -95,6 +95,49 @@ function enzyme_adjoint_ic(c::Context, μx0, μy0, ρ0, E0, n_steps, Δt, γ, α
     return J[], d..., gb[], ab[]
 end
 
+# run_forward_store(q0..., Δt, T, basis, mesh, γ, α; n_iter, solver)
+# (2D/forward.jl:111; 1D/forward.jl:137) -> (snapshots, n_steps).  The records
+# stay in HBM; `Snapshots` indexes them like the reference's vector of tuples.
+struct Snapshots
+    c::Context
+    n_steps::Int
+end
+Base.length(s::Snapshots) = s.n_steps + 1
+function Base.getindex(s::Snapshots, k::Int)        # snapshots[k], k = 1 .. n_steps + 1
+    1 <= k <= length(s) || throw(BoundsError(s, k))
+    q = Vector{Float64}(undef, s.c.nv * s.c.N); Σ = Vector{Float64}(undef, s.c.N)
+    check(s.c, ccall((:igr_get_snapshot, libigr), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
+                     s.c.h, k - 1, q, Σ))
+    return (ntuple(v -> q[(v - 1) * s.c.N + 1:v * s.c.N], s.c.nv)..., Σ)   # flat fields; reshape as needed
+end
+
+run_forward_store(c::Context, μx0, μy0, ρ0, E0, Δt, T, γ, α; n_iter::Int=10, solver::Symbol=:pcg) =
+    _run_forward_store(c, pack(μx0, μy0, ρ0, E0), Δt, T, γ, α, n_iter, solver)      # 2D
+run_forward_store(c::Context, μ0, ρ0, E0, Δt, T, γ, α; n_iter::Int=10, solver::Symbol=:pcg) =
+    _run_forward_store(c, pack(μ0, ρ0, E0), Δt, T, γ, α, n_iter, solver)            # 1D
+function _run_forward_store(c::Context, q0, Δt, T, γ, α, n_iter, solver)
+    set_state!(c, q0)
+    ns = Ref{Cint}(0)
+    check(c, ccall((:igr_run_forward_store, libigr), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Cint, Cint, Ptr{Cint}),
+                   c.h, Δt, T, γ, α, n_iter, solver_code(solver), ns))
+    return Snapshots(c, ns[]), Int(ns[])
+end
+
+# run_adjoint_conservative(snapshots, n_steps, Δt, T, basis, mesh, γ, α; n_iter, objective, solver)
+# (2D/adjoint_pde.jl:522; 1D/adjoint_pde.jl:466) -> the adjoint fields at t = 0.
+function run_adjoint_conservative(s::Snapshots, n_steps, Δt, T, γ, α;
+                                  n_iter::Int=10, objective::Symbol=:l2_density, solver::Symbol=:pcg)
+    haskey(OBJECTIVES, objective) || error("Unknown objective: $objective")
+    c = s.c
+    a = Vector{Float64}(undef, c.nv * c.N)
+    check(c, ccall((:igr_run_adjoint_conservative, libigr), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cint, Cint,
+                    Ptr{Cdouble}),
+                   c.h, OBJECTIVES[objective], C_NULL, n_steps, Δt, T, γ, α, n_iter, solver_code(solver), a))
+    return ntuple(v -> a[(v - 1) * c.N + 1:v * c.N], c.nv)
+end
+
 # 1D enzyme_adjoint_ic (1D/enzyme_adjoint.jl:255-314): the gradient of the
 # fixed-step objective, mapped to primitive adjoints as the reference does
 # (:306-311).  Returns (J, adj_ρ, adj_u, adj_p), plus dJ/dγ, dJ/dα.
