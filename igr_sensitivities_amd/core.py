@@ -369,7 +369,9 @@ def run_forward_(q, Sigma, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="
 def run_sensitivity(q0, s0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg", elliptic_rhs="form2",
                     sgamma=0.0):
     """run_sensitivity (IGR_adjoints_2d/sensitivity.jl:353; 1D :475): returns
-    (q, Sigma, s, varsigma) at T.  elliptic_rhs and sgamma apply in 1D only."""
+    (q, Sigma, s, varsigma) at T.  elliptic_rhs applies in 1D only; sgamma is
+    the gamma source of a d/dgamma tangent (1D as the reference; in 2D the
+    same term, which the reference's 2D tangent does not have)."""
     c = _ctx(mesh)
     code = _solver_code(solver)
     form = {"form1": 1, "form2": 2}.get(str(elliptic_rhs).lstrip(":"))
@@ -386,7 +388,7 @@ def run_sensitivity_batch(runs, n_iter=10, solver="pcg", elliptic_rhs="form2"):
 
     runs: list of dicts with keys ctx (a Context whose state is the primal
     initial state), s0 (n_tan, nv, ...) tangent initial states (or None),
-    dt, T, gamma, alpha and, in 1D, sgamma.  Each context's state ends at
+    dt, T, gamma, alpha and sgamma.  Each context's state ends at
     (q(T), Sigma(T)); returns [(s, varsigma, n_steps)] per run, each exactly
     what Context.run_sensitivity gives for that run alone.  There is no
     reference counterpart: the reference loops over run_sensitivity calls."""

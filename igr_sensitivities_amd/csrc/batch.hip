@@ -189,7 +189,7 @@ struct Rec2 {
   Ctx* c;
   TanBufs* T;
   int nt, n_steps;
-  double dts, gamma, alpha;
+  double dts, gamma, alpha, sgam;
 };
 
 }  // namespace
@@ -207,7 +207,6 @@ int igr_batch_run_sensitivity(igr_batch_item* items, int n_items, int n_iter, in
     it.n_steps = 0;
     if (!it.ctx || !(it.dt > 0.0) || !(it.t_end > 0.0) || it.n_tan < 0 || it.n_tan > K1_MAXTAN) return IGR_EINVAL;
     if (it.n_tan > 0 && !it.s0) return IGR_EINVAL;
-    if (it.ctx->dim == 2 && it.sgam != 0.0) return seterr(*it.ctx, IGR_EINVAL, "the 2D tangent has no gamma source");
     if (it.ctx->dim == 1 && form != 1 && form != 2)
       return seterr(*it.ctx, IGR_EINVAL, "Unknown elliptic_rhs (use form 1 or 2)");
     for (int j = 0; j < k; ++j)
@@ -273,7 +272,7 @@ int igr_batch_run_sensitivity(igr_batch_item* items, int n_items, int n_iter, in
       igr_batch_item& it = items[k];
       if (it.ctx->dim != 2) continue;
       Rec2 e;
-      e.c = it.ctx; e.T = tb[k]; e.nt = it.n_tan; e.gamma = it.gamma; e.alpha = it.alpha;
+      e.c = it.ctx; e.T = tb[k]; e.nt = it.n_tan; e.gamma = it.gamma; e.alpha = it.alpha; e.sgam = it.sgam;
       e.n_steps = sens2_steps(it.dt, it.t_end, e.dts);
       it.n_steps = e.n_steps;
       r2.push_back(e);
@@ -286,7 +285,7 @@ int igr_batch_run_sensitivity(igr_batch_item* items, int n_items, int n_iter, in
           const Rec2& e = r2[ids[k]];
           g_rec = &recs[k];
           if (what == 0) sens2_begin(*e.c, *e.T, e.nt);
-          else if (what == 1) sens2_step(*e.c, *e.T, e.nt, e.dts, e.gamma, e.alpha, n_iter, solver);
+          else if (what == 1) sens2_step(*e.c, *e.T, e.nt, e.dts, e.gamma, e.alpha, n_iter, solver, e.sgam);
           else sens2_end(*e.c, *e.T, e.nt, e.gamma, e.alpha, n_iter, solver);
           g_rec = nullptr;
         }

@@ -53,7 +53,8 @@ struct TanBufs {
 };
 int tan_bufs(Ctx& c, int nt, TanBufs*& T);
 void sens2_begin(Ctx& c, TanBufs& T, int nt);
-void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alpha, int n_iter, int solver);
+void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alpha, int n_iter, int solver,
+                double sgam);
 void sens2_end(Ctx& c, TanBufs& T, int nt, double gamma, double alpha, int n_iter, int solver);
 int sens2_steps(double dt, double t_end, double& dts);
 int sens1_args(Ctx& c, TanBufs& T, int n_tan, double dt, double t_end, double gamma, double alpha, int n_iter,

@@ -75,6 +75,7 @@ struct SensArgs {
   double* out;        // b (k2_sens_rhs) or the tangent stage output / ds
   int mode;           // 0: ds ; 1, 2, 3: SSP-RK3 stage
   double cdt;
+  double sgam;        // gamma source sgam * rho e of a d/dgamma tangent (0: the reference's 2D tangent)
   int* flag;
 };
 

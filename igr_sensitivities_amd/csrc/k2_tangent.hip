@@ -84,14 +84,18 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs_batch(const Sen
 #pragma clang fp contract(off)
 // sensitivity_flux_x / _y with _pressure_sensitivity (sensitivity.jl:34-76).
 // q = (mx, my, rho, E, Sigma), sv = (zx, zy, sigma, eta, varsigma)
-__device__ __forceinline__ void sflux2(int dir, double gam, const double q[5], const double sv[5], double f[4]) {
+__device__ __forceinline__ void sflux2(int dir, double gam, const double q[5], const double sv[5], double sgam,
+                                       double f[4]) {
   const double mx = q[0], my = q[1], r = q[2], E = q[3], S = q[4];
   const double zx = sv[0], zy = sv[1], sg = sv[2], et = sv[3], vs = sv[4];
   const double ux = mx / r, uy = my / r;
   const double sux = (zx - ux * sg) / r;
   const double suy = (zy - uy * sg) / r;
   const double P = (gam - 1.0) * (E - (mx * mx + my * my) / (2.0 * r));
-  const double Phi = (gam - 1.0) * (et - sg * (ux * ux + uy * uy) / 2.0 - r * (ux * sux + uy * suy));
+  double Phi = (gam - 1.0) * (et - sg * (ux * ux + uy * uy) / 2.0 - r * (ux * sux + uy * suy));
+  // gamma source of a d/dgamma tangent, as in the 1D tangent (IGR_adjoints_1d/sensitivity.jl:41);
+  // the reference's 2D tangent has none (sensitivity.jl:15-16): sgam = 0
+  if (sgam != 0.0) Phi = Phi + sgam * r * (E / r - (ux * ux + uy * uy) / 2.0);
   const double Pt = Phi + vs;
   if (dir == 0) {
     f[0] = zx * ux + mx * sux + Pt; f[1] = zx * uy + mx * suy; f[2] = zx;
@@ -148,8 +152,8 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
     }
   }
   double fxo[4], fyo[4];
-  sflux2(0, gam, qo, so, fxo);
-  sflux2(1, gam, qo, so, fyo);
+  sflux2(0, gam, qo, so, a.sgam, fxo);
+  sflux2(1, gam, qo, so, a.sgam, fyo);
 #pragma unroll
   for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
   __syncthreads();
@@ -170,8 +174,8 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
                   double f[4]) {
     const double lam = fmax_first(pwave(dir, gam, qL, a.flag), pwave(dir, gam, qR, a.flag));
     double fL[4], fR[4];
-    sflux2(dir, gam, qL, sL, fL);
-    sflux2(dir, gam, qR, sR, fR);
+    sflux2(dir, gam, qL, sL, a.sgam, fL);
+    sflux2(dir, gam, qR, sR, a.sgam, fR);
 #pragma unroll
     for (int v = 0; v < 4; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sR[v]) / 2.0;
   };

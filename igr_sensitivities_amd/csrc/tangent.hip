@@ -47,7 +47,7 @@ static SensArgs sens_base(const Ctx& c, double gamma, double alpha) {
   SensArgs a;
   a.g = geo2(c); a.B = c.basis; a.gamma = gamma; a.alpha = alpha; a.flag = c.flag;
   a.q = nullptr; a.S = c.S; a.s = nullptr; a.vs = nullptr; a.sn = nullptr; a.out = nullptr;
-  a.mode = 0; a.cdt = 0.0;
+  a.mode = 0; a.cdt = 0.0; a.sgam = 0.0;
   return a;
 }
 
@@ -68,7 +68,8 @@ void sens2_begin(Ctx& c, TanBufs& T, int nt) {
   for (int t = 0; t < nt; ++t) launch_fill(c, T.vs[t], c.N, 0.0);
 }
 
-void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alpha, int n_iter, int solver) {
+void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alpha, int n_iter, int solver,
+                double sgam) {
   const size_t N = c.N;
   const double cdt[3] = {dt, 1.0 / 4.0 * dt, 2.0 / 3.0 * dt};
   double* qst[3] = {c.q, c.q1, c.q2};
@@ -84,6 +85,7 @@ void sens2_step(Ctx& c, TanBufs& T, int nt, double dt, double gamma, double alph
       else launch_fill(c, T.vs[t], N, 0.0);
       SensArgs h = sens_base(c, gamma, alpha);
       h.q = qs; h.s = ss; h.vs = T.vs[t]; h.sn = T.s[t]; h.out = so; h.mode = st + 1; h.cdt = cdt[st];
+      h.sgam = sgam;
       launch2_sens_hyp(c, h);
     }
     HypArgs h = hyp_base(c, gamma);
@@ -148,7 +150,6 @@ int igr_run_sensitivity(igr_ctx* c, int n_tan, const double* s0, double dt, doub
   if (!c || n_tan < 1 || n_tan > 8 || !s0 || !(dt > 0.0) || !(t_end > 0.0) || n_iter < 0) return IGR_EINVAL;
   if (solver < 0 || solver > 2) return seterr(*c, IGR_EINVAL, "Unknown solver");
   if (c->dim == 1 && form != 1 && form != 2) return seterr(*c, IGR_EINVAL, "Unknown elliptic_rhs (use form 1 or 2)");
-  if (c->dim == 2 && sgam != 0.0) return seterr(*c, IGR_EINVAL, "the 2D tangent has no gamma source");
   int rc = begin_call(*c);
   if (rc) return rc;
   rc = ensure_scal(*c, n_iter);
@@ -164,7 +165,7 @@ int igr_run_sensitivity(igr_ctx* c, int n_tan, const double* s0, double dt, doub
     double dts = 0.0;
     n_steps = sens2_steps(dt, t_end, dts);
     sens2_begin(*c, *T, n_tan);
-    for (int k = 0; k < n_steps; ++k) sens2_step(*c, *T, n_tan, dts, gamma, alpha, n_iter, solver);
+    for (int k = 0; k < n_steps; ++k) sens2_step(*c, *T, n_tan, dts, gamma, alpha, n_iter, solver, sgam);
     sens2_end(*c, *T, n_tan, gamma, alpha, n_iter, solver);
   } else {
     if (n_tan > K1_MAXTAN) return seterr(*c, IGR_EINVAL, "1D tangent count above 4");

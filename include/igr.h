@@ -114,9 +114,12 @@ int igr_run_steps(igr_ctx* ctx, int n_steps, double dt, double gamma, double alp
  * states (nv*N each, back to back).  Returns the tangents s(T) and their
  * varsigma(T) (N each).  2D: n_steps = ceil(T/dt), dt = T/n_steps; 1D: the
  * reference's `while t < T - 1e-14; dt = min(dt, T - t)` loop.  form (1D
- * only): 1 = elliptic_rhs :form1, 2 = :form2; sgam (1D only): the gamma
- * source s_gamma of sensitivity_flux (sensitivity.jl:36-50).  The
- * dissipation of the tangent LLF flux uses the primal wave speed. */
+ * only): 1 = elliptic_rhs :form1, 2 = :form2; sgam: the gamma source
+ * s_gamma rho e of the tangent pressure, a d/dgamma direction (1D:
+ * sensitivity_flux, sensitivity.jl:36-50; 2D: the same term, which the
+ * reference's 2D tangent does not have, sensitivity.jl:15-16; 0 gives the
+ * reference's 2D tangent).  The dissipation of the tangent LLF flux uses the
+ * primal wave speed. */
 int igr_run_sensitivity(igr_ctx* ctx, int n_tan, const double* s0, double dt, double t_end, double gamma,
                         double alpha, int n_iter, int solver, int form, double sgam, double* s_out,
                         double* vs_out, int* n_steps_out);
@@ -127,7 +130,7 @@ typedef struct igr_batch_item {
   igr_ctx* ctx;          /* the instance's mesh; its state set with igr_set_state */
   double dt, t_end;      /* stepping as in igr_run_sensitivity */
   double gamma, alpha;
-  double sgam;           /* 1D gamma source of the tangent flux (0 in 2D) */
+  double sgam;           /* gamma source of the tangent flux, as in igr_run_sensitivity */
   int n_tan;             /* 0..4 tangent directions */
   const double* s0;      /* n_tan*nv*N tangent initial states (host); NULL when n_tan = 0 */
   double* s_out;         /* optional: n_tan*nv*N tangents s(T) */

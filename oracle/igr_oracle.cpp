@@ -93,15 +93,15 @@ int orc2_objective(G2ARGS, int obj, const double* prm, double g, const double* q
   MAKE_G2; return finish(objective2<double>(G, obj, prm, g, q, J));
 }
 int orc2_run_sensitivity(G2ARGS, double g, double alpha, int n_iter, int solver, double dt, double T,
-                         double* q, double* S, double* s, double* vs, int* n_steps) {
-  MAKE_G2; return finish(run_sensitivity2<double>(G, g, alpha, n_iter, solver, dt, T, q, S, s, vs, n_steps));
+                         double* q, double* S, double* s, double* vs, int* n_steps, double sgam) {
+  MAKE_G2; return finish(run_sensitivity2<double>(G, g, alpha, n_iter, solver, dt, T, q, S, s, vs, n_steps, sgam));
 }
 int orc2_sens_elliptic_rhs(G2ARGS, double alpha, const double* q, const double* S, const double* s, double* b) {
   MAKE_G2; sens_elliptic_rhs2<double>(G, alpha, q, S, s, b); return finish(0);
 }
 int orc2_sens_hyperbolic_rhs(G2ARGS, double g, const double* q, const double* S, const double* s,
-                             const double* vs, double* ds) {
-  MAKE_G2; sens_hyperbolic_rhs2<double>(G, g, q, S, s, vs, ds); return finish(0);
+                             const double* vs, double* ds, double sgam) {
+  MAKE_G2; sens_hyperbolic_rhs2<double>(G, g, q, S, s, vs, ds, sgam); return finish(0);
 }
 
 // ---- exact discrete JVPs (dual numbers) ----

@@ -649,7 +649,7 @@ int objective2(const Grid2& G, int obj, const double* prm, T g, const T* q, T* J
 // ------------------------------------------------ 2D tangent (sensitivity) --
 // IGR_adjoints_2d/sensitivity.jl.  s = (zx, zy, sig, eta), vs = varsigma.
 template <class T>
-inline void sflux2(int dir, T g, const T q[5], const T s[5], T f[4]) {
+inline void sflux2(int dir, T g, const T q[5], const T s[5], T f[4], T sgam = 0.0) {
   // sensitivity_flux_x/_y, sensitivity.jl:43-76 with _pressure_sensitivity :34-36
   T mx = q[0], my = q[1], r = q[2], E = q[3], S = q[4];
   T zx = s[0], zy = s[1], sg = s[2], et = s[3], vs = s[4];
@@ -657,7 +657,10 @@ inline void sflux2(int dir, T g, const T q[5], const T s[5], T f[4]) {
   T sux = (zx - ux * sg) / r;
   T suy = (zy - uy * sg) / r;
   T P = pressure2(g, mx, my, r, E);
+  // sgam: the gamma source sgam * rho e of a d/dgamma tangent, as in 1D (IGR_adjoints_1d/sensitivity.jl:41);
+  // the reference's 2D tangent has none (IGR_adjoints_2d/sensitivity.jl:15-16), so it is 0 there
   T Phi = (g - 1.0) * (et - sg * (ux * ux + uy * uy) / 2.0 - r * (ux * sux + uy * suy));
+  if (val(sgam) != 0.0) Phi = Phi + sgam * r * (E / r - (ux * ux + uy * uy) / 2.0);
   T Pt = Phi + vs;
   if (dir == 0) {
     f[0] = zx * ux + mx * sux + Pt; f[1] = zx * uy + mx * suy; f[2] = zx;
@@ -669,16 +672,16 @@ inline void sflux2(int dir, T g, const T q[5], const T s[5], T f[4]) {
 }
 // sensitivity_llf_flux_x/_y, sensitivity.jl:85-118 (primal wavespeed, no delta-lambda).
 template <class T>
-inline void sllf2(int dir, T g, const T qL[5], const T qR[5], const T sL[5], const T sR[5], T f[4]) {
+inline void sllf2(int dir, T g, const T qL[5], const T qR[5], const T sL[5], const T sR[5], T f[4], T sgam = 0.0) {
   T lam = dir == 0 ? max_(wave_x2(g, qL[0], qL[1], qL[2], qL[3]), wave_x2(g, qR[0], qR[1], qR[2], qR[3]))
                    : max_(wave_y2(g, qL[0], qL[1], qL[2], qL[3]), wave_y2(g, qR[0], qR[1], qR[2], qR[3]));
   T fL[4], fR[4];
-  sflux2(dir, g, qL, sL, fL); sflux2(dir, g, qR, sR, fR);
+  sflux2(dir, g, qL, sL, fL, sgam); sflux2(dir, g, qR, sR, fR, sgam);
   for (int v = 0; v < 4; ++v) f[v] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sR[v]) / 2.0;
 }
 // compute_sensitivity_hyperbolic_rhs!, sensitivity.jl:131-215.
 template <class T>
-void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* s, const T* vs, T* ds) {
+void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* s, const T* vs, T* ds, T sgam = 0.0) {
   const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
   double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
   std::vector<T> Fx(4 * (size_t)N), Fy(4 * (size_t)N);
@@ -686,8 +689,8 @@ void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* 
   auto gs = [&](size_t n, T o[5]) { o[0] = s[n]; o[1] = s[N + n]; o[2] = s[2 * (size_t)N + n]; o[3] = s[3 * (size_t)N + n]; o[4] = vs[n]; };
   for (int n = 0; n < N; ++n) {
     T qq[5], ss[5], f[4]; gq(n, qq); gs(n, ss);
-    sflux2(0, g, qq, ss, f); for (int v = 0; v < 4; ++v) Fx[(size_t)v * N + n] = f[v];
-    sflux2(1, g, qq, ss, f); for (int v = 0; v < 4; ++v) Fy[(size_t)v * N + n] = f[v];
+    sflux2(0, g, qq, ss, f, sgam); for (int v = 0; v < 4; ++v) Fx[(size_t)v * N + n] = f[v];
+    sflux2(1, g, qq, ss, f, sgam); for (int v = 0; v < 4; ++v) Fy[(size_t)v * N + n] = f[v];
   }
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex)
@@ -707,7 +710,7 @@ void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* 
         size_t L = G.at(np - 1, j, ex, ey), R = G.at(0, j, exR, ey);
         T qL[5], qR[5], sL[5], sR[5], f[4];
         gq(L, qL); gq(R, qR); gs(L, sL); gs(R, sR);
-        sllf2(0, g, qL, qR, sL, sR, f);
+        sllf2(0, g, qL, qR, sL, sR, f, sgam);
         for (int v = 0; v < 4; ++v) ds[(size_t)v * N + L] -= invJx * (f[v] - Fx[(size_t)v * N + L]) / B.w[np - 1];
         for (int v = 0; v < 4; ++v) ds[(size_t)v * N + R] += invJx * (f[v] - Fx[(size_t)v * N + R]) / B.w[0];
       }
@@ -719,7 +722,7 @@ void sens_hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, const T* 
         size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
         T qL[5], qR[5], sL[5], sR[5], f[4];
         gq(L, qL); gq(R, qR); gs(L, sL); gs(R, sR);
-        sllf2(1, g, qL, qR, sL, sR, f);
+        sllf2(1, g, qL, qR, sL, sR, f, sgam);
         for (int v = 0; v < 4; ++v) ds[(size_t)v * N + L] -= invJy * (f[v] - Fy[(size_t)v * N + L]) / B.w[np - 1];
         for (int v = 0; v < 4; ++v) ds[(size_t)v * N + R] += invJy * (f[v] - Fy[(size_t)v * N + R]) / B.w[0];
       }
@@ -787,7 +790,7 @@ int solve_sens_elliptic2(const Grid2& G, T alpha, int n_iter, int solver, const 
 // run_sensitivity, sensitivity.jl:353-413 (coupled primal + tangent SSP-RK3).
 template <class T>
 int run_sensitivity2(const Grid2& G, T g, T alpha, int n_iter, int solver, double dt, double Tend,
-                     T* q, T* S, T* s, T* vs, int* n_steps_out) {
+                     T* q, T* S, T* s, T* vs, int* n_steps_out, T sgam = 0.0) {
   const int N = G.N; const size_t nvN = 4 * (size_t)N;
   for (int n = 0; n < N; ++n) { S[n] = 0.0; vs[n] = 0.0; }
   std::vector<T> q1(nvN), q2(nvN), dq(nvN), s1(nvN), s2(nvN), ds(nvN);
@@ -799,7 +802,7 @@ int run_sensitivity2(const Grid2& G, T g, T alpha, int n_iter, int solver, doubl
     if (rc) return rc;
     if (alpha > 0.0) { rc = solve_sens_elliptic2(G, alpha, n_iter, solver, qs, S, ss, vs); if (rc) return rc; }
     else for (int n = 0; n < N; ++n) vs[n] = 0.0;
-    sens_hyperbolic_rhs2(G, g, qs, S, ss, vs, dso);
+    sens_hyperbolic_rhs2(G, g, qs, S, ss, vs, dso, sgam);
     return 0;
   };
   for (int st = 0; st < n_steps; ++st) {

@@ -185,12 +185,13 @@ class Grid2D:
         _chk(lib().orc2_objective(*self._g(), _i(obj), _p(prm), _d(gamma), _p(q), C.byref(J)))
         return J.value
 
-    def run_sensitivity(self, gamma, alpha, n_iter, solver, dt, T, q0, s0):
+    def run_sensitivity(self, gamma, alpha, n_iter, solver, dt, T, q0, s0, sgam=0.0):
+        """sgam: coefficient of the gamma source (d/dgamma tangent); the reference's 2D tangent has none."""
         q, s = q0.copy(), s0.copy()
         S, vs = self.field(), self.field()
         ns = _i(0)
         _chk(lib().orc2_run_sensitivity(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _d(dt),
-                                        _d(T), _p(q), _p(S), _p(s), _p(vs), C.byref(ns)))
+                                        _d(T), _p(q), _p(S), _p(s), _p(vs), C.byref(ns), _d(sgam)))
         return q, S, s, vs, ns.value
 
     def sens_elliptic_rhs(self, alpha, q, S, s):
@@ -198,9 +199,10 @@ class Grid2D:
         _chk(lib().orc2_sens_elliptic_rhs(*self._g(), _d(alpha), _p(q), _p(S), _p(s), _p(b)))
         return b
 
-    def sens_hyperbolic_rhs(self, gamma, q, S, s, vs):
+    def sens_hyperbolic_rhs(self, gamma, q, S, s, vs, sgam=0.0):
         ds = np.zeros_like(s)
-        _chk(lib().orc2_sens_hyperbolic_rhs(*self._g(), _d(gamma), _p(q), _p(S), _p(s), _p(vs), _p(ds)))
+        _chk(lib().orc2_sens_hyperbolic_rhs(*self._g(), _d(gamma), _p(q), _p(S), _p(s), _p(vs), _p(ds),
+                                            _d(sgam)))
         return ds
 
     # --- exact discrete JVPs ---

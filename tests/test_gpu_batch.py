@@ -111,5 +111,3 @@ def test_batch_errors():
         igr.run_sensitivity_batch([r, r])          # the same context twice
     with pytest.raises(ValueError):
         igr.run_sensitivity_batch([r], solver="gmres")
-    with pytest.raises(ValueError):
-        igr.run_sensitivity_batch([dict(r, sgamma=1.0)])   # no gamma source in 2D

@@ -34,6 +34,39 @@ def test_2d_tangent_matches_oracle(orc, solver, alpha):
         assert rel(S, Sr) < 1e-8 and rel(vs[0], vsr) < 1e-8
 
 
+@pytest.mark.parametrize("solver,alpha", [("chebyshev", 1.5e-3), ("pcg", 0.0)])
+def test_2d_gamma_tangent_matches_oracle(orc, solver, alpha):
+    """A d/dgamma tangent in 2D (sgam = 1, an extension: the reference's 2D
+    tangent has no gamma source) against the oracle, whose own gamma tangent
+    is pinned to central differences in gamma (test_oracle_tangent.py)."""
+    print(f"[case] 2D gamma tangent {solver} alpha={alpha}", flush=True)
+    g, q0, _, dt, gam = tg_case(orc, A=0.3)
+    mx, my, r, E = q0
+    s0 = np.zeros_like(q0)
+    s0[3] = -(gam - 1) * (E - (mx * mx + my * my) / (2 * r)) / (gam - 1) ** 2
+    T = 12 * dt
+    code = igr.SOLVERS[solver]
+    qr, Sr, sr, vsr, nsr = g.run_sensitivity(gam, alpha, 30, code, dt, T, q0, s0, sgam=1.0)
+    c = igr.Context(igr.CartesianMesh2D(8, 8, 1.0, 1.0, igr.DGBasis(3)))
+    c.set_state(q0)
+    s, vs, ns = c.run_sensitivity(s0, dt, T, gam, alpha, 30, solver, sgam=1.0)
+    assert ns == nsr
+    assert rel(s[0], sr) < 1e-9
+    if alpha > 0:
+        assert rel(vs[0], vsr) < 1e-8
+    # in a batch with the reference's tangent (sgam = 0) beside it
+    c2 = igr.Context(igr.CartesianMesh2D(8, 8, 1.0, 1.0, igr.DGBasis(3)))
+    c.set_state(q0)
+    c2.set_state(q0)
+    out = igr.run_sensitivity_batch([dict(ctx=c, s0=s0[None], dt=dt, T=T, gamma=gam, alpha=alpha, sgamma=1.0),
+                                     dict(ctx=c2, s0=s0[None], dt=dt, T=T, gamma=gam, alpha=alpha)],
+                                    n_iter=30, solver=solver)
+    np.testing.assert_array_equal(out[0][0], s)
+    assert np.max(np.abs(out[1][0] - s)) > 1e-6 * np.max(np.abs(s))
+    c.close()
+    c2.close()
+
+
 def test_2d_multiple_tangents_equal_single():
     print("[case] 2D four tangents", flush=True)
     q0 = configs.taylor_green2(3, 8, 8, A=0.3)
