@@ -321,7 +321,7 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   A(c->invR, N); A(c->invM, N); A(c->b, N); A(c->r, N); A(c->r2, N);
   A(c->pA, N); A(c->pB, N); A(c->Ap, N); A(c->xA, N); A(c->dA, N); A(c->dB, N); A(c->tmp, N);
   A(c->dscalar, 64);
-  if (dim == 1) { A(c->k1F, 3 * N); A(c->k1face, 6 * (size_t)nex); }
+  if (dim == 1) { A(c->k1F, 3 * N); A(c->k1face, 6 * (size_t)nex); A(c->k1args, sizeof(K1Args) / sizeof(double) + 1); }
   // two regions of 4 partial sums per workgroup, sized for the smallest tile
   // (2x2 elements) so any tile choice fits
   default_tiles(*c);

@@ -60,6 +60,7 @@ struct Ctx {
   // 1D whole-instance kernel workspace
   double* k1F = nullptr;      // 3N fluxes
   double* k1face = nullptr;   // 6*Ne face values
+  double* k1args = nullptr;   // device copy of the 1D kernel's argument block (K1Args)
 
   // 2D elements per workgroup (p = 3): tile_* for the SIP/solver kernels and
   // their reverse (the kernels with grid reductions), htile_* for the

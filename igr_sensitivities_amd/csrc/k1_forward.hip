@@ -28,77 +28,75 @@ namespace igr {
 //               solve; (state, Sigma) stored after the initial solve and
 //               after every step
 template <int NP>
-__device__ void k1_ops(const K1Args& a) {
-  K1<NP> k(a);
+__device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP>& k) {
   const int N = k.N;
   if (a.op == OP_STORE) {
-    auto store = [&](int rec) {
-      double* d = a.snaps + (size_t)rec * 4 * N;
-      for (int n = k.tid; n < 3 * N; n += k.nt) d[n] = a.q[n];
-      for (int n = k.tid; n < N; n += k.nt) d[3 * N + n] = a.S[n];
-    };
-    for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0;
+    for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0;
     __syncthreads();
-    if (a.alpha > 0.0) k.solve(a.q, 0);
-    store(0);
-    for (int step = 0; step < a.n_steps; ++step) {
-      const double* st[3] = {a.q, a.q1, a.q2};
-      double* ou[3] = {a.q1, a.q2, a.q};
-      const double cd[3] = {a.dt, 1.0 / 4.0 * a.dt, 2.0 / 3.0 * a.dt};
-      for (int s = 0; s < 3; ++s) {
-        if (a.alpha > 0.0) k.solve(st[s], 0);
-        else { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
-        k.hyp_stage(st[s], s + 1, cd[s], ou[s]);
+    if (a.alpha > 0.0) k.solve(k.w.q, 0);
+    for (int step = 0; step <= a.n_steps; ++step) {
+      if (step > 0) {
+        for (int s = 0; s < 3; ++s) {
+          // stage s: q -> q1, q1 -> q2, q2 -> q (selects, not arrays: no scratch)
+          const double* st = s == 0 ? k.w.q : (s == 1 ? k.w.q1 : k.w.q2);
+          double* ou = s == 0 ? k.w.q1 : (s == 1 ? k.w.q2 : k.w.q);
+          const double cd = s == 0 ? a.dt : (s == 1 ? 1.0 / 4.0 * a.dt : 2.0 / 3.0 * a.dt);
+          if (a.alpha > 0.0) k.solve(st, 0);
+          else { for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0; __syncthreads(); }
+          k.hyp_stage(st, s + 1, cd, ou);
+        }
+        if (a.alpha > 0.0) k.solve(k.w.q, 0);
       }
-      if (a.alpha > 0.0) k.solve(a.q, 0);
-      store(step + 1);
+      double* d = a.snaps + (size_t)step * 4 * N;   // record `step`: after the initial solve / after step `step`
+      for (int n = k.tid; n < 3 * N; n += k.nt) d[n] = k.w.q[n];
+      for (int n = k.tid; n < N; n += k.nt) d[3 * N + n] = k.w.S[n];
     }
     return;
   }
   if (a.op == OP_APPLY) {
-    for (int n = k.tid; n < N; n += k.nt) a.invR[n] = 1.0 / a.q[N + n];
+    for (int n = k.tid; n < N; n += k.nt) k.w.invR[n] = 1.0 / k.w.q[N + n];
     __syncthreads();
     k.apply(a.x_in, a.y_out);
     return;
   }
-  if (a.op == OP_SOLVE) { k.solve(a.q, 1); return; }
+  if (a.op == OP_SOLVE) { k.solve(k.w.q, 1); return; }
   if (a.op == OP_RHS || a.op == OP_HYP) {
     if (a.op == OP_RHS) {
-      if (a.alpha > 0.0) k.solve(a.q, 0);
-      else { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
+      if (a.alpha > 0.0) k.solve(k.w.q, 0);
+      else { for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0; __syncthreads(); }
     }
-    k.hyp_stage(a.q, 0, 0.0, a.dq);
+    k.hyp_stage(k.w.q, 0, 0.0, a.dq);
     return;
   }
   // OP_RUN
   if (a.zero_sigma) {
     for (int n = k.tid; n < N; n += k.nt) {
-      a.S[n] = 0.0;
+      k.w.S[n] = 0.0;
       for (int t = 0; t < a.n_tan; ++t) a.tvs[t][n] = 0.0;
     }
     __syncthreads();
   }
   for (int step = 0; step < a.n_steps + a.n_steps2; ++step) {
     const double dt = step < a.n_steps ? a.dt : a.dt2;
-    const double* st[3] = {a.q, a.q1, a.q2};
-    double* ou[3] = {a.q1, a.q2, a.q};
-    const double cd[3] = {dt, 1.0 / 4.0 * dt, 2.0 / 3.0 * dt};
     for (int s = 0; s < 3; ++s) {
-      if (a.alpha > 0.0) k.solve(st[s], 0);
-      else { for (int n = k.tid; n < N; n += k.nt) a.S[n] = 0.0; __syncthreads(); }
+      const double* st = s == 0 ? k.w.q : (s == 1 ? k.w.q1 : k.w.q2);
+      double* ou = s == 0 ? k.w.q1 : (s == 1 ? k.w.q2 : k.w.q);
+      const double cd = s == 0 ? dt : (s == 1 ? 1.0 / 4.0 * dt : 2.0 / 3.0 * dt);
+      if (a.alpha > 0.0) k.solve(st, 0);
+      else { for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0; __syncthreads(); }
       for (int t = 0; t < a.n_tan; ++t) {
         const double* ss = s == 0 ? a.ts[t] : (s == 1 ? a.ts1[t] : a.ts2[t]);
         double* so = s == 0 ? a.ts1[t] : (s == 1 ? a.ts2[t] : a.ts[t]);
-        if (a.alpha > 0.0) { k.sens_rhs(st[s], ss); k.iterate(a.tvs[t], a.bs, 0); }
+        if (a.alpha > 0.0) { k.sens_rhs(st, ss); k.iterate(a.tvs[t], k.w.bs, 0); }
         else { for (int n = k.tid; n < N; n += k.nt) a.tvs[t][n] = 0.0; __syncthreads(); }
-        k.sens_hyp_stage(st[s], ss, a.tvs[t], a.ts[t], s + 1, cd[s], so);
+        k.sens_hyp_stage(st, ss, a.tvs[t], a.ts[t], s + 1, cd, so);
       }
-      k.hyp_stage(st[s], s + 1, cd[s], ou[s]);
+      k.hyp_stage(st, s + 1, cd, ou);
     }
   }
   if (a.final_solve && a.alpha > 0.0) {
-    k.solve(a.q, 0);
-    for (int t = 0; t < a.n_tan; ++t) { k.sens_rhs(a.q, a.ts[t]); k.iterate(a.tvs[t], a.bs, 0); }
+    k.solve(k.w.q, 0);
+    for (int t = 0; t < a.n_tan; ++t) { k.sens_rhs(k.w.q, a.ts[t]); k.iterate(a.tvs[t], k.w.bs, 0); }
   }
 }
 
@@ -108,48 +106,27 @@ __device__ void k1_ops(const K1Args& a) {
 // barriers.  The state and Sigma are copied in first and back at the end;
 // tangents, snapshots and traces stay in global memory.
 template <int NP>
-__device__ void k1_body(const K1Args& a) {
-  if (!a.lds) { k1_ops<NP>(a); return; }
+__device__ __forceinline__ void k1_body(const K1Args& a) {
   extern __shared__ double k1_lds[];
-  __shared__ __align__(16) unsigned char raw[sizeof(K1Args)];
-  K1Args* la = reinterpret_cast<K1Args*>(raw);
-  const int N = a.Ne * NP, Ne = a.Ne;
-  if (threadIdx.x == 0) {
-    __builtin_memcpy(raw, &a, sizeof(K1Args));
-    double* p = k1_lds;
-    la->q = p; p += 3 * N;
-    la->q1 = p; p += 3 * N;
-    la->q2 = p; p += 3 * N;
-    la->S = p; p += N;
-    la->invR = p; p += N;
-    la->invM = p; p += N;
-    la->b = p; p += N;
-    la->r = p; p += N;
-    la->p = p; p += N;
-    la->z = p; p += N;
-    la->Ap = p; p += N;
-    la->u = p; p += N;
-    la->g = p; p += N;
-    la->bs = p; p += N;
-    la->F = p; p += 3 * N;
-    la->fstar = p; p += 3 * Ne;
-    la->fjump = p; p += Ne;
-    la->favg = p; p += Ne;
-    la->fsig = p;
+  K1<NP> k(a, a.lds ? k1_lds : nullptr);
+  if (a.lds) {
+    for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) k.w.q[n] = a.q[n];
+    for (int n = threadIdx.x; n < k.N; n += blockDim.x) k.w.S[n] = a.S[n];
+    __syncthreads();
   }
-  __syncthreads();
-  for (int n = threadIdx.x; n < 3 * N; n += blockDim.x) la->q[n] = a.q[n];
-  for (int n = threadIdx.x; n < N; n += blockDim.x) la->S[n] = a.S[n];
-  __syncthreads();
-  k1_ops<NP>(*la);
-  __syncthreads();
-  for (int n = threadIdx.x; n < 3 * N; n += blockDim.x) a.q[n] = la->q[n];
-  for (int n = threadIdx.x; n < N; n += blockDim.x) a.S[n] = la->S[n];
+  k1_ops<NP>(a, k);
+  if (a.lds) {
+    __syncthreads();
+    for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) a.q[n] = k.w.q[n];
+    for (int n = threadIdx.x; n < k.N; n += blockDim.x) a.S[n] = k.w.S[n];
+  }
 }
 
-// Single instance: arguments by value in the kernel-argument segment.
+// Single instance: the argument block is read from device memory.  (Taken by
+// value, it would be copied to per-lane scratch: the run keeps references to
+// it, and the kernel-argument segment has no generic address.)
 template <int NP>
-__global__ __launch_bounds__(1024) void k1_run(K1Args a) { k1_body<NP>(a); }
+__global__ __launch_bounds__(1024) void k1_run(const K1Args* __restrict__ a) { k1_body<NP>(*a); }
 // Batch: the device array is the work list (instances sorted by decreasing
 // cost); workgroup b runs entries b, b + gridDim.x, ... one after another.
 template <int NP>
@@ -177,7 +154,10 @@ static void launch1_run_np(const Ctx& c, K1Args a, int block) {
     }
     if (attr) { a.lds = 1; smem = bytes; }
   }
-  hipLaunchKernelGGL(k1_run<NP>, dim3(1), dim3(block), smem, c.stream, a);
+  // ordered on the context stream after the previous launch that read the block
+  K1Args* d = reinterpret_cast<K1Args*>(c.k1args);
+  (void)hipMemcpyAsync(d, &a, sizeof(K1Args), hipMemcpyHostToDevice, c.stream);
+  hipLaunchKernelGGL(k1_run<NP>, dim3(1), dim3(block), smem, c.stream, static_cast<const K1Args*>(d));
 }
 void launch1_run(const Ctx& c, const K1Args& a, int block) {
   switch (c.np) {

@@ -10,55 +10,77 @@
 
 namespace igr {
 
+// Work arrays of one instance: the argument block's (global memory) or
+// carved from dynamic LDS.  Held by value, so the array addresses stay in
+// registers instead of being reloaded from the argument block after stores.
+struct K1Bufs {
+  double *q, *q1, *q2, *S, *invR, *invM, *b, *r, *p, *z, *Ap, *u, *g, *F, *fstar, *fjump, *favg, *fsig, *bs;
+};
+
 template <int NP>
 struct K1 {
   const K1Args& a;
   const int N, Ne, tid, nt;
-  __device__ K1(const K1Args& args) : a(args), N(args.Ne * NP), Ne(args.Ne), tid(threadIdx.x), nt(blockDim.x) {}
+  K1Bufs w;
+  // lds: k1_lds_doubles(Ne, NP) doubles of LDS for the state, stage buffers and scratch
+  __device__ __forceinline__ K1(const K1Args& args, double* lds = nullptr)
+      : a(args), N(args.Ne * NP), Ne(args.Ne), tid(threadIdx.x), nt(blockDim.x) {
+    if (!lds) {
+      w = K1Bufs{args.q, args.q1, args.q2, args.S, args.invR, args.invM, args.b, args.r, args.p, args.z, args.Ap,
+                 args.u, args.g, args.F, args.fstar, args.fjump, args.favg, args.fsig, args.bs};
+      return;
+    }
+    const size_t n = N, e = Ne;
+    w.q = lds; w.q1 = lds + 3 * n; w.q2 = lds + 6 * n; w.S = lds + 9 * n;
+    w.invR = lds + 10 * n; w.invM = lds + 11 * n; w.b = lds + 12 * n; w.r = lds + 13 * n; w.p = lds + 14 * n;
+    w.z = lds + 15 * n; w.Ap = lds + 16 * n; w.u = lds + 17 * n; w.g = lds + 18 * n; w.bs = lds + 19 * n;
+    w.F = lds + 20 * n; w.fstar = lds + 23 * n; w.fjump = lds + 23 * n + 3 * e; w.favg = lds + 23 * n + 4 * e;
+    w.fsig = lds + 23 * n + 5 * e;
+  }
 
-  __device__ double D(int i, int k) const { return a.B.D[i * NP + k]; }
+  __device__ __forceinline__ double D(int i, int k) const { return a.B.D[i * NP + k]; }
 
   // y = A x (apply_sip!, elliptic.jl:32-116) in gather form.  x must be
   // complete (barrier) before the call; y is complete when it returns.
-  __device__ void apply(const double* x, double* y) {
+  __device__ __forceinline__ void apply(const double* x, double* y) {
     const double al = a.alpha, invJ = a.invJ;
     for (int n = tid; n < N; n += nt) {
       const int i = n % NP, e = n / NP;
       double dxi = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) dxi += D(i, k) * x[e * NP + k];
-      a.g[n] = al * a.B.w[i] * invJ * dxi * a.invR[n];
+      w.g[n] = al * a.B.w[i] * invJ * dxi * w.invR[n];
       if (i == 0) {   // face e: between element e-1 (L) and e (R)
         const int eL = e == 0 ? Ne - 1 : e - 1;
         double dL = 0.0;
 #pragma unroll
         for (int k = 0; k < NP; ++k) dL += D(NP - 1, k) * x[eL * NP + k];
-        const double irL = a.invR[eL * NP + NP - 1], irR = a.invR[n];
-        a.fjump[e] = x[eL * NP + NP - 1] - x[n];
-        a.favg[e] = invJ * (irL * dL + irR * dxi) / 2.0;
-        a.fsig[e] = a.eta * fmax(irL, irR) / a.dx;
+        const double irL = w.invR[eL * NP + NP - 1], irR = w.invR[n];
+        w.fjump[e] = x[eL * NP + NP - 1] - x[n];
+        w.favg[e] = invJ * (irL * dL + irR * dxi) / 2.0;
+        w.fsig[e] = a.eta * fmax(irL, irR) / a.dx;
       }
     }
     __syncthreads();
     for (int n = tid; n < N; n += nt) {
       const int i = n % NP, e = n / NP;
       const int eR = e == Ne - 1 ? 0 : e + 1;
-      double v = a.B.w[i] * a.J * x[n] * a.invR[n];
+      double v = a.B.w[i] * a.J * x[n] * w.invR[n];
       double st = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) st += D(k, i) * a.g[e * NP + k];
+      for (int k = 0; k < NP; ++k) st += D(k, i) * w.g[e * NP + k];
       v += st;
       // symmetry terms of the left face (this element is R) and right face (L)
-      v += -al * (a.invR[e * NP] / 2.0) * invJ * D(0, i) * a.fjump[e];
-      v += -al * (a.invR[e * NP + NP - 1] / 2.0) * invJ * D(NP - 1, i) * a.fjump[eR];
-      if (i == 0) v += al * a.favg[e] - al * a.fsig[e] * a.fjump[e];
-      if (i == NP - 1) v += -al * a.favg[eR] + al * a.fsig[eR] * a.fjump[eR];
+      v += -al * (w.invR[e * NP] / 2.0) * invJ * D(0, i) * w.fjump[e];
+      v += -al * (w.invR[e * NP + NP - 1] / 2.0) * invJ * D(NP - 1, i) * w.fjump[eR];
+      if (i == 0) v += al * w.favg[e] - al * w.fsig[e] * w.fjump[e];
+      if (i == NP - 1) v += -al * w.favg[eR] + al * w.fsig[eR] * w.fjump[eR];
       y[n] = v;
     }
     __syncthreads();
   }
 
-  __device__ double dot(const double* u, const double* v) {
+  __device__ __forceinline__ double dot(const double* u, const double* v) {
     __shared__ double sred[24];
     double s[1] = {0.0};
     for (int n = tid; n < N; n += nt) s[0] += u[n] * v[n];
@@ -72,87 +94,87 @@ struct K1 {
 
   // 1/rho, IGR source b and diag(A) of stage state qs
   // (compute_igr_rhs!, elliptic.jl:126-154; compute_sip_diagonal, :206-253)
-  __device__ void prepare(const double* qs) {
+  __device__ __forceinline__ void prepare(const double* qs) {
     const double* m = qs;
     const double* rho = qs + N;
     const double al = a.alpha, invJ = a.invJ;
     for (int n = tid; n < N; n += nt) {
-      a.invR[n] = 1.0 / rho[n];
-      a.u[n] = m[n] / rho[n];
+      w.invR[n] = 1.0 / rho[n];
+      w.u[n] = m[n] / rho[n];
     }
     __syncthreads();
     for (int n = tid; n < N; n += nt) {
       const int i = n % NP, e = n / NP;
       double du = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) du += D(i, k) * a.u[e * NP + k];
+      for (int k = 0; k < NP; ++k) du += D(i, k) * w.u[e * NP + k];
       du *= invJ;
-      a.b[n] = a.B.w[i] * a.J * (2.0 * al * (du * du));
-      const double ir = a.invR[n];
+      w.b[n] = a.B.w[i] * a.J * (2.0 * al * (du * du));
+      const double ir = w.invR[n];
       double Md = a.B.w[i] * a.J * ir;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) Md += al * a.B.w[k] * invJ * (D(k, i) * D(k, i)) * a.invR[e * NP + k];
+      for (int k = 0; k < NP; ++k) Md += al * a.B.w[k] * invJ * (D(k, i) * D(k, i)) * w.invR[e * NP + k];
       if (i == NP - 1) {
         const int eR = e == Ne - 1 ? 0 : e + 1;
-        const double sig = a.eta * fmax(ir, a.invR[eR * NP]) / a.dx;
+        const double sig = a.eta * fmax(ir, w.invR[eR * NP]) / a.dx;
         Md += -al * invJ * D(NP - 1, NP - 1) * ir;
         Md += al * sig;
       }
       if (i == 0) {
         const int eL = e == 0 ? Ne - 1 : e - 1;
-        const double sig = a.eta * fmax(a.invR[eL * NP + NP - 1], ir) / a.dx;
+        const double sig = a.eta * fmax(w.invR[eL * NP + NP - 1], ir) / a.dx;
         Md += al * invJ * D(0, 0) * ir;
         Md += al * sig;
       }
-      a.invM[n] = 1.0 / Md;
+      w.invM[n] = 1.0 / Md;
     }
     __syncthreads();
   }
 
   // fixed-iteration solve A x = b with the prepared 1/rho and diag(A)
   // (pcg_fixed! :307-339, jacobi_fixed! :359-368, chebyshev_fixed! :389-430)
-  __device__ void iterate(double* x, const double* bb, int trace) {
+  __device__ __forceinline__ void iterate(double* x, const double* bb, int trace) {
     const int nI = a.n_iter;
     if (a.solver == 0) {
-      apply(x, a.Ap);
+      apply(x, w.Ap);
       for (int n = tid; n < N; n += nt) {
-        const double r = bb[n] - a.Ap[n];
-        a.r[n] = r;
-        a.p[n] = r * a.invM[n];
+        const double r = bb[n] - w.Ap[n];
+        w.r[n] = r;
+        w.p[n] = r * w.invM[n];
         if (a.stR) a.stR[n] = r;
       }
       __syncthreads();
-      double rz = dot(a.r, a.p);
+      double rz = dot(w.r, w.p);
       for (int it = 0; it < nI; ++it) {
-        apply(a.p, a.Ap);
-        const double pAp = dot(a.p, a.Ap);
+        apply(w.p, w.Ap);
+        const double pAp = dot(w.p, w.Ap);
         const double ak = rz / (pAp + 1e-30);
         for (int n = tid; n < N; n += nt) {
-          if (a.stP) { a.stP[(size_t)it * N + n] = a.p[n]; a.stAP[(size_t)it * N + n] = a.Ap[n]; }
-          x[n] = x[n] + ak * a.p[n];
-          a.r[n] = a.r[n] - ak * a.Ap[n];
-          a.z[n] = a.r[n] * a.invM[n];
-          if (a.stR) a.stR[(size_t)(it + 1) * N + n] = a.r[n];
+          if (a.stP) { a.stP[(size_t)it * N + n] = w.p[n]; a.stAP[(size_t)it * N + n] = w.Ap[n]; }
+          x[n] = x[n] + ak * w.p[n];
+          w.r[n] = w.r[n] - ak * w.Ap[n];
+          w.z[n] = w.r[n] * w.invM[n];
+          if (a.stR) a.stR[(size_t)(it + 1) * N + n] = w.r[n];
         }
         __syncthreads();
-        const double rzn = dot(a.r, a.z);
+        const double rzn = dot(w.r, w.z);
         const double bk = rzn / (rz + 1e-30);
         if (trace && tid == 0 && a.trace) { a.trace[it] = ak; a.trace[nI + it] = bk; }
         if (tid == 0 && a.stsc) {   // rz_k (k <= n) | pAp_k | a_k | beta_k
           a.stsc[it] = rz; a.stsc[nI + 1 + it] = pAp; a.stsc[2 * nI + 1 + it] = ak; a.stsc[3 * nI + 1 + it] = bk;
           if (it == nI - 1) a.stsc[nI] = rzn;
         }
-        for (int n = tid; n < N; n += nt) a.p[n] = a.z[n] + bk * a.p[n];
+        for (int n = tid; n < N; n += nt) w.p[n] = w.z[n] + bk * w.p[n];
         __syncthreads();
         rz = rzn;
       }
       if (nI == 0 && tid == 0 && a.stsc) a.stsc[0] = rz;
     } else if (a.solver == 1) {
       for (int it = 0; it < nI; ++it) {
-        apply(x, a.Ap);
+        apply(x, w.Ap);
         for (int n = tid; n < N; n += nt) {
-          if (a.stP) { a.stP[(size_t)it * N + n] = x[n]; a.stAP[(size_t)it * N + n] = a.Ap[n]; }
-          x[n] = x[n] + (2.0 / 3.0) * (bb[n] - a.Ap[n]) * a.invM[n];
+          if (a.stP) { a.stP[(size_t)it * N + n] = x[n]; a.stAP[(size_t)it * N + n] = w.Ap[n]; }
+          x[n] = x[n] + (2.0 / 3.0) * (bb[n] - w.Ap[n]) * w.invM[n];
         }
         __syncthreads();
       }
@@ -160,10 +182,10 @@ struct K1 {
       const double lmin = 1.0, lmax = a.cheb_lmax;
       const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
       const double sigma = theta / fmax(delta, 1e-10);
-      apply(x, a.Ap);
+      apply(x, w.Ap);
       for (int n = tid; n < N; n += nt) {
-        a.r[n] = bb[n] - a.Ap[n];
-        if (a.stR) a.stR[n] = a.r[n];
+        w.r[n] = bb[n] - w.Ap[n];
+        if (a.stR) a.stR[n] = w.r[n];
       }
       __syncthreads();
       double rc = 1.0 / sigma;
@@ -171,38 +193,38 @@ struct K1 {
         double c1 = 0.0, c2 = 0.0;
         if (k > 1) { const double rn = 1.0 / (2.0 * sigma - rc); c1 = rn * rc; c2 = 2.0 * rn / delta; rc = rn; }
         for (int n = tid; n < N; n += nt) {
-          const double z = a.r[n] * a.invM[n];
-          const double d = (k == 1) ? z / theta : c1 * a.p[n] + c2 * z;
-          a.p[n] = d;
+          const double z = w.r[n] * w.invM[n];
+          const double d = (k == 1) ? z / theta : c1 * w.p[n] + c2 * z;
+          w.p[n] = d;
           x[n] = x[n] + d;
           if (a.stP) a.stP[(size_t)(k - 1) * N + n] = x[n];
         }
         __syncthreads();
-        apply(x, a.Ap);
+        apply(x, w.Ap);
         for (int n = tid; n < N; n += nt) {
-          a.r[n] = bb[n] - a.Ap[n];
-          if (a.stR && k < nI) a.stR[(size_t)k * N + n] = a.r[n];
+          w.r[n] = bb[n] - w.Ap[n];
+          if (a.stR && k < nI) a.stR[(size_t)k * N + n] = w.r[n];
         }
         __syncthreads();
       }
     }
   }
 
-  // solve_elliptic! (elliptic.jl:451-467) on state qs, warm start a.S
-  __device__ void solve(const double* qs, int trace) {
+  // solve_elliptic! (elliptic.jl:451-467) on state qs, warm start w.S
+  __device__ __forceinline__ void solve(const double* qs, int trace) {
     prepare(qs);
-    iterate(a.S, a.b, trace);
+    iterate(w.S, w.b, trace);
   }
 
-  // right-hand side of the tangent elliptic equation into a.bs
+  // right-hand side of the tangent elliptic equation into w.bs
   // (form 1: sensitivity.jl:168-256, SIP-like faces; form 2: :272-314, pointwise)
-  __device__ void sens_rhs(const double* qs, const double* ss) {
+  __device__ __forceinline__ void sens_rhs(const double* qs, const double* ss) {
     const double al = a.alpha, invJ = a.invJ, J = a.J;
     const double* m = qs;
     const double* r = qs + N;
     const double* sm = ss;
     const double* sr = ss + N;
-    const double* S = a.S;
+    const double* S = w.S;
     if (a.form == 2) {
       for (int n = tid; n < N; n += nt) {
         const int i = n % NP, e = n / NP;
@@ -219,7 +241,7 @@ struct K1 {
         ux *= invJ; sux *= invJ; dso *= invJ; dS *= invJ;
         const double so = sr[n] / r[n];
         const double R = al * (2.0 * ux * (2.0 * sux + so * ux) - dso * dS / r[n]);
-        a.bs[n] = a.B.w[i] * J * R;
+        w.bs[n] = a.B.w[i] * J * R;
       }
       __syncthreads();
       return;
@@ -234,9 +256,9 @@ struct K1 {
       double dL = 0.0, dR = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) { dL += D(NP - 1, k) * S[eL * NP + k]; dR += D(0, k) * S[e * NP + k]; }
-      a.fjump[e] = S[nl] - S[n];
-      a.favg[e] = invJ * (kL * dL + kR * dR) / 2.0;
-      a.fsig[e] = a.eta * fmax_first(kL, kR) / a.dx;
+      w.fjump[e] = S[nl] - S[n];
+      w.favg[e] = invJ * (kL * dL + kR * dR) / 2.0;
+      w.fsig[e] = a.eta * fmax_first(kL, kR) / a.dx;
     }
     // kappa * dSigma at every node (volume stiffness-like term)
     for (int n = tid; n < N; n += nt) {
@@ -244,7 +266,7 @@ struct K1 {
       double dS = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) dS += D(i, k) * S[e * NP + k];
-      a.g[n] = al * a.B.w[i] * invJ * (sr[n] / (r[n] * r[n])) * dS;
+      w.g[n] = al * a.B.w[i] * invJ * (sr[n] / (r[n] * r[n])) * dS;
     }
     __syncthreads();
     for (int n = tid; n < N; n += nt) {
@@ -263,47 +285,47 @@ struct K1 {
       double v = a.B.w[i] * J * src;
       double st = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) st += a.g[e * NP + k] * D(k, i);
+      for (int k = 0; k < NP; ++k) st += w.g[e * NP + k] * D(k, i);
       v += st;
       const int n0 = e * NP, n1 = e * NP + NP - 1;
       const double k0 = sr[n0] / (r[n0] * r[n0]), k1 = sr[n1] / (r[n1] * r[n1]);
-      v += -al * (k0 / 2.0) * invJ * D(0, i) * a.fjump[e];
-      v += -al * (k1 / 2.0) * invJ * D(NP - 1, i) * a.fjump[eR];
-      if (i == 0) v += al * a.favg[e] - al * a.fsig[e] * a.fjump[e];
-      if (i == NP - 1) v += -al * a.favg[eR] + al * a.fsig[eR] * a.fjump[eR];
-      a.bs[n] = v;
+      v += -al * (k0 / 2.0) * invJ * D(0, i) * w.fjump[e];
+      v += -al * (k1 / 2.0) * invJ * D(NP - 1, i) * w.fjump[eR];
+      if (i == 0) v += al * w.favg[e] - al * w.fsig[e] * w.fjump[e];
+      if (i == NP - 1) v += -al * w.favg[eR] + al * w.fsig[eR] * w.fjump[eR];
+      w.bs[n] = v;
     }
     __syncthreads();
   }
 
   // sensitivity hyperbolic RHS (sensitivity.jl:91-150) + SSP-RK3 stage of a tangent
-  __device__ void sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn, int mode,
+  __device__ __forceinline__ void sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn, int mode,
                                  double cdt, double* out);
 
   // hyperbolic RHS (hyperbolic.jl:23-83) fused with the SSP-RK3 stage update
-  __device__ void hyp_stage(const double* qs, int mode, double cdt, double* out);
+  __device__ __forceinline__ void hyp_stage(const double* qs, int mode, double cdt, double* out);
 };
 
 #pragma clang fp contract(off)
 template <int NP>
-__device__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double* out) {
+__device__ __forceinline__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double* out) {
   const double g = a.gamma, invJ = a.invJ;
   for (int n = tid; n < N; n += nt) {
     double f[3];
-    flux1(g, qs[n], qs[N + n], qs[2 * N + n], a.S[n], f);
-    a.F[n] = f[0]; a.F[N + n] = f[1]; a.F[2 * N + n] = f[2];
+    flux1(g, qs[n], qs[N + n], qs[2 * N + n], w.S[n], f);
+    w.F[n] = f[0]; w.F[N + n] = f[1]; w.F[2 * N + n] = f[2];
     const int i = n % NP, e = n / NP;
     if (i == 0) {     // LLF at face e (between e-1 and e), llf_flux euler.jl:64-78
       const int eL = e == 0 ? Ne - 1 : e - 1;
       const int nl = eL * NP + NP - 1;
-      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], a.S[nl]};
-      const double qR[4] = {qs[n], qs[N + n], qs[2 * N + n], a.S[n]};
+      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], w.S[nl]};
+      const double qR[4] = {qs[n], qs[N + n], qs[2 * N + n], w.S[n]};
       const double lam = fmax_first(wave1(g, qL, a.flag), wave1(g, qR, a.flag));
       double fL[3], fR[3];
       flux1(g, qL[0], qL[1], qL[2], qL[3], fL);
       flux1(g, qR[0], qR[1], qR[2], qR[3], fR);
 #pragma unroll
-      for (int v = 0; v < 3; ++v) a.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
+      for (int v = 0; v < 3; ++v) w.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (qL[v] - qR[v]) / 2.0;
     }
   }
   __syncthreads();
@@ -315,49 +337,49 @@ __device__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double
     for (int v = 0; v < 3; ++v) {
       acc[v] = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * a.F[v * N + e * NP + k];
+      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * w.F[v * N + e * NP + k];
     }
     if (i == NP - 1)
 #pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (a.fstar[v * Ne + eR] - a.F[v * N + n]) / a.B.w[NP - 1];
+      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (w.fstar[v * Ne + eR] - w.F[v * N + n]) / a.B.w[NP - 1];
     if (i == 0)
 #pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] += invJ * (a.fstar[v * Ne + e] - a.F[v * N + n]) / a.B.w[0];
+      for (int v = 0; v < 3; ++v) acc[v] += invJ * (w.fstar[v * Ne + e] - w.F[v * N + n]) / a.B.w[0];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       const size_t gi = (size_t)v * N + n;
       double o;
       if (mode == 0) o = acc[v];
       else if (mode == 1) o = qs[gi] + cdt * acc[v];
-      else if (mode == 2) o = 3.0 / 4.0 * a.q[gi] + 1.0 / 4.0 * qs[gi] + cdt * acc[v];
-      else o = 1.0 / 3.0 * a.q[gi] + 2.0 / 3.0 * qs[gi] + cdt * acc[v];
+      else if (mode == 2) o = 3.0 / 4.0 * w.q[gi] + 1.0 / 4.0 * qs[gi] + cdt * acc[v];
+      else o = 1.0 / 3.0 * w.q[gi] + 2.0 / 3.0 * qs[gi] + cdt * acc[v];
       out[gi] = o;
     }
   }
   __syncthreads();
 }
 template <int NP>
-__device__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
+__device__ __forceinline__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
                                        int mode, double cdt, double* out) {
   const double g = a.gamma, invJ = a.invJ;
   for (int n = tid; n < N; n += nt) {
-    const double qq[4] = {qs[n], qs[N + n], qs[2 * N + n], a.S[n]};
+    const double qq[4] = {qs[n], qs[N + n], qs[2 * N + n], w.S[n]};
     const double sv[4] = {ss[n], ss[N + n], ss[2 * N + n], svs[n]};
     double f[3];
     sflux1(g, qq, sv, a.sgam, f);
-    a.F[n] = f[0]; a.F[N + n] = f[1]; a.F[2 * N + n] = f[2];
+    w.F[n] = f[0]; w.F[N + n] = f[1]; w.F[2 * N + n] = f[2];
     const int i = n % NP, e = n / NP;
     if (i == 0) {     // sensitivity_llf_flux at face e, primal wave speed (sensitivity.jl:58-78)
       const int eL = e == 0 ? Ne - 1 : e - 1;
       const int nl = eL * NP + NP - 1;
-      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], a.S[nl]};
+      const double qL[4] = {qs[nl], qs[N + nl], qs[2 * N + nl], w.S[nl]};
       const double sL[4] = {ss[nl], ss[N + nl], ss[2 * N + nl], svs[nl]};
       const double lam = fmax_first(wave1(g, qL, a.flag), wave1(g, qq, a.flag));
       double fL[3], fR[3];
       sflux1(g, qL, sL, a.sgam, fL);
       sflux1(g, qq, sv, a.sgam, fR);
 #pragma unroll
-      for (int v = 0; v < 3; ++v) a.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sv[v]) / 2.0;
+      for (int v = 0; v < 3; ++v) w.fstar[v * Ne + e] = (fL[v] + fR[v]) / 2.0 + lam * (sL[v] - sv[v]) / 2.0;
     }
   }
   __syncthreads();
@@ -369,14 +391,14 @@ __device__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const
     for (int v = 0; v < 3; ++v) {
       acc[v] = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * a.F[v * N + e * NP + k];
+      for (int k = 0; k < NP; ++k) acc[v] -= invJ * D(i, k) * w.F[v * N + e * NP + k];
     }
     if (i == NP - 1)
 #pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (a.fstar[v * Ne + eR] - a.F[v * N + n]) / a.B.w[NP - 1];
+      for (int v = 0; v < 3; ++v) acc[v] -= invJ * (w.fstar[v * Ne + eR] - w.F[v * N + n]) / a.B.w[NP - 1];
     if (i == 0)
 #pragma unroll
-      for (int v = 0; v < 3; ++v) acc[v] += invJ * (a.fstar[v * Ne + e] - a.F[v * N + n]) / a.B.w[0];
+      for (int v = 0; v < 3; ++v) acc[v] += invJ * (w.fstar[v * Ne + e] - w.F[v * N + n]) / a.B.w[0];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       const size_t gi = (size_t)v * N + n;
