@@ -56,6 +56,7 @@ PROTOTYPES = {
     "igr_get_snapshot": (_i, [_vp, _i, _pd, _pd]),
     "igr_set_snapshots": (_i, [_vp, _i, _pd]),
     "igr_run_adjoint_conservative": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _d, _i, _i, _pd]),
+    "igr_release_cache": (_i, []),
 }
 
 _lib = None

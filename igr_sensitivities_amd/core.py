@@ -470,6 +470,11 @@ def run_forward(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
     return q, S
 
 
+def release_cache():
+    """igr_release_cache: free the gradient's kept-iterate block that outlives contexts."""
+    _lib.lib().igr_release_cache()
+
+
 class Snapshots:
     """The `snapshots` vector of run_forward_store, kept on the device of the
     context that made it: snaps[k] is (q, Sigma) after step k (k = 0: the

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "igr_ctx.hpp"
@@ -62,6 +63,71 @@ struct StepSet {
   StepView view() { return {st, Q, Sg}; }
 };
 
+// The block that holds kept whole-step iterates can be most of the HBM, and
+// fresh device memory that another allocation just released costs about
+// 30 ms per GB to obtain again (measured on MI355X: a 170 GB block, 5 s).  One
+// such block per device therefore outlives the gradient workspace that used
+// it and is handed to the next one, whichever context it belongs to, until
+// igr_release_cache() frees it.
+struct SetPool {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool in_use = false;
+};
+static std::map<int, SetPool> g_setpool;
+static std::mutex g_pool_mu;   // guards g_setpool (contexts may run on several host threads)
+
+static int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+// bytes of the idle pooled block of this device (counts as free for sizing)
+static size_t pool_idle_bytes() {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto it = g_setpool.find(current_device());
+  return (it != g_setpool.end() && !it->second.in_use) ? it->second.bytes : 0;
+}
+// a block of at least `bytes`: the pooled one if idle and large enough, else
+// a new one (which replaces an idle pooled block, or is private when the pooled
+// block is in use by another workspace)
+static void* pool_take(size_t bytes, bool& pooled) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  SetPool& sp = g_setpool[current_device()];
+  pooled = false;
+  void* q = nullptr;
+  if (!sp.in_use) {
+    if (sp.p && sp.bytes >= bytes) { sp.in_use = true; pooled = true; return sp.p; }
+    if (sp.p) { (void)hipDeviceSynchronize(); (void)hipFree(sp.p); sp.p = nullptr; sp.bytes = 0; }
+    if (hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    sp.p = q; sp.bytes = bytes; sp.in_use = true; pooled = true;
+    return q;
+  }
+  if (hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  return q;
+}
+static void pool_give(void* q, bool pooled) {
+  if (!q) return;
+  if (!pooled) { (void)hipFree(q); return; }
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (auto& kv : g_setpool)
+    if (kv.second.p == q) kv.second.in_use = false;
+}
+// frees the idle pooled block of device `dev` (caller holds g_pool_mu)
+static bool pool_drop_idle(int dev) {
+  auto it = g_setpool.find(dev);
+  if (it == g_setpool.end() || it->second.in_use || !it->second.p) return false;
+  int d = 0;
+  (void)hipGetDevice(&d);
+  (void)hipSetDevice(dev);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(it->second.p);
+  (void)hipSetDevice(d);
+  it->second.p = nullptr;
+  it->second.bytes = 0;
+  return true;
+}
+
 // Workspace of the gradient (allocated per (n_iter) on demand, kept in the context).
 struct GradWork {
   int n_iter = -1;
@@ -81,10 +147,9 @@ struct GradWork {
   std::vector<StepSet> sets;
   void* set_block = nullptr;
   size_t set_block_bytes = 0;
+  bool set_block_pooled = false;
   StepView view() { return {st, Q, Sg}; }
-  ~GradWork() {
-    if (set_block) (void)hipFree(set_block);
-  }
+  ~GradWork() { pool_give(set_block, set_block_pooled); }
 };
 
 // Doubles of one StepSet for n solver iterations on N nodes.
@@ -100,9 +165,10 @@ static int alloc_sets(Ctx& c, GradWork& W, int K) {
   const size_t N = c.N;
   const int n = W.n_iter;
   const size_t per = step_set_doubles(N, n) + 64;
-  if (W.set_block) { (void)hipFree(W.set_block); W.set_block = nullptr; W.sets.clear(); }
+  if (W.set_block) { pool_give(W.set_block, W.set_block_pooled); W.set_block = nullptr; W.sets.clear(); }
   const size_t bytes = per * sizeof(double) * (size_t)K;
-  if (hipMalloc(&W.set_block, bytes) != hipSuccess) { W.set_block = nullptr; return IGR_ENOMEM; }
+  W.set_block = pool_take(bytes, W.set_block_pooled);
+  if (!W.set_block) return IGR_ENOMEM;
   W.set_block_bytes = bytes;
   double* p = (double*)W.set_block;
   auto take = [&](size_t cnt) { double* r = p; p += (cnt + 31) / 32 * 32; return r; };
@@ -677,7 +743,7 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
     size_t fr = 0, tot = 0;
     HIPCHK(*c, hipMemGetInfo(&fr, &tot));
     const size_t per = (step_set_doubles(N, n_iter) + 64) * sizeof(double);
-    const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes));
+    const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes + pool_idle_bytes()));
     const int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
     if (K >= 1 && alloc_sets(*c, *run.W, K) == IGR_OK) run.kfirst = n_steps - 1 - K;
   }
@@ -807,7 +873,26 @@ int igr_solve_elliptic_vjp(igr_ctx* c, double alpha, int n_iter, int solver, con
 
 }  // extern "C"
 
+extern "C" int igr_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (auto& kv : g_setpool) pool_drop_idle(kv.first);
+  return IGR_OK;
+}
+
 namespace igr {
+hipError_t dev_malloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
+  (void)hipGetLastError();
+  bool dropped = false;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    dropped = pool_drop_idle(current_device());
+  }
+  if (dropped && hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
+  (void)hipGetLastError();
+  return hipErrorOutOfMemory;
+}
+
 void free_grad_work(const Ctx* c) {
   auto it = g_work.find(c);
   if (it != g_work.end()) { delete it->second; g_work.erase(it); }

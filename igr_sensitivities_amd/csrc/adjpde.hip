@@ -44,8 +44,7 @@ static int ensure_snaps(Ctx& c, int n_rec) {
   c.snaps_n = -1;
   void* p = nullptr;
   const size_t bytes = (size_t)n_rec * rec_doubles(c) * sizeof(double);
-  if (hipMalloc(&p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
+  if (dev_malloc(&p, bytes) != hipSuccess) {
     return seterr(c, IGR_ENOMEM, "cannot allocate " + std::to_string(bytes >> 20) + " MiB of HBM for " +
                                      std::to_string(n_rec) + " snapshots");
   }

@@ -43,7 +43,7 @@ struct DevBufs {
   T* put(const std::vector<T>& h) {
     if (h.empty()) return nullptr;
     void* d = nullptr;
-    if (hipMalloc(&d, h.size() * sizeof(T)) != hipSuccess) return nullptr;
+    if (dev_malloc(&d, h.size() * sizeof(T)) != hipSuccess) return nullptr;
     p.push_back(d);
     if (hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return (T*)d;

@@ -28,7 +28,7 @@ void free_tan_bufs(const Ctx* c);
 
 double* Ctx::alloc(size_t n) {
   void* p = nullptr;
-  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
+  if (dev_malloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
   owned.push_back(p);
   return static_cast<double*>(p);
 }
@@ -329,8 +329,8 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   c->partials_n = 2 * c->pstride;
   A(c->partials, c->partials_n);
   void* pp = nullptr;
-  if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
-  if (ok && hipMalloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->flag = (int*)pp; } else ok = false;
+  if (ok && dev_malloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
+  if (ok && dev_malloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->flag = (int*)pp; } else ok = false;
   if (ok) ok = ensure_scal(*c, 64) == IGR_OK;
   if (ok) ok = hipMemset(c->counter, 0, 64) == hipSuccess && hipMemset(c->flag, 0, 64) == hipSuccess &&
                hipMemset(c->S, 0, N * sizeof(double)) == hipSuccess &&

@@ -13,6 +13,10 @@
 
 namespace igr {
 
+// hipMalloc that, when the device is out of memory, first frees the idle
+// cached kept-iterate block of the current device and tries once more
+hipError_t dev_malloc(void** p, size_t bytes);
+
 struct DevBuf {
   double* p = nullptr;
   size_t n = 0;

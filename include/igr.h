@@ -228,6 +228,17 @@ int igr_run_adjoint_conservative(igr_ctx* ctx, int objective, const double* prm,
                                  double t_end, double gamma, double alpha, int n_iter, int solver,
                                  double* a_out);
 
+/* Frees the device memory the library keeps between contexts: the block that
+ * holds the gradient's whole-step iterates (up to 85% of the free HBM) stays
+ * allocated after its context is destroyed, because obtaining that much fresh
+ * HBM again costs seconds; the next gradient on the device reuses it.  Blocks
+ * in use by a live gradient workspace are not freed.  The library's own device
+ * allocations free an idle cached block by themselves when the device is out
+ * of memory; other allocators (PyTorch, another library) need this call.
+ * Thread-safe.  No counterpart in the
+ * reference (Julia's GC owns its tape). */
+int igr_release_cache(void);
+
 /* Mean device time (ms) per launch of one hot kernel on the current state,
  * each launch stamped with start/end events by hipExtLaunchKernelGGL on the
  * context stream.  kernel 0 (fused SIP apply + PCG direction update) and 1

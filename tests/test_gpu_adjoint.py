@@ -232,3 +232,23 @@ def test_gradient_other_degrees(orc, p):
     Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.4, a, 0.3 * a, 12, orc.PCG, ns, dt, 1, q0, u)
     close(J, Jr, 1e-12)
     close(dJ, np.vdot(qb, u) + 0.4 * gb + 0.3 * a * ab, 1e-10)
+
+
+def test_gradient_across_contexts_and_release_cache():
+    """The kept-iterate block outlives its context and serves the next
+    gradient (any context); results are bit-equal with a fresh block after
+    igr_release_cache."""
+    print("[case] gradient pool across contexts", flush=True)
+    q0 = configs.smooth_random2(3, 12, 12, seed=4, amp=0.2)
+    dt = 0.3 * (1 / 12) / (7 * configs.max_wavespeed2(q0, 1.4))
+    outs = []
+    for release in (False, False, True):
+        if release:
+            igr.release_cache()
+        c = igr.Context(igr.CartesianMesh2D(12, 12, 1.0, 1.0, igr.DGBasis(3)))
+        c.set_state(q0)
+        outs.append(c.gradient(0, 6, dt, 1.4, 2e-3, 15, "pcg"))
+        c.close()
+    for J, qb, gb, ab in outs[1:]:
+        assert J == outs[0][0] and gb == outs[0][2] and ab == outs[0][3]
+        np.testing.assert_array_equal(qb, outs[0][1])
