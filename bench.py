@@ -274,6 +274,11 @@ def main():
         c0 = time.perf_counter()
         g.run_steps(gam, al, n_iter, O.PCG, args.cpu_steps, dt, cfg["q"])
         c1 = time.perf_counter()
+        # the OpenMP build of the same loops, on every thread OpenMP is given
+        # (the host's CPU share), for context
+        c2 = time.perf_counter()
+        _, _, omp_threads = O.run_steps2_omp(g, gam, al, n_iter, O.PCG, args.cpu_steps, dt, cfg["q"])
+        c3 = time.perf_counter()
         cpu = {
             "value": 4 * n_nodes * 3 * args.cpu_steps / (c1 - c0),
             "unit": "DOF-stage updates/s",
@@ -282,6 +287,9 @@ def main():
             "sample": f"first {args.cpu_steps} SSP-RK3 steps of the same 128x128 PCG-60 workload, "
                       f"C++ restatement of the reference loops (-O2 -ffp-contract=off), 1 thread on "
                       f"{cpu_model()}; Julia reference not run (no Julia on the host)",
+            "openmp": {"value": 4 * n_nodes * 3 * args.cpu_steps / (c3 - c2), "threads": omp_threads,
+                       "note": "same sample, OpenMP build of the same loops (bit-identical results); "
+                               "dot products serial"},
         }
 
     out = {

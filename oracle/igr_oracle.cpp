@@ -9,6 +9,9 @@
 // -1 unknown solver/objective code (Julia ErrorException).
 #include "igr_oracle.hpp"
 #include "igr_oracle_adjpde.hpp"
+#ifdef IGR_ORACLE_OMP
+#include <omp.h>
+#endif
 
 namespace orc {
 thread_local int g_domain_error = 0;
@@ -40,6 +43,15 @@ void from_dual(const std::vector<Dual>& v, double* a, double* da) {
 #define MAKE_G1 Basis B = make_basis(p); Mesh1D M(Ne, L); Grid1 G(B, M); g_domain_error = 0
 
 extern "C" {
+
+// threads the parallel loops use (1 in the serial build)
+int orc_max_threads(void) {
+#ifdef IGR_ORACLE_OMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 int orc_basis(int p, double* xi, double* w, double* D) {
   if (p < 0 || p + 1 > MAXNP) return -1;

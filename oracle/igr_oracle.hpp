@@ -61,6 +61,19 @@ inline double val(const Dual& x) { return x.v; }
 inline double tan_(double) { return 0.0; }
 inline double tan_(const Dual& x) { return x.d; }
 
+// Loops marked IGR_PAR_FOR run in parallel in the OpenMP timing build
+// (liboracle_omp.so, compiled with -DIGR_ORACLE_OMP -fopenmp; it gives the CPU
+// throughput figure only and is never the checker).  Each marked loop writes
+// disjoint nodes from different iterations and keeps every node's order of
+// additions, so both builds produce the same bits; dot products stay serial.
+// Face loops that write both neighbours run over rows (x-faces) or columns
+// (y-faces) so the two elements a face touches belong to one iteration.
+#ifdef IGR_ORACLE_OMP
+#define IGR_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define IGR_PAR_FOR
+#endif
+
 // Domain flag: Julia's sqrt throws DomainError on a negative argument
 // (IGR_adjoints_2d/euler.jl:70, IGR_adjoints_1d/euler.jl:44).  The oracle
 // records it and carries on with NaN; the C wrapper reports it.
@@ -234,6 +247,7 @@ void hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, T* dq) {
   const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
   double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
   std::vector<T> Fx(4 * (size_t)N), Fy(4 * (size_t)N);
+  IGR_PAR_FOR
   for (int n = 0; n < N; ++n) {   // :39-48
     T f[4];
     flux_x2(g, q[n], q[N + n], q[2 * N + n], q[3 * N + n], S[n], f);
@@ -241,6 +255,7 @@ void hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, T* dq) {
     flux_y2(g, q[n], q[N + n], q[2 * N + n], q[3 * N + n], S[n], f);
     for (int v = 0; v < 4; ++v) Fy[(size_t)v * N + n] = f[v];
   }
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)     // :51-72
     for (int ex = 0; ex < M.Nex; ++ex)
       for (int j = 0; j < np; ++j)
@@ -252,6 +267,7 @@ void hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, T* dq) {
           }
           for (int v = 0; v < 4; ++v) dq[(size_t)v * N + G.at(i, j, ex, ey)] = a[v];
         }
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)     // :75-94
     for (int ex = 0; ex < M.Nex; ++ex) {
       int exR = ex == M.Nex - 1 ? 0 : ex + 1;
@@ -264,8 +280,9 @@ void hyperbolic_rhs2(const Grid2& G, T g, const T* q, const T* S, T* dq) {
         for (int v = 0; v < 4; ++v) dq[(size_t)v * N + R] += invJx * (f[v] - Fx[(size_t)v * N + R]) / B.w[0];
       }
     }
-  for (int ey = 0; ey < M.Ney; ++ey)     // :97-116
-    for (int ex = 0; ex < M.Nex; ++ex) {
+  IGR_PAR_FOR
+  for (int ex = 0; ex < M.Nex; ++ex)     // :97-116 (column-outer: same per-node order)
+    for (int ey = 0; ey < M.Ney; ++ey) {
       int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
       for (int i = 0; i < np; ++i) {
         size_t L = G.at(i, np - 1, ex, ey), R = G.at(i, 0, ex, eyR);
@@ -285,7 +302,9 @@ void apply_sip2(const Grid2& G, T alpha, const T* x, const T* rho, T* y) {
   const double Jx = M.Jx, Jy = M.Jy, invJx = 1.0 / Jx, invJy = 1.0 / Jy;
   const double eta = (double)((B.p + 1) * (B.p + 1));
   const double* w = B.w;
+  IGR_PAR_FOR
   for (int n = 0; n < N; ++n) y[n] = 0.0;
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex)
       for (int j = 0; j < np; ++j)
@@ -305,6 +324,7 @@ void apply_sip2(const Grid2& G, T alpha, const T* x, const T* rho, T* y) {
             y[G.at(i, k, ex, ey)] += cy * B.Dik(j, k);
           }
         }
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex) {
       int exR = ex == M.Nex - 1 ? 0 : ex + 1;
@@ -330,8 +350,9 @@ void apply_sip2(const Grid2& G, T alpha, const T* x, const T* rho, T* y) {
         y[R] -= alpha * sig * jump * wf;
       }
     }
-  for (int ey = 0; ey < M.Ney; ++ey)
-    for (int ex = 0; ex < M.Nex; ++ex) {
+  IGR_PAR_FOR
+  for (int ex = 0; ex < M.Nex; ++ex)     // column-outer: same per-node order
+    for (int ey = 0; ey < M.Ney; ++ey) {
       int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
       for (int i = 0; i < np; ++i) {
         double wf = w[i] * Jx;
@@ -363,6 +384,7 @@ void igr_rhs2(const Grid2& G, T alpha, const T* q, T* b) {
   const int np = G.np, N = G.N; const Basis& B = G.B; const Mesh2D& M = G.M;
   const double invJx = 1.0 / M.Jx, invJy = 1.0 / M.Jy;
   const T* mx = q; const T* my = q + N; const T* r = q + 2 * (size_t)N;
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex)
       for (int j = 0; j < np; ++j)
@@ -391,7 +413,9 @@ void sip_diag2(const Grid2& G, T alpha, const T* rho, T* dg) {
   const double Jx = M.Jx, Jy = M.Jy, invJx = 1.0 / Jx, invJy = 1.0 / Jy;
   const double eta = (double)(np * np);
   const double* w = B.w;
+  IGR_PAR_FOR
   for (int n = 0; n < N; ++n) dg[n] = 0.0;
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex) {
       for (int j = 0; j < np; ++j)
@@ -412,6 +436,7 @@ void sip_diag2(const Grid2& G, T alpha, const T* rho, T* dg) {
           }
         }
     }
+  IGR_PAR_FOR
   for (int ey = 0; ey < M.Ney; ++ey)
     for (int ex = 0; ex < M.Nex; ++ex) {
       int exR = ex == M.Nex - 1 ? 0 : ex + 1;
@@ -424,8 +449,9 @@ void sip_diag2(const Grid2& G, T alpha, const T* rho, T* dg) {
         dg[R] += (alpha * invJx * B.Dik(0, 0) / rR + alpha * sig) * wf;
       }
     }
-  for (int ey = 0; ey < M.Ney; ++ey)
-    for (int ex = 0; ex < M.Nex; ++ex) {
+  IGR_PAR_FOR
+  for (int ex = 0; ex < M.Nex; ++ex)     // column-outer: same per-node order
+    for (int ey = 0; ey < M.Ney; ++ey) {
       int eyR = ey == M.Ney - 1 ? 0 : ey + 1;
       for (int i = 0; i < np; ++i) {
         double wf = w[i] * Jx;
@@ -449,18 +475,24 @@ void pcg_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const std::v
   size_t N = x.size();
   std::vector<T> r(N), z(N), p(N), Ap(N);
   A(x.data(), Ap.data());
+  IGR_PAR_FOR
   for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ap[n];
+  IGR_PAR_FOR
   for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
   p = z;
   T rz_old = dot_(r, z);
   for (int it = 0; it < n_iter; ++it) {
     A(p.data(), Ap.data());
     T a = rz_old / (dot_(p, Ap) + 1e-30);
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) x[n] = x[n] + a * p[n];
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) r[n] = r[n] - a * Ap[n];
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
     T rz_new = dot_(r, z);
     T beta = rz_new / (rz_old + 1e-30);
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) p[n] = z[n] + beta * p[n];
     rz_old = rz_new;
     if (tr) { tr->a.push_back(val(a)); tr->b.push_back(val(beta)); }
@@ -474,6 +506,7 @@ void jacobi_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const std
   std::vector<T> Ax(N);
   for (int it = 0; it < n_iter; ++it) {
     A(x.data(), Ax.data());
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) x[n] = x[n] + omega * (b[n] - Ax[n]) / Md[n];
   }
 }
@@ -489,19 +522,25 @@ void chebyshev_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const 
   double delta = (lmax - lmin) / 2.0;
   double sigma = theta / std::max(delta, 1e-10);
   A(x.data(), Ax.data());
+  IGR_PAR_FOR
   for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ax[n];
   double rc = 1.0 / sigma;
   for (int k = 1; k <= n_iter; ++k) {
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) z[n] = r[n] / Md[n];
     if (k == 1) {
-      for (size_t n = 0; n < N; ++n) d[n] = z[n] / theta;
+      IGR_PAR_FOR
+    for (size_t n = 0; n < N; ++n) d[n] = z[n] / theta;
     } else {
       double rn = 1.0 / (2.0 * sigma - rc);
-      for (size_t n = 0; n < N; ++n) d[n] = rn * rc * d[n] + (2.0 * rn / delta) * z[n];
+      IGR_PAR_FOR
+    for (size_t n = 0; n < N; ++n) d[n] = rn * rc * d[n] + (2.0 * rn / delta) * z[n];
       rc = rn;
     }
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) x[n] = x[n] + d[n];
     A(x.data(), Ax.data());
+    IGR_PAR_FOR
     for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ax[n];
   }
 }
@@ -552,12 +591,15 @@ int ssp_rk3_step(Rhs rhs, size_t nvN, T* q, double dt, std::vector<T>& q1, std::
                  std::vector<T>& dq) {
   int rc = rhs(q, dq.data());
   if (rc) return rc;
+  IGR_PAR_FOR
   for (size_t n = 0; n < nvN; ++n) q1[n] = q[n] + dt * dq[n];
   rc = rhs(q1.data(), dq.data());
   if (rc) return rc;
+  IGR_PAR_FOR
   for (size_t n = 0; n < nvN; ++n) q2[n] = 3.0 / 4.0 * q[n] + 1.0 / 4.0 * q1[n] + 1.0 / 4.0 * dt * dq[n];
   rc = rhs(q2.data(), dq.data());
   if (rc) return rc;
+  IGR_PAR_FOR
   for (size_t n = 0; n < nvN; ++n) q[n] = 1.0 / 3.0 * q[n] + 2.0 / 3.0 * q2[n] + 2.0 / 3.0 * dt * dq[n];
   return 0;
 }

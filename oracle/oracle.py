@@ -40,6 +40,31 @@ def lib():
     return _lib
 
 
+_OMP_PATH = os.path.join(_HERE, "_build", "liboracle_omp.so")
+_omp_lib = None
+
+
+def omp_lib():
+    """The OpenMP build of the same oracle (identical results; used to time
+    the multi-core CPU baseline, never as the checker)."""
+    global _omp_lib
+    if _omp_lib is None:
+        if not os.path.exists(_OMP_PATH):
+            build()
+        _omp_lib = C.CDLL(_OMP_PATH)
+    return _omp_lib
+
+
+def run_steps2_omp(g, gamma, alpha, n_iter, solver, n_steps, dt, q0):
+    """Grid2D.run_steps on the OpenMP build: (q, S, threads)."""
+    L = omp_lib()
+    q = q0.copy()
+    S = g.field()
+    _chk(L.orc2_run_steps(*g._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _i(n_steps),
+                          _d(dt), _p(q), _p(S)))
+    return q, S, L.orc_max_threads()
+
+
 class OracleError(RuntimeError):
     pass
 
