@@ -27,9 +27,15 @@ namespace igr {
 //               on the initial state, then n_steps steps each followed by a
 //               solve; (state, Sigma) stored after the initial solve and
 //               after every step
-template <int NP>
-__device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP>& k) {
+template <int NP, bool LDS>
+__device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP, LDS>& k) {
   const int N = k.N;
+  // the stage buffers as values: a stage-indexed choice between members of
+  // `k` would become a load at a variable offset into it, which keeps the
+  // whole solver object in per-lane scratch memory
+  double* const Q0 = k.w.q;
+  double* const Q1 = k.w.q1;
+  double* const Q2 = k.w.q2;
   if (a.op == OP_STORE) {
     for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0;
     __syncthreads();
@@ -38,8 +44,8 @@ __device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP>& k) {
       if (step > 0) {
         for (int s = 0; s < 3; ++s) {
           // stage s: q -> q1, q1 -> q2, q2 -> q (selects, not arrays: no scratch)
-          const double* st = s == 0 ? k.w.q : (s == 1 ? k.w.q1 : k.w.q2);
-          double* ou = s == 0 ? k.w.q1 : (s == 1 ? k.w.q2 : k.w.q);
+          const double* st = s == 0 ? Q0 : (s == 1 ? Q1 : Q2);
+          double* ou = s == 0 ? Q1 : (s == 1 ? Q2 : Q0);
           const double cd = s == 0 ? a.dt : (s == 1 ? 1.0 / 4.0 * a.dt : 2.0 / 3.0 * a.dt);
           if (a.alpha > 0.0) k.solve(st, 0);
           else { for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0; __syncthreads(); }
@@ -79,8 +85,8 @@ __device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP>& k) {
   for (int step = 0; step < a.n_steps + a.n_steps2; ++step) {
     const double dt = step < a.n_steps ? a.dt : a.dt2;
     for (int s = 0; s < 3; ++s) {
-      const double* st = s == 0 ? k.w.q : (s == 1 ? k.w.q1 : k.w.q2);
-      double* ou = s == 0 ? k.w.q1 : (s == 1 ? k.w.q2 : k.w.q);
+      const double* st = s == 0 ? Q0 : (s == 1 ? Q1 : Q2);
+      double* ou = s == 0 ? Q1 : (s == 1 ? Q2 : Q0);
       const double cd = s == 0 ? dt : (s == 1 ? 1.0 / 4.0 * dt : 2.0 / 3.0 * dt);
       if (a.alpha > 0.0) k.solve(st, 0);
       else { for (int n = k.tid; n < N; n += k.nt) k.w.S[n] = 0.0; __syncthreads(); }
@@ -104,35 +110,39 @@ __device__ __forceinline__ void k1_ops(const K1Args& a, K1<NP>& k) {
 // A 1D instance of the reference's sizes (C1: 512 nodes, 100 KB) fits, and
 // every solver phase then waits on LDS instead of L2 between its workgroup
 // barriers.  The state and Sigma are copied in first and back at the end;
-// tangents, snapshots and traces stay in global memory.
-template <int NP>
+// tangents, snapshots and traces stay in global memory.  ALLOW_LDS = false
+// compiles the global-memory form only (the batch kernel never uses LDS).
+template <int NP, bool ALLOW_LDS>
 __device__ __forceinline__ void k1_body(const K1Args& a) {
-  extern __shared__ double k1_lds[];
-  K1<NP> k(a, a.lds ? k1_lds : nullptr);
-  if (a.lds) {
-    for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) k.w.q[n] = a.q[n];
-    for (int n = threadIdx.x; n < k.N; n += blockDim.x) k.w.S[n] = a.S[n];
-    __syncthreads();
+  if constexpr (ALLOW_LDS) {
+    if (a.lds) {
+      extern __shared__ double k1_lds[];
+      K1<NP, true> k(a, k1_lds);
+      for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) k.w.q[n] = a.q[n];
+      for (int n = threadIdx.x; n < k.N; n += blockDim.x) k.w.S[n] = a.S[n];
+      __syncthreads();
+      k1_ops<NP>(a, k);
+      __syncthreads();
+      for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) a.q[n] = k.w.q[n];
+      for (int n = threadIdx.x; n < k.N; n += blockDim.x) a.S[n] = k.w.S[n];
+      return;
+    }
   }
+  K1<NP, false> k(a);
   k1_ops<NP>(a, k);
-  if (a.lds) {
-    __syncthreads();
-    for (int n = threadIdx.x; n < 3 * k.N; n += blockDim.x) a.q[n] = k.w.q[n];
-    for (int n = threadIdx.x; n < k.N; n += blockDim.x) a.S[n] = k.w.S[n];
-  }
 }
 
 // Single instance: the argument block is read from device memory.  (Taken by
 // value, it would be copied to per-lane scratch: the run keeps references to
 // it, and the kernel-argument segment has no generic address.)
 template <int NP>
-__global__ __launch_bounds__(1024) void k1_run(const K1Args* __restrict__ a) { k1_body<NP>(*a); }
+__global__ __launch_bounds__(1024) void k1_run(const K1Args* __restrict__ a) { k1_body<NP, true>(*a); }
 // Batch: the device array is the work list (instances sorted by decreasing
 // cost); workgroup b runs entries b, b + gridDim.x, ... one after another.
 template <int NP>
 __global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ args, int n) {
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
-    k1_body<NP>(args[k]);
+    k1_body<NP, false>(args[k]);
     __syncthreads();
   }
 }

@@ -17,15 +17,19 @@ struct K1Bufs {
   double *q, *q1, *q2, *S, *invR, *invM, *b, *r, *p, *z, *Ap, *u, *g, *F, *fstar, *fjump, *favg, *fsig, *bs;
 };
 
-template <int NP>
+// LDS: the work arrays live in `lds` (k1_lds_doubles(Ne, NP) doubles of
+// dynamic LDS) instead of the argument block's global arrays.  A template
+// parameter, so that every access through them compiles to LDS instructions
+// rather than generic (flat) ones.
+template <int NP, bool LDS = false>
 struct K1 {
   const K1Args& a;
   const int N, Ne, tid, nt;
   K1Bufs w;
-  // lds: k1_lds_doubles(Ne, NP) doubles of LDS for the state, stage buffers and scratch
   __device__ __forceinline__ K1(const K1Args& args, double* lds = nullptr)
       : a(args), N(args.Ne * NP), Ne(args.Ne), tid(threadIdx.x), nt(blockDim.x) {
-    if (!lds) {
+    if constexpr (!LDS) {
+      (void)lds;
       w = K1Bufs{args.q, args.q1, args.q2, args.S, args.invR, args.invM, args.b, args.r, args.p, args.z, args.Ap,
                  args.u, args.g, args.F, args.fstar, args.fjump, args.favg, args.fsig, args.bs};
       return;
@@ -43,6 +47,13 @@ struct K1 {
   // y = A x (apply_sip!, elliptic.jl:32-116) in gather form.  x must be
   // complete (barrier) before the call; y is complete when it returns.
   __device__ __forceinline__ void apply(const double* x, double* y) {
+    apply_with(x, [&](int n, double v) { y[n] = v; });
+  }
+  // The same apply with epi(n, (A x)_n) run by the thread that owns node n
+  // in place of the store.  The second phase reads x at the own node only, so
+  // epi may overwrite x[n].
+  template <class Epi>
+  __device__ __forceinline__ void apply_with(const double* x, Epi epi) {
     const double al = a.alpha, invJ = a.invJ;
     for (int n = tid; n < N; n += nt) {
       const int i = n % NP, e = n / NP;
@@ -75,7 +86,7 @@ struct K1 {
       v += -al * (w.invR[e * NP + NP - 1] / 2.0) * invJ * D(NP - 1, i) * w.fjump[eR];
       if (i == 0) v += al * w.favg[e] - al * w.fsig[e] * w.fjump[e];
       if (i == NP - 1) v += -al * w.favg[eR] + al * w.fsig[eR] * w.fjump[eR];
-      y[n] = v;
+      epi(n, v);
     }
     __syncthreads();
   }
@@ -170,13 +181,12 @@ struct K1 {
       }
       if (nI == 0 && tid == 0 && a.stsc) a.stsc[0] = rz;
     } else if (a.solver == 1) {
+      // damped Jacobi: the update is the apply's epilogue (two barriers per iteration)
       for (int it = 0; it < nI; ++it) {
-        apply(x, w.Ap);
-        for (int n = tid; n < N; n += nt) {
-          if (a.stP) { a.stP[(size_t)it * N + n] = x[n]; a.stAP[(size_t)it * N + n] = w.Ap[n]; }
-          x[n] = x[n] + (2.0 / 3.0) * (bb[n] - w.Ap[n]) * w.invM[n];
-        }
-        __syncthreads();
+        apply_with(x, [&](int n, double v) {
+          if (a.stP) { a.stP[(size_t)it * N + n] = x[n]; a.stAP[(size_t)it * N + n] = v; }
+          x[n] = x[n] + (2.0 / 3.0) * (bb[n] - v) * w.invM[n];
+        });
       }
     } else {
       const double lmin = 1.0, lmax = a.cheb_lmax;
@@ -307,8 +317,8 @@ struct K1 {
 };
 
 #pragma clang fp contract(off)
-template <int NP>
-__device__ __forceinline__ void K1<NP>::hyp_stage(const double* qs, int mode, double cdt, double* out) {
+template <int NP, bool LDS>
+__device__ __forceinline__ void K1<NP, LDS>::hyp_stage(const double* qs, int mode, double cdt, double* out) {
   const double g = a.gamma, invJ = a.invJ;
   for (int n = tid; n < N; n += nt) {
     double f[3];
@@ -358,8 +368,8 @@ __device__ __forceinline__ void K1<NP>::hyp_stage(const double* qs, int mode, do
   }
   __syncthreads();
 }
-template <int NP>
-__device__ __forceinline__ void K1<NP>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
+template <int NP, bool LDS>
+__device__ __forceinline__ void K1<NP, LDS>::sens_hyp_stage(const double* qs, const double* ss, const double* svs, const double* sn,
                                        int mode, double cdt, double* out) {
   const double g = a.gamma, invJ = a.invJ;
   for (int n = tid; n < N; n += nt) {
