@@ -159,6 +159,30 @@ def c1_leg(igr, configs, cpu_steps=200):
     return out
 
 
+def size_sweep_leg(igr, configs, sizes=(128, 256, 512, 1024), reps=100):
+    """Per-launch time and algorithmic bandwidth of the three forward kernels
+    on the headline workload's data at growing mesh sizes.  At 128^2 a launch
+    moves about one latency-bandwidth product of data (12.6 MB), so the time
+    is mostly latency; larger meshes show the streaming rate the same
+    kernels reach.  Algorithmic bytes per node: PCG_A 6 fields (read r, 1/M,
+    p_old, 1/rho; write p, Ap), PCG_B 7 (read x, p, r, Ap, 1/M; write x, r),
+    hyperbolic stage 2 13 (read q_stage, Sigma, q^n; write 4)."""
+    rows = []
+    for N in sizes:
+        cfg = configs.config2(N=N)
+        ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(3)))
+        ctx.set_state(cfg["q"])
+        n_nodes = 16 * N * N
+        row = {"mesh": f"{N}x{N}", "nodes": n_nodes}
+        for name, kern, fields in (("pcg_a", 0, 6), ("pcg_b", 1, 7), ("hyp", 2, 13)):
+            ms = ctx.time_kernel(kern, cfg["gamma"], cfg["alpha"], reps)
+            row[name + "_us"] = ms * 1e3
+            row[name + "_gbs"] = fields * 8 * n_nodes / (ms * 1e-3) / 1e9
+        ctx.close()
+        rows.append(row)
+    return rows
+
+
 def adjoint_pde_leg(igr, configs, cfg, steps, prm, qbar, t_forward):
     """Continuous adjoint of the windowed-density objective on the headline
     workload: run_forward_store (a Sigma solve after every step, snapshots in
@@ -197,6 +221,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=8, help="SSP-RK3 steps of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the kernel size sweep")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 256x256 blast-vortex forward + gradient leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1D blast run")
@@ -266,6 +291,7 @@ def main():
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)
     blast1 = None if args.no_c1 else c1_leg(igr, configs)
+    sweep = None if args.no_sweep else size_sweep_leg(igr, configs)
 
     cpu = None
     if not args.no_cpu:
@@ -330,6 +356,7 @@ def main():
         "ensemble": ens,
         "blast_vortex_256": c5,
         "blast_1d": blast1,
+        "kernel_size_sweep": sweep,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -51,8 +51,9 @@ struct K1 {
   }
   // The same apply with epi(n, (A x)_n) run by the thread that owns node n
   // in place of the store.  The second phase reads x at the own node only, so
-  // epi may overwrite x[n].
-  template <class Epi>
+  // epi may overwrite x[n].  END_SYNC = false leaves out the closing barrier
+  // for a caller whose next step is a barrier anyway.
+  template <bool END_SYNC = true, class Epi>
   __device__ __forceinline__ void apply_with(const double* x, Epi epi) {
     const double al = a.alpha, invJ = a.invJ;
     for (int n = tid; n < N; n += nt) {
@@ -88,7 +89,21 @@ struct K1 {
       if (i == NP - 1) v += -al * w.favg[eR] + al * w.fsig[eR] * w.fjump[eR];
       epi(n, v);
     }
+    if (END_SYNC) __syncthreads();
+  }
+
+  // Workgroup total of one value per thread, returned to every thread, in
+  // the order of block_sum (butterfly within each wave, then the wave sums in
+  // wave order), so it equals dot() bit for bit.  One barrier; `slot` (LDS,
+  // MAX_WAVES doubles) must not be written again before the next barrier.
+  __device__ __forceinline__ double block_total(double v, double* slot) {
+    v = wave_sum(v);
+    const int nw = (nt + WAVE - 1) / WAVE;
+    if ((tid & (WAVE - 1)) == 0) slot[tid / WAVE] = v;
     __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < nw; ++q) t += slot[q];
+    return t;
   }
 
   __device__ __forceinline__ double dot(const double* u, const double* v) {
@@ -147,28 +162,41 @@ struct K1 {
   __device__ __forceinline__ void iterate(double* x, const double* bb, int trace) {
     const int nI = a.n_iter;
     if (a.solver == 0) {
+      // Four barriers per iteration: <p, Ap> is summed in the apply's
+      // epilogue and <r, z> in the update, each with one block_total (two
+      // slot arrays, so a total is never overwritten while being read).
+      __shared__ double slotA[MAX_WAVES], slotB[MAX_WAVES];
       apply(x, w.Ap);
+      double part = 0.0;
       for (int n = tid; n < N; n += nt) {
         const double r = bb[n] - w.Ap[n];
+        const double z = r * w.invM[n];
         w.r[n] = r;
-        w.p[n] = r * w.invM[n];
+        w.p[n] = z;
+        part += r * z;
         if (a.stR) a.stR[n] = r;
       }
-      __syncthreads();
-      double rz = dot(w.r, w.p);
+      double rz = block_total(part, slotB);
       for (int it = 0; it < nI; ++it) {
-        apply(w.p, w.Ap);
-        const double pAp = dot(w.p, w.Ap);
+        double pp = 0.0;
+        apply_with<false>(w.p, [&](int n, double v) {
+          w.Ap[n] = v;
+          pp += w.p[n] * v;
+        });
+        const double pAp = block_total(pp, slotA);
         const double ak = rz / (pAp + 1e-30);
+        double rr = 0.0;
         for (int n = tid; n < N; n += nt) {
           if (a.stP) { a.stP[(size_t)it * N + n] = w.p[n]; a.stAP[(size_t)it * N + n] = w.Ap[n]; }
           x[n] = x[n] + ak * w.p[n];
-          w.r[n] = w.r[n] - ak * w.Ap[n];
-          w.z[n] = w.r[n] * w.invM[n];
-          if (a.stR) a.stR[(size_t)(it + 1) * N + n] = w.r[n];
+          const double r = w.r[n] - ak * w.Ap[n];
+          const double z = r * w.invM[n];
+          w.r[n] = r;
+          w.z[n] = z;
+          rr += r * z;
+          if (a.stR) a.stR[(size_t)(it + 1) * N + n] = r;
         }
-        __syncthreads();
-        const double rzn = dot(w.r, w.z);
+        const double rzn = block_total(rr, slotB);
         const double bk = rzn / (rz + 1e-30);
         if (trace && tid == 0 && a.trace) { a.trace[it] = ak; a.trace[nI + it] = bk; }
         if (tid == 0 && a.stsc) {   // rz_k (k <= n) | pAp_k | a_k | beta_k
@@ -192,12 +220,11 @@ struct K1 {
       const double lmin = 1.0, lmax = a.cheb_lmax;
       const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
       const double sigma = theta / fmax(delta, 1e-10);
-      apply(x, w.Ap);
-      for (int n = tid; n < N; n += nt) {
-        w.r[n] = bb[n] - w.Ap[n];
+      // r = b - A x as the apply's epilogue (three barriers per iteration)
+      apply_with(x, [&](int n, double v) {
+        w.r[n] = bb[n] - v;
         if (a.stR) a.stR[n] = w.r[n];
-      }
-      __syncthreads();
+      });
       double rc = 1.0 / sigma;
       for (int k = 1; k <= nI; ++k) {
         double c1 = 0.0, c2 = 0.0;
@@ -210,12 +237,10 @@ struct K1 {
           if (a.stP) a.stP[(size_t)(k - 1) * N + n] = x[n];
         }
         __syncthreads();
-        apply(x, w.Ap);
-        for (int n = tid; n < N; n += nt) {
-          w.r[n] = bb[n] - w.Ap[n];
+        apply_with(x, [&](int n, double v) {
+          w.r[n] = bb[n] - v;
           if (a.stR && k < nI) a.stR[(size_t)k * N + n] = w.r[n];
-        }
-        __syncthreads();
+        });
       }
     }
   }
