@@ -8,5 +8,5 @@ from .core import (DGBasis, CartesianMesh2D, PeriodicMesh1D, Context, DomainErro
                    SOLVERS, compute_hyperbolic_rhs, apply_sip_, compute_igr_rhs_, compute_sip_diagonal,
                    solve_elliptic_, compute_igr_euler_rhs_, run_forward_, run_forward,
                    enzyme_adjoint_ic, OBJECTIVES, run_sensitivity, run_sensitivity_batch,
-                   run_forward_store, run_adjoint_conservative, Snapshots, release_cache)
+                   run_forward_store, run_adjoint_conservative, Snapshots, release_cache, build_ic, IC_TYPES)
 from ._lib import lib, LIB_PATH  # noqa: F401

@@ -57,6 +57,7 @@ PROTOTYPES = {
     "igr_set_snapshots": (_i, [_vp, _i, _pd]),
     "igr_run_adjoint_conservative": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _d, _i, _i, _pd]),
     "igr_release_cache": (_i, []),
+    "igr_build_ic": (_i, [_vp, _i, _d, _d, _pd, _i, _pd]),
 }
 
 _lib = None

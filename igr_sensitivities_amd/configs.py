@@ -337,6 +337,47 @@ def taylor_green2(p, Nex, Ney, A=0.1, gamma=1.4, Lx=1.0, Ly=1.0):
     return q
 
 
+def constant2(p, Nex, Ney, gamma=1.4, rho0=1.0, ux0=1.0, uy0=1.0, p0=1.0, Lx=1.0, Ly=1.0):
+    """init_constant_state (IGR_adjoints_2d/initial_conditions.jl:18-25)."""
+    shape = (Ney, Nex, p + 1, p + 1)
+    q = np.empty((4,) + shape)
+    q[0], q[1], q[2] = rho0 * ux0, rho0 * uy0, rho0
+    q[3] = p0 / (gamma - 1) + 0.5 * rho0 * (ux0 ** 2 + uy0 ** 2)
+    return q
+
+
+def isentropic_vortex2(p, Nex, Ney, gamma=1.4, beta=5.0, ux0=1.0, uy0=1.0, Lx=1.0, Ly=1.0):
+    """init_isentropic_vortex (IGR_adjoints_2d/initial_conditions.jl:42-71)."""
+    x, y = coords2(p, Nex, Ney, Lx, Ly)
+    dx, dy = x - Lx / 2, y - Ly / 2
+    r2 = dx ** 2 + dy ** 2
+    ef = np.exp((1 - r2) / 2)
+    du = -dy * (beta / (2 * np.pi)) * ef
+    dv = dx * (beta / (2 * np.pi)) * ef
+    dT = -(gamma - 1) * beta ** 2 / (8 * gamma * np.pi ** 2) * np.exp(1 - r2)
+    rho = (1 + dT) ** (1 / (gamma - 1))
+    ux, uy = ux0 + du, uy0 + dv
+    pr = rho ** gamma
+    return np.stack([rho * ux, rho * uy, rho, pr / (gamma - 1) + 0.5 * rho * (ux ** 2 + uy ** 2)])
+
+
+def sedov_blast2(p, Nex, Ney, A, gamma=1.4, p_bg=1.0, sigma=0.1, Lx=1.0, Ly=1.0):
+    """build_sedov_blast_ic! (IGR_adjoints_2d/initial_conditions.jl:136-151)."""
+    x, y = coords2(p, Nex, Ney, Lx, Ly)
+    r2 = (x - Lx / 2) ** 2 + (y - Ly / 2) ** 2
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0
+    q[3] = (p_bg + A * np.exp(-r2 / sigma ** 2)) / (gamma - 1)
+    return q
+
+
+def entropy_wave_init1(p, Ne, gamma=1.4, L=1.0):
+    """init_entropy_wave (IGR_adjoints_1d/initial_conditions.jl:24-44)."""
+    x = coords1(p, Ne, L)
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * x / L)
+    return np.stack([rho, rho, 1.0 / (gamma - 1) + 0.5 * rho])
+
+
 def smooth_random2(p, Nex, Ney, seed, gamma=1.4, amp=0.1):
     """Smooth positive random state (few low modes) for parity tests."""
     rng = np.random.default_rng(seed)

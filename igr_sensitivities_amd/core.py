@@ -226,6 +226,19 @@ class Context:
         self.check(self._L.igr_objective(self.h, int(obj), _p(p), float(gamma), C.byref(J)))
         return J.value
 
+    def build_ic(self, ic, A=0.0, gamma=1.4, prm=None, dA=False):
+        """Initial condition built on the device into the context state (the
+        reference's IC builders: IGR_adjoints_2d/initial_conditions.jl,
+        IGR_adjoints_1d/initial_conditions.jl; names in IC_TYPES).  prm=None
+        takes the reference's keyword defaults.  With dA=True returns
+        d q0 / d A (conservative variables), the tangent seed of the
+        amplitude."""
+        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64).ravel()
+        dq = np.empty((self.nv,) + self.shape) if dA else None
+        self.check(self._L.igr_build_ic(self.h, _ic_code(ic), float(A), float(gamma), _p(p),
+                                        0 if p is None else int(p.size), _p(dq)))
+        return dq
+
     def gradient(self, obj, n_steps, dt, gamma, alpha, n_iter, solver="pcg", prm=None, ckpt_slots=0):
         """(J, dJ/dq0, dJ/dgamma, dJ/dalpha) of the objective after n_steps fixed steps from the state."""
         J, gb, ab = C.c_double(0), C.c_double(0), C.c_double(0)
@@ -499,6 +512,21 @@ class Snapshots:
         return out
 
 
+# IC names: the reference's IC_BUILDERS symbols and init_* functions
+IC_TYPES = {"constant": 0, "isentropic_vortex": 1, "taylor_green": 2, "sedov_blast": 3, "oseen_pair_blast": 4,
+            "init_entropy_wave": 5, "acoustic_pulse": 6, "entropy_wave": 7, "density_pulse": 8,
+            "pressure_pulse": 9, "velocity_pulse": 10}
+
+
+def _ic_code(ic):
+    key = ic.lstrip(":") if isinstance(ic, str) else ic
+    if isinstance(key, str):
+        if key not in IC_TYPES:
+            raise ValueError(f"Unknown IC type: {ic}")
+        key = IC_TYPES[key]
+    return int(key)
+
+
 def _objective_code(objective):
     key = objective.lstrip(":") if isinstance(objective, str) else objective
     if isinstance(key, str):
@@ -515,6 +543,17 @@ def run_forward_store(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="p
     c.set_state(q0)
     n = c.run_forward_store(dt, T, gamma, alpha, n_iter, solver)
     return Snapshots(c, n), n
+
+
+def build_ic(ic, A, basis, mesh, gamma, prm=None, dA=False):
+    """The reference's IC builders (build_<type>_ic!(..., A, mesh, gamma) and
+    the init_* functions; names in IC_TYPES) evaluated on the device: returns
+    the state stacked as (mx, my, rho, E) / (mu, rho, E), and with dA=True
+    also d q0 / d A."""
+    c = _ctx(mesh)
+    dq = c.build_ic(ic, A, gamma, prm, dA)
+    q, _ = c.get_state()
+    return (q, dq) if dA else q
 
 
 def run_adjoint_conservative(snapshots, n_steps, dt, T, basis, mesh, gamma, alpha, n_iter=10,

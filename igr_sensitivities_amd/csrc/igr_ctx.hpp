@@ -59,6 +59,7 @@ struct Ctx {
   unsigned* counter = nullptr;
   int* flag = nullptr;        // domain flag
   double* dscalar = nullptr;  // small device scalar scratch
+  double* ic_tab = nullptr;   // enthalpy-defect table of the Oseen-pair initial condition
   std::vector<void*> owned;   // every hipMalloc, freed by igr_destroy
 
   // 1D whole-instance kernel workspace

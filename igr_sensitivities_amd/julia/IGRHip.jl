@@ -61,6 +61,29 @@ function get_state!(c, q, Σ)
     check(c, ccall((:igr_get_state, libigr), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}), c.h, q, Σ))
 end
 
+# The reference's IC builders (IC_BUILDERS, build_oseen_pair_blast_ic!, the
+# init_* functions; 2D/initial_conditions.jl, 1D/initial_conditions.jl)
+# evaluated on the GPU.  build_ic!(c, (μx0, μy0, ρ0, E0), :taylor_green, A, γ)
+# fills the caller's arrays as builder!(μx0, μy0, ρ0, E0, A, mesh, γ) does
+# and leaves the state in the context; prm overrides the keyword defaults in
+# the order of include/igr.h.  dA: optional arrays receiving ∂q0/∂A.
+const IC_TYPES = Dict(:constant => 0, :isentropic_vortex => 1, :taylor_green => 2, :sedov_blast => 3,
+                      :oseen_pair_blast => 4, :init_entropy_wave => 5, :acoustic_pulse => 6, :entropy_wave => 7,
+                      :density_pulse => 8, :pressure_pulse => 9, :velocity_pulse => 10)
+function build_ic!(c::Context, fields::Tuple, ic_type::Symbol, A, γ; prm=Float64[], dA=nothing)
+    haskey(IC_TYPES, ic_type) || error("Unknown IC type: $ic_type")
+    p = Vector{Float64}(prm)
+    d = dA === nothing ? C_NULL : Vector{Float64}(undef, c.nv * c.N)
+    check(c, ccall((:igr_build_ic, libigr), Cint,
+                   (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Ptr{Cdouble}, Cint, Ptr{Cdouble}),
+                   c.h, IC_TYPES[ic_type], A, γ, isempty(p) ? C_NULL : p, length(p), d))
+    q = Vector{Float64}(undef, c.nv * c.N)
+    get_state!(c, q, C_NULL)
+    unpack!(q, fields...)
+    dA === nothing || unpack!(d, dA...)
+    return nothing
+end
+
 # run_forward!(μx, μy, ρ, E, Σ, Δt, T, basis, mesh, γ, α, n_iter, solver) (2D/forward.jl:160)
 function run_forward!(c::Context, fields_and_Σ, Δt, T, γ, α, n_iter::Int, solver::Symbol=:pcg)
     fields = fields_and_Σ[1:end-1]; Σ = fields_and_Σ[end]

@@ -167,6 +167,34 @@ enum igr_objective {
 /* Objective value on the context state (2D). */
 int igr_objective(igr_ctx* ctx, int objective, const double* prm, double gamma, double* J);
 
+/* Initial conditions, built on the device straight into the context state
+ * (Sigma is left as it is).  Each replaces one of the reference's builders,
+ * evaluated at the mesh nodes with the same expressions:
+ *   2D, IGR_adjoints_2d/initial_conditions.jl:
+ *     IGR_IC_CONSTANT           init_constant_state (:18-25)     prm = (rho0, ux0, uy0, p0)
+ *     IGR_IC_ISENTROPIC_VORTEX  init_isentropic_vortex (:42-71)  prm = (beta, ux0, uy0)
+ *     IGR_IC_TAYLOR_GREEN       build_taylor_green_ic! (:105-121), amplitude A
+ *     IGR_IC_SEDOV_BLAST        build_sedov_blast_ic! (:136-151)  A; prm = (p_bg, sigma)
+ *     IGR_IC_OSEEN_PAIR_BLAST   build_oseen_pair_blast_ic! (:175-232)  A; prm = (a_core,
+ *                               M_star, blast_xc, blast_yc, sigma, then xc, yc, sign per
+ *                               vortex); its enthalpy-defect table is built on the device
+ *   1D, IGR_adjoints_1d/initial_conditions.jl:
+ *     IGR_IC_CONSTANT           init_constant_state (:52-60)     prm = (rho0, u0, p0)
+ *     IGR_IC_ENTROPY_WAVE_INIT  init_entropy_wave (:24-44)
+ *     IGR_IC_ACOUSTIC_PULSE, IGR_IC_ENTROPY_WAVE, IGR_IC_DENSITY_PULSE,
+ *     IGR_IC_PRESSURE_PULSE, IGR_IC_VELOCITY_PULSE   build_<kind>_ic! (:94-210), A
+ * n_prm = 0 takes the reference's keyword defaults.  dq_out (host, nv*N,
+ * optional) receives d q0 / d A in conservative variables: the tangent seed
+ * of the amplitude (ic_dA_weights, 2D :246-258, 1D :227-244).  An IC of the
+ * other dimension or an unknown code is IGR_EINVAL (the reference's
+ * "Unknown IC type" error). */
+enum igr_ic {
+  IGR_IC_CONSTANT = 0, IGR_IC_ISENTROPIC_VORTEX = 1, IGR_IC_TAYLOR_GREEN = 2, IGR_IC_SEDOV_BLAST = 3,
+  IGR_IC_OSEEN_PAIR_BLAST = 4, IGR_IC_ENTROPY_WAVE_INIT = 5, IGR_IC_ACOUSTIC_PULSE = 6, IGR_IC_ENTROPY_WAVE = 7,
+  IGR_IC_DENSITY_PULSE = 8, IGR_IC_PRESSURE_PULSE = 9, IGR_IC_VELOCITY_PULSE = 10
+};
+int igr_build_ic(igr_ctx* ctx, int ic, double A, double gamma, const double* prm, int n_prm, double* dq_out);
+
 /* Discrete adjoint: enzyme_adjoint_ic(q0..., n_steps, dt, basis, mesh,
  * gamma, alpha; n_iter, solver, objective) (IGR_adjoints_2d/enzyme_adjoint.jl:
  * 270-296; IGR_adjoints_1d/enzyme_adjoint.jl:255-314) extended to gamma and
