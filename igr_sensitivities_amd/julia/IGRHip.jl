@@ -237,14 +237,200 @@ set_option!(c::Context, opt, value) =
     check(c, ccall((:igr_set_option, libigr), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, opt, value))
 autotune!(c::Context) = set_option!(c, OPT_AUTOTUNE, 0)
 
-# ChainRulesCore rrule for the loss J(q0) (ChainRulesCore is not a dependency of
-# the reference; load it in the caller's environment to enable this rule):
-#
-#   function ChainRulesCore.rrule(::typeof(igr_loss), c, μx0, μy0, ρ0, E0, n_steps, Δt, γ, α; kw...)
-#       J, dμx, dμy, dρ, dE, dγ, dα = enzyme_adjoint_ic(c, μx0, μy0, ρ0, E0, n_steps, Δt, γ, α; kw...)
-#       pullback(J̄) = (NoTangent(), NoTangent(), J̄ .* dμx, J̄ .* dμy, J̄ .* dρ, J̄ .* dE,
-#                      NoTangent(), NoTangent(), J̄ * dγ, J̄ * dα)
-#       return J, pullback
-#   end
+# ---------------------------------------------------------------------------
+# The reference's own signatures (basis and mesh arguments, no context), so a
+# paper script's hot-path calls run unchanged on the GPU path: take these
+# names from IGRHip.D2 / IGRHip.D1 instead of IGRAdjoints2D / IGRAdjoints1D
+# (types, meshes and plotting still come from the reference).  One context
+# per mesh is created on first use and kept; results come back shaped like
+# the inputs.  The ChainRulesCore rule for D2.loss is in IGRHipChainRules.jl.
+const _CTX = Dict{Any, Context}()
+ctx_for(basis, mesh) = get!(() -> Context(basis, mesh), _CTX, objectid(mesh))
+ctx_key(key, make) = get!(make, _CTX, key)
+shaped(c::Context, q, like) = ntuple(v -> reshape(q[(v - 1) * c.N + 1:v * c.N], size(like)), c.nv)
+
+function run_steps_objective(c::Context, q0, n_steps, Δt, γ, α, n_iter, solver, objective)
+    haskey(OBJECTIVES, objective) || error("Unknown objective: $objective")
+    set_state!(c, q0)
+    check(c, ccall((:igr_run_steps, libigr), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint, Cint),
+                   c.h, n_steps, Δt, γ, α, max(n_iter, 1), solver_code(solver)))
+    J = Ref{Cdouble}(0)
+    check(c, ccall((:igr_objective, libigr), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cdouble, Ptr{Cdouble}),
+                   c.h, OBJECTIVES[objective], C_NULL, γ, J))
+    return J[]
+end
+
+module D2   # IGRAdjoints2D's hot-path names
+import ..IGRHip
+import ..IGRHip: Context, Snapshots, ctx_for, ctx_key, shaped, pack, unpack!, set_state!, get_state!
+export run_forward!, run_forward, run_forward_store, run_adjoint_conservative, enzyme_adjoint_ic, run_sensitivity,
+       loss, init_constant_state, init_isentropic_vortex, build_taylor_green_ic!, build_sedov_blast_ic!,
+       build_oseen_pair_blast_ic!, IC_BUILDERS
+
+ctx_mesh(mesh) = ctx_key(objectid(mesh), () -> Context(2, size(mesh.x, 1) - 1, mesh.N_ex, mesh.N_ey, mesh.Lx, mesh.Ly))
+
+# forward.jl:41
+run_forward!(μx, μy, ρ, E, Σ, Δt, T, basis, mesh, γ, α, n_iter::Int, solver::Symbol=:pcg) =
+    IGRHip.run_forward!(ctx_for(basis, mesh), (μx, μy, ρ, E, Σ), Δt, T, γ, α, n_iter, solver)
+# forward.jl:84, 91
+function run_forward(μx0, μy0, ρ0, E0, Δt, T, basis, mesh, γ, α, n_iter::Int, solver::Symbol=:pcg)
+    μx, μy, ρ, E = copy(μx0), copy(μy0), copy(ρ0), copy(E0)
+    Σ = zeros(size(μx))
+    run_forward!(μx, μy, ρ, E, Σ, Δt, T, basis, mesh, γ, α, n_iter, solver)
+    return μx, μy, ρ, E, Σ
+end
+run_forward(μx0, μy0, ρ0, E0, Δt, T, basis, mesh, γ, α; n_iter=10, solver=:pcg) =
+    run_forward(μx0, μy0, ρ0, E0, Δt, T, basis, mesh, γ, α, Int(n_iter), solver)
+# forward.jl:111 -> (snapshots, n_steps); the records stay in HBM
+run_forward_store(μx0, μy0, ρ0, E0, Δt, T, basis, mesh, γ, α; n_iter=10, solver=:pcg) =
+    IGRHip.run_forward_store(ctx_for(basis, mesh), μx0, μy0, ρ0, E0, Δt, T, γ, α; n_iter=Int(n_iter), solver=solver)
+# adjoint_pde.jl:522 -> (aμx, aμy, aρ, aE)
+function run_adjoint_conservative(snapshots::Snapshots, n_steps, Δt, T, basis, mesh, γ, α;
+                                  n_iter=10, objective=:l2_density, solver=:pcg)
+    a = IGRHip.run_adjoint_conservative(snapshots, n_steps, Δt, T, γ, α; n_iter=Int(n_iter), objective=objective,
+                                        solver=solver)
+    return ntuple(v -> reshape(a[v], size(mesh.x)), 4)
+end
+# enzyme_adjoint.jl:270 -> (J, dμx0, dμy0, dρ0, dE0)
+function enzyme_adjoint_ic(μx0, μy0, ρ0, E0, n_steps, Δt, basis, mesh, γ, α;
+                           n_iter::Int=10, solver::Symbol=:pcg, objective::Symbol=:l2_density)
+    r = IGRHip.enzyme_adjoint_ic(ctx_for(basis, mesh), μx0, μy0, ρ0, E0, n_steps, Δt, γ, α;
+                                 n_iter=n_iter, solver=solver, objective=objective)
+    return r[1:5]
+end
+# J of the Enzyme objectives (enzyme_adjoint.jl:50-76): the primal of the ChainRulesCore rule
+loss(μx0, μy0, ρ0, E0, n_steps, Δt, basis, mesh, γ, α; n_iter::Int=10, solver::Symbol=:pcg,
+     objective::Symbol=:l2_density) =
+    IGRHip.run_steps_objective(ctx_for(basis, mesh), pack(μx0, μy0, ρ0, E0), n_steps, Δt, γ, α, n_iter, solver,
+                               objective)
+# sensitivity.jl:353 -> (μx, μy, ρ, E, Σ, ζx, ζy, σ, η, ς)
+function run_sensitivity(μx0, μy0, ρ0, E0, ζx0, ζy0, σ0, η0, Δt, T, basis, mesh, γ, α; n_iter=10, solver=:pcg)
+    c = ctx_for(basis, mesh)
+    q, Σ, s, ς = IGRHip.run_sensitivity(c, pack(μx0, μy0, ρ0, E0), pack(ζx0, ζy0, σ0, η0), Δt, T, γ, α;
+                                        n_iter=Int(n_iter), solver=solver)
+    sz = size(μx0)
+    return (shaped(c, q, μx0)..., reshape(Σ, sz), shaped(c, s, μx0)..., reshape(ς, sz))
+end
+
+# initial_conditions.jl:18-232, built on the GPU
+function init_constant_state(mesh, γ; ρ0=1.0, ux0=1.0, uy0=1.0, p0=1.0)
+    f = ntuple(_ -> zeros(size(mesh.x)), 4)
+    IGRHip.build_ic!(ctx_mesh(mesh), f, :constant, 0.0, γ; prm=[ρ0, ux0, uy0, p0])
+    return f
+end
+function init_isentropic_vortex(mesh, γ; β=5.0, ux0=1.0, uy0=1.0)
+    f = ntuple(_ -> zeros(size(mesh.x)), 4)
+    IGRHip.build_ic!(ctx_mesh(mesh), f, :isentropic_vortex, 0.0, γ; prm=[β, ux0, uy0])
+    return f
+end
+build_taylor_green_ic!(μx0, μy0, ρ0, E0, A, mesh, γ) =
+    IGRHip.build_ic!(ctx_mesh(mesh), (μx0, μy0, ρ0, E0), :taylor_green, A, γ)
+build_sedov_blast_ic!(μx0, μy0, ρ0, E0, A, mesh, γ; p_bg=1.0, σ=0.1) =
+    IGRHip.build_ic!(ctx_mesh(mesh), (μx0, μy0, ρ0, E0), :sedov_blast, A, γ; prm=[p_bg, σ])
+function build_oseen_pair_blast_ic!(μx0, μy0, ρ0, E0, A, mesh, γ; a_core=0.05, M_star=0.5,
+                                    vort_centers=((0.45, 0.60, +1), (0.45, 0.40, -1)),
+                                    blast_xc=0.25, blast_yc=0.50, σ=0.09)
+    prm = Float64[a_core, M_star, blast_xc, blast_yc, σ]
+    for (xc, yc, sgn) in vort_centers
+        append!(prm, (xc, yc, sgn))
+    end
+    IGRHip.build_ic!(ctx_mesh(mesh), (μx0, μy0, ρ0, E0), :oseen_pair_blast, A, γ; prm=prm)
+end
+const IC_BUILDERS = Dict{Symbol, Tuple{Function, String}}(
+    :taylor_green => (build_taylor_green_ic!, "ux=A sin cos, uy=-A cos sin, ρ=1, p=1"),
+    :sedov_blast  => (build_sedov_blast_ic!,  "ρ=1, u=0, p=p_bg+A exp(-r²/σ²)"),
+)
+end # module D2
+
+module D1   # IGRAdjoints1D's hot-path names
+import ..IGRHip
+import ..IGRHip: Context, Snapshots, ctx_for, ctx_key, shaped, pack, unpack!, set_state!, get_state!
+export run_forward!, run_forward, run_forward_store, run_adjoint_conservative, enzyme_adjoint_ic, run_sensitivity,
+       init_constant_state, init_entropy_wave, build_acoustic_pulse_ic!, build_entropy_wave_ic!,
+       build_density_pulse_ic!, build_pressure_pulse_ic!, build_velocity_pulse_ic!, IC_BUILDERS
+
+ctx_x(mesh_x, L) = ctx_key((size(mesh_x), L), () -> Context(1, size(mesh_x, 1) - 1, size(mesh_x, 2), 1, L, 1.0))
+
+# forward.jl:50
+run_forward!(μ, ρ, E, Σ, Δt, T, basis, mesh, γ, α, n_iter::Int, solver::Symbol=:pcg) =
+    IGRHip.run_forward!(ctx_for(basis, mesh), (μ, ρ, E, Σ), Δt, T, γ, α, n_iter, solver)
+# forward.jl:97, 110
+function run_forward(μ0, ρ0, E0, Δt, T, basis, mesh, γ, α, n_iter::Int, solver::Symbol=:pcg)
+    μ, ρ, E = copy(μ0), copy(ρ0), copy(E0)
+    Σ = zeros(size(μ))
+    run_forward!(μ, ρ, E, Σ, Δt, T, basis, mesh, γ, α, n_iter, solver)
+    return μ, ρ, E, Σ
+end
+run_forward(μ0, ρ0, E0, Δt, T, basis, mesh, γ, α; n_iter=10, solver=:pcg) =
+    run_forward(μ0, ρ0, E0, Δt, T, basis, mesh, γ, α, Int(n_iter), solver)
+# forward.jl:137
+run_forward_store(μ0, ρ0, E0, Δt, T, basis, mesh, γ, α; n_iter=10, solver=:pcg) =
+    IGRHip.run_forward_store(ctx_for(basis, mesh), μ0, ρ0, E0, Δt, T, γ, α; n_iter=Int(n_iter), solver=solver)
+# adjoint_pde.jl:466 -> (aμ, aρ, aE)
+function run_adjoint_conservative(snapshots::Snapshots, n_steps, Δt, T, basis, mesh, γ, α;
+                                  n_iter=10, objective=:l2_density, solver=:pcg)
+    a = IGRHip.run_adjoint_conservative(snapshots, n_steps, Δt, T, γ, α; n_iter=Int(n_iter), objective=objective,
+                                        solver=solver)
+    return ntuple(v -> reshape(a[v], size(mesh.x)), 3)
+end
+# enzyme_adjoint.jl:267: the IC family at amplitude A on its own mesh, the
+# fixed-step objective's gradient, mapped to primitive adjoints (:306-311)
+function enzyme_adjoint_ic(A, p, N_e, L, γ, α, CFL, T; n_iter::Int=10, ic_type::Symbol=:acoustic_pulse,
+                           solver::Symbol=:pcg, objective::Symbol=:l2_density)
+    objective in (:l2_density, :l2_pressure) ||
+        error("Unknown objective: $objective (use :l2_density or :l2_pressure)")
+    c = ctx_key((:enzyme1d, p, N_e, L), () -> Context(1, p, N_e, 1, L, 1.0))
+    n_p = p + 1
+    Δx = L / N_e
+    max_ws_est = 2.5
+    Δt = CFL * Δx / ((2 * p + 1) * max_ws_est)
+    n_steps = ceil(Int, T / Δt)
+    Δt = T / n_steps
+    μ0 = zeros(n_p, N_e); ρ0 = zeros(n_p, N_e); E0 = zeros(n_p, N_e)
+    IGRHip.build_ic!(c, (μ0, ρ0, E0), ic_type, A, γ)
+    r = IGRHip.enzyme_adjoint_ic_1d(c, μ0, ρ0, E0, n_steps, Δt, γ, α; n_iter=n_iter, solver=solver,
+                                    objective=objective)
+    return r[1], reshape(r[2], n_p, N_e), reshape(r[3], n_p, N_e), reshape(r[4], n_p, N_e)
+end
+# sensitivity.jl:475 -> (μf, ρf, Ef, Σ, sμf, sρf, sEf, sΣ)
+function run_sensitivity(μ0, ρ0, E0, sμ0, sρ0, sE0, Δt, T, basis, mesh, γ, α;
+                         n_iter=10, elliptic_rhs=:form2, solver=:pcg, sγ=0.0, callback=nothing)
+    callback === nothing || error("run_sensitivity: callback is not supported on the GPU path")
+    c = ctx_for(basis, mesh)
+    q, Σ, s, ς = IGRHip.run_sensitivity(c, pack(μ0, ρ0, E0), pack(sμ0, sρ0, sE0), Δt, T, γ, α;
+                                        n_iter=Int(n_iter), solver=solver, elliptic_rhs=elliptic_rhs, sγ=sγ)
+    sz = size(μ0)
+    return (shaped(c, q, μ0)..., reshape(Σ, sz), shaped(c, s, μ0)..., reshape(ς, sz))
+end
+
+# initial_conditions.jl:24-210, built on the GPU
+function init_entropy_wave(mesh, γ)
+    f = ntuple(_ -> zeros(size(mesh.x)), 3)
+    IGRHip.build_ic!(ctx_x(mesh.x, mesh.L), f, :init_entropy_wave, 0.0, γ)
+    return f
+end
+function init_constant_state(mesh, γ; ρ0=1.0, u0=1.0, p0=1.0)
+    f = ntuple(_ -> zeros(size(mesh.x)), 3)
+    IGRHip.build_ic!(ctx_x(mesh.x, mesh.L), f, :constant, 0.0, γ; prm=[ρ0, u0, p0])
+    return f
+end
+build_acoustic_pulse_ic!(μ0, ρ0, E0, A, mesh_x, L, γ) =
+    IGRHip.build_ic!(ctx_x(mesh_x, L), (μ0, ρ0, E0), :acoustic_pulse, A, γ)
+build_entropy_wave_ic!(μ0, ρ0, E0, A, mesh_x, L, γ) =
+    IGRHip.build_ic!(ctx_x(mesh_x, L), (μ0, ρ0, E0), :entropy_wave, A, γ)
+build_density_pulse_ic!(μ0, ρ0, E0, A, mesh_x, L, γ) =
+    IGRHip.build_ic!(ctx_x(mesh_x, L), (μ0, ρ0, E0), :density_pulse, A, γ)
+build_pressure_pulse_ic!(μ0, ρ0, E0, A, mesh_x, L, γ) =
+    IGRHip.build_ic!(ctx_x(mesh_x, L), (μ0, ρ0, E0), :pressure_pulse, A, γ)
+build_velocity_pulse_ic!(μ0, ρ0, E0, A, mesh_x, L, γ) =
+    IGRHip.build_ic!(ctx_x(mesh_x, L), (μ0, ρ0, E0), :velocity_pulse, A, γ)
+const IC_BUILDERS = Dict{Symbol, Tuple{Function, String}}(
+    :acoustic_pulse => (build_acoustic_pulse_ic!, "ρ=1+A sin, u=A sin, p=1+A sin"),
+    :entropy_wave   => (build_entropy_wave_ic!,   "ρ=1+A sin, u=1, p=1"),
+    :density_pulse  => (build_density_pulse_ic!,  "ρ=1+A sin, u=0, p=1"),
+    :pressure_pulse => (build_pressure_pulse_ic!, "ρ=1, u=0, p=1+A sin"),
+    :velocity_pulse => (build_velocity_pulse_ic!, "ρ=1, u=A sin, p=1"),
+)
+end # module D1
 
 end # module
