@@ -218,6 +218,22 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       }
     }
   }
+  // own-node inputs of the outputs, loaded here so their latency overlaps
+  // the tile loads instead of following the LDS phases
+  double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0;
+  if constexpr (MODE == RV_PCG_ITER) {
+    if (valid) {
+      f_irb = a.irbar[gme];
+      f_imb = a.imbar[gme];
+      f_xb = a.xbar[gme];
+      f_apk = a.AP[(size_t)k * N + gme];
+      if (k >= 1) {
+        f_pm = a.P[(size_t)(k - 1) * N + gme];
+        f_apm = a.AP[(size_t)(k - 1) * N + gme];
+        f_Rk = a.R[(size_t)k * N + gme];
+      }
+    }
+  }
   if constexpr (MODE == RV_PCG_ITER) {
     double d[4];
     grid_total<4>(a.part_in, gridDim.x, d, red);
@@ -430,19 +446,19 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   } else if constexpr (MODE == RV_PCG_ITER) {
     double dv[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
-      a.irbar[gme] += irb;
-      a.imbar[gme] += o_zb * o_R;
+      a.irbar[gme] = f_irb + irb;
+      a.imbar[gme] = f_imb + o_zb * o_R;
       a.rbar_out[gme] = o_rb;
-      const double xb = a.xbar[gme];
-      const double pbk = bk * o_pb + ak * xb + papb * a.AP[(size_t)k * N + gme] + y;
+      const double xb = f_xb;
+      const double pbk = bk * o_pb + ak * xb + papb * f_apk + y;
       a.pbar_out[gme] = pbk;
       if (k >= 1) {
-        const double pm = a.P[(size_t)(k - 1) * N + gme], apm = a.AP[(size_t)(k - 1) * N + gme];
-        const double iM = a.invM[gme];
+        const double pm = f_pm, apm = f_apm;
+        const double iM = l_iM[0];      // own node: slot tid of the tile loads
         dv[0] = pbk * pm;
         dv[1] = xb * pm;
         dv[2] = (o_rb + pbk * iM) * apm;
-        dv[3] = (a.R[(size_t)k * N + gme] * iM) * apm;
+        dv[3] = (f_Rk * iM) * apm;
       }
     }
     double rv[1] = {ab};

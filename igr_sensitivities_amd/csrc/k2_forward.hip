@@ -391,10 +391,15 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
   double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0};
+  double qn[4] = {0.0, 0.0, 0.0, 0.0};   // q^n of stages 2 and 3, loaded with the stage state
   if (valid) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) qo[v] = a.q[v * N + gme];
     qo[4] = a.S[gme];
+    if (a.mode >= 2) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qn[v] = a.qn[v * N + gme];
+    }
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
@@ -531,8 +536,8 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
     double o;
     if (a.mode == 0) o = acc[v];
     else if (a.mode == 1) o = qo[v] + a.cdt * acc[v];
-    else if (a.mode == 2) o = 3.0 / 4.0 * a.qn[gi] + 1.0 / 4.0 * qo[v] + a.cdt * acc[v];
-    else o = 1.0 / 3.0 * a.qn[gi] + 2.0 / 3.0 * qo[v] + a.cdt * acc[v];
+    else if (a.mode == 2) o = 3.0 / 4.0 * qn[v] + 1.0 / 4.0 * qo[v] + a.cdt * acc[v];
+    else o = 1.0 / 3.0 * qn[v] + 2.0 / 3.0 * qo[v] + a.cdt * acc[v];
     a.out[gi] = o;
   }
 }
