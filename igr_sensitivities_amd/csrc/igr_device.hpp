@@ -83,11 +83,13 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* red, double* p
 
 constexpr int MAX_WAVES = 16;   // waves of a 1024-thread workgroup
 
-// Store this workgroup's NV partial sums at part[k*nb + bid] (no atomics,
-// no fences: the consumer is the next kernel on the stream).  One barrier;
-// the per-wave sums go through this helper's own LDS, so it must run once
-// per kernel, after the kernel's last use of its other LDS reductions.
-template <int NV>
+// Store this workgroup's partial sums of the first NSTORE of its NV values
+// at part[k*nb + bid] (no atomics, no fences: the consumer is the next kernel
+// on the stream); thread 0 is left with all NV workgroup totals in v, summed
+// in block_sum's order.  One barrier; the per-wave sums go through this
+// helper's own LDS, so it must run once per kernel, after the kernel's last
+// use of its other LDS reductions.
+template <int NV, int NSTORE = NV>
 __device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*red*/, double* part, int nb,
                                                      int bid) {
   __shared__ double sp[NV * MAX_WAVES];
@@ -105,7 +107,8 @@ __device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
       for (int w = 0; w < nw; ++w) s += sp[k * nw + w];
-      part[(size_t)k * nb + bid] = s;
+      v[k] = s;
+      if (k < NSTORE) part[(size_t)k * nb + bid] = s;
     }
   }
 }

@@ -461,10 +461,11 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
         dv[3] = (f_Rk * iM) * apm;
       }
     }
-    double rv[1] = {ab};
-    block_sum<1>(rv, red);
-    if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
-    block_partials_store<4>(dv, red, a.part_out, gridDim.x, blockIdx.x);
+    // the four dot products for the next iteration and this block's alpha-bar
+    // share one reduction (the alpha-bar total is not stored as a partial)
+    double dv5[5] = {dv[0], dv[1], dv[2], dv[3], ab};
+    block_partials_store<5, 4>(dv5, red, a.part_out, gridDim.x, blockIdx.x);
+    if (tid == 0) a.abar_blk[blockIdx.x] += dv5[4];
   } else if constexpr (MODE == RV_JAC_ITER) {
     if (valid) {
       const double xb1 = o_pb, iM = a.invM[gme];
