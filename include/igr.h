@@ -271,7 +271,10 @@ int igr_release_cache(void);
  * each launch stamped with start/end events by hipExtLaunchKernelGGL on the
  * context stream.  kernel 0 (fused SIP apply + PCG direction update) and 1
  * (PCG x/r update) are timed inside a PCG solve of `reps` iterations;
- * kernel 2 (hyperbolic RHS + RK3 stage update) over `reps` launches. */
+ * kernel 2 (hyperbolic RHS + RK3 stage update) over `reps` launches.
+ * kernel 3: kernel 0 of iteration 1 launched `reps` times back to back with
+ * events around the batch only (per-launch stamps add their own overhead):
+ * the mean in-stream time per launch, gaps included. */
 int igr_time_kernel(igr_ctx* ctx, int kernel, double gamma, double alpha, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus
