@@ -211,3 +211,32 @@ def test_adjoint_pde_errors():
     with pytest.raises(ValueError):
         c1.run_adjoint_conservative(3, 1e-3, 2e-3, 1.4, 0.0, 1)       # windowed KE is 2D only
     c1.close()
+
+
+@pytest.mark.parametrize("ic,A,prm", [("oseen_pair_blast", 2.0, None), ("sedov_blast", 3.0, (1.0, 0.1))])
+def test_paper_script_path_device_ic(ic, A, prm):
+    """The OtD-refinement scripts' path (IGR_adjoints_2d/paper_scripts/
+    generate_otd_refinement_{oseen,sedov}_data.jl): initial condition from the
+    reference's builder, run_forward_store, run_adjoint_conservative of the
+    energy objective.  Here the IC is built on the device; the oracle runs the
+    same steps from the same bits."""
+    print(f"[case] paper-script path {ic}", flush=True)
+    p, N, gam, alpha = 3, 12, 1.4, 1e-3
+    c = igr.Context(igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(p)))
+    c.build_ic(ic, A, gam, prm)
+    q0, _ = c.get_state()
+    dt = 0.3 * (1.0 / N) / ((2 * p + 1) * configs.max_wavespeed2(q0, gam))
+    T = 3 * dt * 0.999
+    n = c.run_forward_store(dt, T, gam, alpha, 15, "pcg")
+    g = O.Grid2D(p, N, N)
+    ref, nr = g.run_forward_store(gam, alpha, 15, O.PCG, dt, T, q0)
+    assert n == nr == 3
+    q, S = c.get_snapshot(n)
+    assert rel(q, ref[n, :4]) < 1e-12
+    assert rel(S, ref[n, 4]) < 1e-10
+    obj = igr.OBJECTIVES["kinetic_energy"]
+    a = c.run_adjoint_conservative(obj, dt, T, gam, alpha, 15, "pcg")
+    a_ref = g.run_adjoint_conservative(ref, n, dt, T, gam, alpha, 15, O.PCG, obj, (0.0, 0.0, 1.0, 0.0))
+    # the GPU adjoint runs on its own snapshots here, so forward rounding is inherited
+    assert rel(a, a_ref) < 1e-9
+    c.close()
