@@ -187,10 +187,14 @@ static int alloc_sets(Ctx& c, GradWork& W, int K) {
   return IGR_OK;
 }
 
+// Per-context gradient workspaces.  The maps are shared by all contexts, so
+// their lookups and erasures hold g_work_mu (contexts may be used from
+// several host threads); the slot a lookup returns belongs to one context.
 static std::map<const Ctx*, GradWork*> g_work;
+static std::mutex g_work_mu;
 
 static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
-  GradWork*& w = g_work[&c];
+  GradWork*& w = [&]() -> GradWork*& { std::lock_guard<std::mutex> lk(g_work_mu); return g_work[&c]; }();
   const size_t N = c.N;
   const int n = std::max(n_iter, 0);
   if (!w || w->n_iter != n) {
@@ -235,11 +239,12 @@ struct GradWork1 {
 static std::map<const Ctx*, GradWork1*> g_work1;
 
 static int alloc_work1(Ctx& c, int n_iter, int slots, GradWork1*& W) {
-  GradWork1*& w = g_work1[&c];
+  GradWork1*& w = [&]() -> GradWork1*& { std::lock_guard<std::mutex> lk(g_work_mu); return g_work1[&c]; }();
   const size_t N = c.N;
   const int n = std::max(n_iter, 0);
   if (!w || w->n_iter != n) {
-    w = new GradWork1();   // earlier buffers stay owned by the context until igr_destroy
+    delete w;              // its c.alloc buffers stay owned by the context until igr_destroy
+    w = new GradWork1();
     w->n_iter = n;
     K1AdjArgs& a = w->base;
     for (int s = 0; s < 3; ++s) {
@@ -894,6 +899,7 @@ hipError_t dev_malloc(void** p, size_t bytes) {
 }
 
 void free_grad_work(const Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_work_mu);
   auto it = g_work.find(c);
   if (it != g_work.end()) { delete it->second; g_work.erase(it); }
   auto it1 = g_work1.find(c);

@@ -25,7 +25,7 @@
 
 namespace igr {
 
-Recorder* g_rec = nullptr;
+thread_local Recorder* g_rec = nullptr;   // set while this host thread records a batch step
 
 void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int n_wg, int block);
 

@@ -156,12 +156,12 @@ static void launch1_run_np(const Ctx& c, K1Args a, int block) {
   const size_t bytes = k1_lds_doubles(a.Ne, NP) * sizeof(double);
   size_t smem = 0;
   if ((a.op == OP_RUN || a.op == OP_STORE) && bytes <= K1_LDS_MAX_BYTES) {
-    static bool attr = false;
-    if (!attr) {
-      attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k1_run<NP>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)K1_LDS_MAX_BYTES) == hipSuccess;
-      (void)hipGetLastError();
-    }
+    // set on every launch (once per whole run): the attribute belongs to the
+    // current device, and this keeps no state shared between host threads
+    const bool attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k1_run<NP>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)K1_LDS_MAX_BYTES) == hipSuccess;
+    (void)hipGetLastError();
     if (attr) { a.lds = 1; smem = bytes; }
   }
   // ordered on the context stream after the previous launch that read the block

@@ -583,7 +583,7 @@ __global__ void k_copy_batch(const CopyArgs* __restrict__ tab) {
 // Kernel-timestamp events for the next SIP/hyperbolic launch (set by
 // igr_time_kernel only): hipExtLaunchKernelGGL stamps them at the start and
 // end of the kernel itself, so their elapsed time excludes launch gaps.
-static hipEvent_t g_kt0 = nullptr, g_kt1 = nullptr;
+static thread_local hipEvent_t g_kt0 = nullptr, g_kt1 = nullptr;
 void set_kernel_timer(hipEvent_t t0, hipEvent_t t1) { g_kt0 = t0; g_kt1 = t1; }
 
 template <typename... Args, typename F = void (*)(Args...)>

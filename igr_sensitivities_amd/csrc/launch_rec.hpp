@@ -28,7 +28,7 @@ struct Recorder {
   }
 };
 
-extern Recorder* g_rec;
+extern thread_local Recorder* g_rec;
 
 // Batched launchers (k2_forward.hip, k2_tangent.hip): `work` has one
 // (instance, workgroup) entry per workgroup of the launch.

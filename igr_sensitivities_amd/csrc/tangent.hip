@@ -7,6 +7,7 @@
 #include <cmath>
 #include <vector>
 #include <map>
+#include <mutex>
 
 #include "igr_ctx.hpp"
 #include "host_common.hpp"
@@ -22,10 +23,12 @@ HypArgs hyp_base(const Ctx& c, double gamma);
 void launch1_run(const Ctx& c, const K1Args& a, int block);
 int k1_block(const Ctx& c);
 
+// Per-context tangent buffers; the shared map is guarded as the gradient's is.
 static std::map<const Ctx*, TanBufs*> g_tan;
+static std::mutex g_tan_mu;
 
 int tan_bufs(Ctx& c, int nt, TanBufs*& T) {
-  TanBufs*& t = g_tan[&c];
+  TanBufs*& t = [&]() -> TanBufs*& { std::lock_guard<std::mutex> lk(g_tan_mu); return g_tan[&c]; }();
   if (!t) t = new TanBufs();
   const size_t nvN = c.nv * c.N;
   while (t->nt < nt) {
@@ -39,6 +42,7 @@ int tan_bufs(Ctx& c, int nt, TanBufs*& T) {
 }
 
 void free_tan_bufs(const Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_tan_mu);
   auto it = g_tan.find(c);
   if (it != g_tan.end()) { delete it->second; g_tan.erase(it); }
 }

@@ -19,7 +19,8 @@
  *    from sqrt of a negative pressure maps to IGR_EDOMAIN.
  *  - The library owns device memory inside the context; state lives in HBM
  *    between calls.  One HIP stream per context; a context is not
- *    thread-safe (the reference is serial).
+ *    thread-safe (the reference is serial), but distinct contexts may be used
+ *    from distinct host threads at the same time.
  */
 #ifndef IGR_H
 #define IGR_H
