@@ -122,13 +122,20 @@ __device__ __forceinline__ void grid_total(const double* part, int nb, double (&
   __shared__ double sg[NV * MAX_WAVES];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  // one pass over the workgroups loads all NV partials together (one memory
+  // round trip, not NV); each value still sums over b in increasing order
   double acc[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)k * nb + b];
-    acc[k] = wave_sum(s);
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = part[(size_t)k * nb + b];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += v[k];
   }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) sg[k * nw + wid] = acc[k];

@@ -235,13 +235,14 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     }
   }
   if constexpr (MODE == RV_PCG_ITER) {
-    double d[4];
-    grid_total<4>(a.part_in, gridDim.x, d, red);
-    const double eps = 1e-30;
+    // the stored scalars are loaded with the partials, not after them
     ak = a.sc[fs.a(k)];
     bk = a.sc[fs.beta(k)];
     const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
     const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
+    double d[4];
+    grid_total<4>(a.part_in, gridDim.x, d, red);
+    const double eps = 1e-30;
     const double bb = d[0];
     rzb = carry_next + bb / (rzk + eps);
     const double abk = d[1] - d[2] - 2.0 * rzb * d[3];

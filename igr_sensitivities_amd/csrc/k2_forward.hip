@@ -78,10 +78,11 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         lr[l] = a.invR[gi];
       }
     }
+    const int nit = a.n_iter, it = a.it;
+    const double rz_prev = a.first ? 0.0 : a.sc[it - 1];   // loaded with the partials, not after them
     double tot[1];
     grid_total<1>(a.part_in, nblk, tot, red);
-    const int nit = a.n_iter, it = a.it;
-    if (!a.first) beta = tot[0] / (a.sc[it - 1] + 1e-30);
+    if (!a.first) beta = tot[0] / (rz_prev + 1e-30);
     if (blk == 0 && tid == 0) {
       a.sc[it] = tot[0];
       if (!a.first) a.sc[3 * nit + 1 + it - 1] = beta;
@@ -319,9 +320,10 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
     gi = t.gidx(s, n);
     x = a.xout[gi]; p = a.pold[gi]; r = a.r ? a.r[gi] : a.rout[gi]; ap = a.y[gi]; im = a.invM[gi];
   }
+  const double rz_it = sc[it];   // loaded with the partials, not after them
   double tot[1];
   grid_total<1>(a.part_in, nblk, tot, red);
-  const double ak = sc[it] / (tot[0] + 1e-30);
+  const double ak = rz_it / (tot[0] + 1e-30);
   if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
   double red_v[1] = {0.0};
   if (valid) {
