@@ -45,14 +45,21 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
   double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0}, dbo[4] = {0.0, 0.0, 0.0, 0.0}, qbo[4] = {0.0, 0.0, 0.0, 0.0};
+  // the accumulators this thread updates at the end, loaded now so their
+  // latency overlaps the state loads (qbar_stage, qbar_n, sbar_in and
+  // sbar_out are distinct buffers in every launch)
+  double pf_stage[4] = {0.0, 0.0, 0.0, 0.0}, pf_n[4] = {0.0, 0.0, 0.0, 0.0}, pf_sb = 0.0;
   if (valid) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       qo[v] = a.q[v * N + gme];
       qbo[v] = a.qbar_out[v * N + gme];
       dbo[v] = a.c_dt * qbo[v];
+      if (a.accumulate) pf_stage[v] = a.qbar_stage[v * N + gme];
+      if (a.qbar_n) pf_n[v] = a.qbar_n[v * N + gme];
     }
     qo[4] = a.S[gme];
+    if (a.sbar_in) pf_sb = a.sbar_in[gme];
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
@@ -155,11 +162,11 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
     for (int v = 0; v < 4; ++v) {
       const size_t gi = v * N + gme;
       const double val = a.c_prev * qbo[v] + qb[v];
-      if (a.accumulate) a.qbar_stage[gi] += val;
+      if (a.accumulate) a.qbar_stage[gi] = pf_stage[v] + val;
       else a.qbar_stage[gi] = val;
-      if (a.qbar_n) a.qbar_n[gi] += a.c_n * qbo[v];
+      if (a.qbar_n) a.qbar_n[gi] = pf_n[v] + a.c_n * qbo[v];
     }
-    a.sbar_out[gme] = (a.sbar_in ? a.sbar_in[gme] : 0.0) + sb;
+    a.sbar_out[gme] = pf_sb + sb;
   }
   double rv[1] = {gb};
   block_sum<1>(rv, red);

@@ -132,10 +132,15 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
   double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0}, so[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double sn[4] = {0.0, 0.0, 0.0, 0.0};   // s^n of stages 2 and 3, loaded with the stage values
   if (valid) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) { qo[v] = a.q[v * N + gme]; so[v] = a.s[v * N + gme]; }
     qo[4] = a.S[gme]; so[4] = a.vs[gme];
+    if (a.mode >= 2) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sn[v] = a.sn[v * N + gme];
+    }
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) { qs[v][tid] = qo[v]; ss[v][tid] = so[v]; }
@@ -245,8 +250,8 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
     double o;
     if (a.mode == 0) o = acc[v];
     else if (a.mode == 1) o = so[v] + a.cdt * acc[v];
-    else if (a.mode == 2) o = 3.0 / 4.0 * a.sn[gi] + 1.0 / 4.0 * so[v] + a.cdt * acc[v];
-    else o = 1.0 / 3.0 * a.sn[gi] + 2.0 / 3.0 * so[v] + a.cdt * acc[v];
+    else if (a.mode == 2) o = 3.0 / 4.0 * sn[v] + 1.0 / 4.0 * so[v] + a.cdt * acc[v];
+    else o = 1.0 / 3.0 * sn[v] + 2.0 / 3.0 * so[v] + a.cdt * acc[v];
     a.out[gi] = o;
   }
 }
