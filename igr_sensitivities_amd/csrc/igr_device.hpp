@@ -115,19 +115,29 @@ __device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*
 
 // Every workgroup sums the nb partials of the previous kernel in the same
 // fixed order (per thread, then per wave, then over the waves), so all
-// workgroups hold bit-identical totals.  One barrier; own LDS, used once
-// per kernel.
+// workgroups hold bit-identical totals.  grid_prefetch issues the loads of
+// the first chunk (b = thread index) so a kernel can start them before its
+// other loads; grid_finish adds any further chunks and reduces.  One
+// barrier; own LDS, used once per kernel.
 template <int NV>
-__device__ __forceinline__ void grid_total(const double* part, int nb, double (&out)[NV], double* /*red*/) {
+__device__ __forceinline__ void grid_prefetch(const double* part, int nb, double (&pre)[NV]) {
+  // no branch: a thread beyond nb reads entry nb - 1 and drops it in
+  // grid_finish (a load inside a branch is waited for before the branch ends)
+  const int b = min((int)threadIdx.x, nb - 1);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) pre[k] = part[(size_t)k * nb + b];
+}
+template <int NV>
+__device__ __forceinline__ void grid_finish(const double* part, int nb, const double (&pre)[NV], double (&out)[NV]) {
   __shared__ double sg[NV * MAX_WAVES];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
-  // one pass over the workgroups loads all NV partials together (one memory
-  // round trip, not NV); each value still sums over b in increasing order
+  // one pass over the remaining workgroups loads all NV partials together;
+  // each value sums over b in increasing order, from 0.0
   double acc[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  for (int k = 0; k < NV; ++k) acc[k] = (int)threadIdx.x < nb ? 0.0 + pre[k] : 0.0;
+  for (int b = threadIdx.x + blockDim.x; b < nb; b += blockDim.x) {
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = part[(size_t)k * nb + b];
@@ -147,6 +157,12 @@ __device__ __forceinline__ void grid_total(const double* part, int nb, double (&
     for (int w = 0; w < nw; ++w) s += sg[k * nw + w];
     out[k] = s;
   }
+}
+template <int NV>
+__device__ __forceinline__ void grid_total(const double* part, int nb, double (&out)[NV], double* /*red*/) {
+  double pre[NV];
+  grid_prefetch<NV>(part, nb, pre);
+  grid_finish<NV>(part, nb, pre, out);
 }
 
 // Julia max(x, y): returns x on ties.

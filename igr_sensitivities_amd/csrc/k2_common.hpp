@@ -114,7 +114,9 @@ struct TileInfo {
   __device__ int nb_right(int s) const { const int lx = s % TX, ly = s / TX; return lx < nvx - 1 ? s + 1 : T + TY + ly; }
   __device__ int nb_bottom(int s) const { const int lx = s % TX, ly = s / TX; return ly > 0 ? s - TX : T + 2 * TY + lx; }
   __device__ int nb_top(int s) const { const int lx = s % TX, ly = s / TX; return ly < nvy - 1 ? s + TX : T + 2 * TY + TX + lx; }
-  // element of LDS slot sl; false when the slot is outside this tile's rows/columns
+  // element of LDS slot sl; false when the slot is outside this tile's rows/columns.
+  // Periodic wrap by comparison (ex0 < Nex, ex0 + nvx <= Nex): a runtime
+  // integer modulo costs tens of instructions ahead of every halo load.
   __device__ bool slot_elem(int sl, int& gex, int& gey) const {
     if (sl < T) {
       const int lx = sl % TX, ly = sl / TX;
@@ -122,14 +124,14 @@ struct TileInfo {
       gex = ex0 + lx; gey = ey0 + ly; return true;
     }
     int h = sl - T;
-    if (h < TY) { if (h >= nvy) return false; gex = (ex0 - 1 + Nex) % Nex; gey = ey0 + h; return true; }
+    if (h < TY) { if (h >= nvy) return false; gex = ex0 > 0 ? ex0 - 1 : Nex - 1; gey = ey0 + h; return true; }
     h -= TY;
-    if (h < TY) { if (h >= nvy) return false; gex = (ex0 + nvx) % Nex; gey = ey0 + h; return true; }
+    if (h < TY) { if (h >= nvy) return false; gex = ex0 + nvx < Nex ? ex0 + nvx : ex0 + nvx - Nex; gey = ey0 + h; return true; }
     h -= TY;
-    if (h < TX) { if (h >= nvx) return false; gex = ex0 + h; gey = (ey0 - 1 + Ney) % Ney; return true; }
+    if (h < TX) { if (h >= nvx) return false; gex = ex0 + h; gey = ey0 > 0 ? ey0 - 1 : Ney - 1; return true; }
     h -= TX;
     if (h >= nvx) return false;
-    gex = ex0 + h; gey = (ey0 + nvy) % Ney; return true;
+    gex = ex0 + h; gey = ey0 + nvy < Ney ? ey0 + nvy : ey0 + nvy - Ney; return true;
   }
   __device__ bool slot_global(int sl, int nn, size_t& gi) const {
     int gex, gey;

@@ -59,69 +59,100 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   const bool valid = lx < t.nvx && ly < t.nvy;
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
+  // basis in LDS: read per lane below, each access to the argument block
+  // would be a memory round trip in the middle of the kernel.  Loaded now
+  // (clamped index, no branch) and written to LDS with the staged vector.
+  __shared__ double sD[NP * NP], sW[NP];
+  const double bD = B.D[tid < NP * NP ? tid : 0];
+  const double bW = B.w[tid < NP ? tid : 0];
+
   double own_d = 0.0;
-  double beta = 0.0;
-  if constexpr (MODE == M_PCG_A) {
-    // Load z = r/M, p_old and 1/rho of the tile and halo first, so these
-    // loads are in flight together with the partial sums; then
-    // beta_{it-1} = rz_it / (rz_{it-1} + 1e-30) and p = z + beta p_old in LDS.
-    constexpr int NL = (NS * NP2 + NT - 1) / NT;
-    double lz[NL], lp[NL], lr[NL];
+  double beta = 0.0, sc_tot = 0.0;
+  // PCG_A: the previous kernel's partial sums are requested first
+  double pre[1] = {0.0};
+  if constexpr (MODE == M_PCG_A) grid_prefetch<1>(a.part_in, nblk, pre);
+
+  // Tile + halo loads.  Every slot's node index first, then all the loads
+  // with no branch between them (an empty slot reads node 0 and is zeroed
+  // afterwards), so they are in flight together; a load inside a per-slot
+  // branch waits for the previous slot's data before it is issued.
+  constexpr int NL = (NS * NP2 + NT - 1) / NT;
+  size_t gis[NL];
+  bool oks[NL];
 #pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const int idx = tid + l * NT;
-      lz[l] = 0.0; lp[l] = 0.0; lr[l] = 0.0;
-      size_t gi;
-      if (idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi)) {
-        lz[l] = a.r[gi] * a.invM[gi];
-        if (!a.first) lp[l] = a.pold[gi];
-        lr[l] = a.invR[gi];
-      }
-    }
-    const int nit = a.n_iter, it = a.it;
-    const double rz_prev = a.first ? 0.0 : a.sc[it - 1];   // loaded with the partials, not after them
-    double tot[1];
-    grid_total<1>(a.part_in, nblk, tot, red);
-    if (!a.first) beta = tot[0] / (rz_prev + 1e-30);
-    if (blk == 0 && tid == 0) {
-      a.sc[it] = tot[0];
-      if (!a.first) a.sc[3 * nit + 1 + it - 1] = beta;
-    }
+  for (int l = 0; l < NL; ++l) {
+    const int idx = tid + l * NT;
+    size_t gi = 0;
+    const bool ok = idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi);
+    gis[l] = ok ? gi : 0;
+    oks[l] = ok;
+  }
+  // PCG_A: r, 1/M, p_old, 1/rho; CHEB: r, 1/M, d, x, 1/rho; INIT: x, rho, mx, my;
+  // APPLY: x, rho; JACOBI: x, 1/rho.  p_old and d are not used in a first
+  // iteration; r stands in for them there, so every load has a valid address.
+  const double* pold = a.first ? a.r : a.pold;
+  const double* dold = a.first ? a.r : a.d;
+  double L0[NL], L1[NL], L2[NL], L3[NL], L4[NL];
 #pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const int idx = tid + l * NT;
-      if (idx < NS * NP2) {
-        vs[idx] = a.first ? lz[l] : lz[l] + beta * lp[l];
-        irs[idx] = lr[l];
-      }
+  for (int l = 0; l < NL; ++l) {
+    const size_t gi = gis[l];
+    L2[l] = 0.0; L3[l] = 0.0; L4[l] = 0.0;
+    if constexpr (MODE == M_PCG_A) {
+      L0[l] = a.r[gi]; L1[l] = a.invM[gi]; L2[l] = pold[gi]; L3[l] = a.invR[gi];
+    } else if constexpr (MODE == M_APPLY) {
+      L0[l] = a.x[gi]; L1[l] = a.rho[gi];
+    } else if constexpr (INIT) {
+      L0[l] = a.x[gi]; L1[l] = a.rho[gi]; L2[l] = a.mx[gi]; L3[l] = a.my[gi];
+    } else if constexpr (MODE == M_JACOBI) {
+      L0[l] = a.x[gi]; L1[l] = a.invR[gi];
+    } else {   // M_CHEB
+      L0[l] = a.r[gi]; L1[l] = a.invM[gi]; L2[l] = dold[gi]; L3[l] = a.x[gi]; L4[l] = a.invR[gi];
     }
   }
+  if constexpr (MODE == M_PCG_A) {
+    // beta_{it-1} = rz_it / (rz_{it-1} + 1e-30), then p = z + beta p_old in LDS
+    const int nit = a.n_iter, it = a.it;
+    const double rz_prev = a.first ? 0.0 : a.sc[it - 1];
+    double tot[1];
+    grid_finish<1>(a.part_in, nblk, pre, tot);
+    if (!a.first) beta = tot[0] / (rz_prev + 1e-30);
+    sc_tot = tot[0];   // stored with the outputs, after the last barrier
+    (void)nit;
+  }
   // ---- stage the vector, 1/rho (and velocities for INIT) in LDS
-  for (int idx = tid; MODE != M_PCG_A && idx < NS * NP2; idx += NT) {
-    const int sl = idx / NP2, nn = idx % NP2;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    const int idx = tid + l * NT;
+    if (idx >= NS * NP2) continue;
+    const int sl = idx / NP2;
     double v = 0.0, ir = 0.0;
-    size_t gi;
-    if (t.slot_global(sl, nn, gi)) {
-      if constexpr (MODE == M_APPLY) {
-        v = a.x[gi]; ir = 1.0 / a.rho[gi];
+    if (oks[l]) {
+      if constexpr (MODE == M_PCG_A) {
+        const double lz = L0[l] * L1[l];
+        v = a.first ? lz : lz + beta * L2[l];
+        ir = L3[l];
+      } else if constexpr (MODE == M_APPLY) {
+        v = L0[l]; ir = 1.0 / L1[l];
       } else if constexpr (INIT) {
-        v = a.x[gi];
-        double rh = a.rho[gi];
+        v = L0[l];
+        const double rh = L1[l];
         ir = 1.0 / rh;
-        if (sl < T) { us[idx] = a.mx[gi] / rh; ws_[idx] = a.my[gi] / rh; }
+        if (sl < T) { us[idx] = L2[l] / rh; ws_[idx] = L3[l] / rh; }
       } else if constexpr (MODE == M_JACOBI) {
-        v = a.x[gi]; ir = a.invR[gi];
-      } else if constexpr (MODE == M_CHEB) {
-        double z = a.r[gi] * a.invM[gi];
-        double dn = a.first ? z / a.theta : a.c1 * a.d[gi] + a.c2 * z;
-        v = a.x[gi] + dn;
-        ir = a.invR[gi];
+        v = L0[l]; ir = L1[l];
+      } else {   // M_CHEB
+        const double z = L0[l] * L1[l];
+        const double dn = a.first ? z / a.theta : a.c1 * L2[l] + a.c2 * z;
+        v = L3[l] + dn;
+        ir = L4[l];
         if (idx == tid) own_d = dn;
       }
     }
     vs[idx] = v;
     irs[idx] = ir;
   }
+  if (tid < NP * NP) sD[tid] = bD;
+  if (tid < NP) sW[tid] = bW;
   __syncthreads();
 
   const double al = a.alpha;
@@ -134,18 +165,18 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     double dxi = 0.0, deta = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      dxi += B.D[i * NP + k] * vs[s * NP2 + k + NP * j];
-      deta += B.D[j * NP + k] * vs[s * NP2 + i + NP * k];
+      dxi += sD[i * NP + k] * vs[s * NP2 + k + NP * j];
+      deta += sD[j * NP + k] * vs[s * NP2 + i + NP * k];
     }
-    gx[s * NP2 + n] = al * B.w[i] * B.w[j] * g.Jy * g.invJx * ir * dxi;
-    gy[s * NP2 + n] = al * B.w[i] * B.w[j] * g.Jx * g.invJy * ir * deta;
+    gx[s * NP2 + n] = al * sW[i] * sW[j] * g.Jy * g.invJx * ir * dxi;
+    gy[s * NP2 + n] = al * sW[i] * sW[j] * g.Jx * g.invJy * ir * deta;
     if (i == 0) {            // x-face between left neighbour (L) and this element (R)
       const double xL = vs[sL * NP2 + (NP - 1) + NP * j];
       const double irL = irs[sL * NP2 + (NP - 1) + NP * j];
       double dL = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) dL += B.D[(NP - 1) * NP + k] * vs[sL * NP2 + k + NP * j];
-      const double jump = xL - vown, wf = B.w[j] * g.Jy;
+      for (int k = 0; k < NP; ++k) dL += sD[(NP - 1) * NP + k] * vs[sL * NP2 + k + NP * j];
+      const double jump = xL - vown, wf = sW[j] * g.Jy;
       const double avg = g.invJx * (irL * dL + ir * dxi) / 2.0;
       sigx_l = g.eta * fmax(irL, ir) / g.dx;
       fc += al * avg * wf;
@@ -157,8 +188,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       const double irR = irs[sR * NP2 + NP * j];
       double dR = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) dR += B.D[k] * vs[sR * NP2 + k + NP * j];
-      const double jump = vown - xR, wf = B.w[j] * g.Jy;
+      for (int k = 0; k < NP; ++k) dR += sD[k] * vs[sR * NP2 + k + NP * j];
+      const double jump = vown - xR, wf = sW[j] * g.Jy;
       const double avg = g.invJx * (ir * dxi + irR * dR) / 2.0;
       sigx_r = g.eta * fmax(ir, irR) / g.dx;
       fc += -al * avg * wf;
@@ -170,8 +201,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       const double irL = irs[sB * NP2 + i + NP * (NP - 1)];
       double dL = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) dL += B.D[(NP - 1) * NP + k] * vs[sB * NP2 + i + NP * k];
-      const double jump = xL - vown, wf = B.w[i] * g.Jx;
+      for (int k = 0; k < NP; ++k) dL += sD[(NP - 1) * NP + k] * vs[sB * NP2 + i + NP * k];
+      const double jump = xL - vown, wf = sW[i] * g.Jx;
       const double avg = g.invJy * (irL * dL + ir * deta) / 2.0;
       sigy_b = g.eta * fmax(irL, ir) / g.dy;
       fc += al * avg * wf;
@@ -183,8 +214,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       const double irR = irs[sT * NP2 + i];
       double dR = 0.0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) dR += B.D[k] * vs[sT * NP2 + i + NP * k];
-      const double jump = vown - xR, wf = B.w[i] * g.Jx;
+      for (int k = 0; k < NP; ++k) dR += sD[k] * vs[sT * NP2 + i + NP * k];
+      const double jump = vown - xR, wf = sW[i] * g.Jx;
       const double avg = g.invJy * (ir * deta + irR * dR) / 2.0;
       sigy_t = g.eta * fmax(ir, irR) / g.dy;
       fc += -al * avg * wf;
@@ -195,15 +226,15 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        uxx += B.D[i * NP + k] * us[s * NP2 + k + NP * j];
-        uyx += B.D[i * NP + k] * ws_[s * NP2 + k + NP * j];
-        uxy += B.D[j * NP + k] * us[s * NP2 + i + NP * k];
-        uyy += B.D[j * NP + k] * ws_[s * NP2 + i + NP * k];
+        uxx += sD[i * NP + k] * us[s * NP2 + k + NP * j];
+        uyx += sD[i * NP + k] * ws_[s * NP2 + k + NP * j];
+        uxy += sD[j * NP + k] * us[s * NP2 + i + NP * k];
+        uyy += sD[j * NP + k] * ws_[s * NP2 + i + NP * k];
       }
       uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
       const double dv = uxx + uyy;
       const double R = al * (dv * dv + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
-      bval = B.w[i] * B.w[j] * g.Jx * g.Jy * R;
+      bval = sW[i] * sW[j] * g.Jx * g.Jy * R;
       if (a.b_in) bval = a.b_in[gme];
     }
   }
@@ -218,16 +249,16 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
 
   double y = 0.0;
   if (valid) {
-    y = B.w[i] * B.w[j] * g.Jx * g.Jy * vown * ir;
+    y = sW[i] * sW[j] * g.Jx * g.Jy * vown * ir;
     double st = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      st += B.D[k * NP + i] * gx[s * NP2 + k + NP * j];
-      st += B.D[k * NP + j] * gy[s * NP2 + i + NP * k];
+      st += sD[k * NP + i] * gx[s * NP2 + k + NP * j];
+      st += sD[k * NP + j] * gy[s * NP2 + i + NP * k];
     }
     y += st;
-    y += sxl[s * NP + j] * B.D[i] + sxr[s * NP + j] * B.D[(NP - 1) * NP + i];
-    y += syb[s * NP + i] * B.D[j] + syt[s * NP + i] * B.D[(NP - 1) * NP + j];
+    y += sxl[s * NP + j] * sD[i] + sxr[s * NP + j] * sD[(NP - 1) * NP + i];
+    y += syb[s * NP + i] * sD[j] + syt[s * NP + i] * sD[(NP - 1) * NP + j];
     y += fc;
   }
 
@@ -235,47 +266,61 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     if (valid) a.y[gme] = y;
   } else if constexpr (INIT) {
     double red_v[1] = {0.0};
+    double o_iM = 0.0, o_r = 0.0;
     if (valid) {
       // diag(A): mass, x/y stiffness self-coupling, face terms
-      double Md = B.w[i] * B.w[j] * g.Jx * g.Jy * ir;
+      double Md = sW[i] * sW[j] * g.Jx * g.Jy * ir;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        const double D2 = B.D[k * NP + i] * B.D[k * NP + i];
-        Md += al * B.w[k] * B.w[j] * g.Jy * g.invJx * D2 * irs[s * NP2 + k + NP * j];
+        const double D2 = sD[k * NP + i] * sD[k * NP + i];
+        Md += al * sW[k] * sW[j] * g.Jy * g.invJx * D2 * irs[s * NP2 + k + NP * j];
       }
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        const double D2 = B.D[k * NP + j] * B.D[k * NP + j];
-        Md += al * B.w[i] * B.w[k] * g.Jx * g.invJy * D2 * irs[s * NP2 + i + NP * k];
+        const double D2 = sD[k * NP + j] * sD[k * NP + j];
+        Md += al * sW[i] * sW[k] * g.Jx * g.invJy * D2 * irs[s * NP2 + i + NP * k];
       }
-      if (i == NP - 1) Md += (-al * g.invJx * B.D[(NP - 1) * NP + NP - 1] * ir + al * sigx_r) * (B.w[j] * g.Jy);
-      if (i == 0) Md += (al * g.invJx * B.D[0] * ir + al * sigx_l) * (B.w[j] * g.Jy);
-      if (j == NP - 1) Md += (-al * g.invJy * B.D[(NP - 1) * NP + NP - 1] * ir + al * sigy_t) * (B.w[i] * g.Jx);
-      if (j == 0) Md += (al * g.invJy * B.D[0] * ir + al * sigy_b) * (B.w[i] * g.Jx);
+      if (i == NP - 1) Md += (-al * g.invJx * sD[(NP - 1) * NP + NP - 1] * ir + al * sigx_r) * (sW[j] * g.Jy);
+      if (i == 0) Md += (al * g.invJx * sD[0] * ir + al * sigx_l) * (sW[j] * g.Jy);
+      if (j == NP - 1) Md += (-al * g.invJy * sD[(NP - 1) * NP + NP - 1] * ir + al * sigy_t) * (sW[i] * g.Jx);
+      if (j == 0) Md += (al * g.invJy * sD[0] * ir + al * sigy_b) * (sW[i] * g.Jx);
       const double iM = 1.0 / Md;
-      a.invR_out[gme] = ir;
-      a.invM_out[gme] = iM;
+      o_iM = iM;
       if (!(ir > 0.0) || !isfinite(ir)) flag_domain(a.flag);
       if constexpr (MODE == M_INIT_PCG) {
         const double r0 = bval - y;
-        a.rout[gme] = r0;
+        o_r = r0;
         red_v[0] = r0 * (r0 * iM);
       } else if constexpr (MODE == M_INIT_CHEB) {
-        a.rout[gme] = bval - y;
+        o_r = bval - y;
+      }
+    }
+    // the reduction's barrier first: stores ahead of a barrier are waited for
+    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+    if (valid) {
+      a.invR_out[gme] = ir;
+      a.invM_out[gme] = o_iM;
+      if constexpr (MODE == M_INIT_PCG) {
+        a.rout[gme] = o_r;
+      } else if constexpr (MODE == M_INIT_CHEB) {
+        a.rout[gme] = o_r;
         a.b_out[gme] = bval;
       } else {
         a.b_out[gme] = bval;
       }
     }
-    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
   } else if constexpr (MODE == M_PCG_A) {
-    double red_v[1] = {0.0};
+    // the reduction's barrier first: stores ahead of a barrier are waited for
+    double red_v[1] = {valid ? vown * y : 0.0};
+    block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
     if (valid) {
       a.pnew[gme] = vown;
       a.y[gme] = y;
-      red_v[0] = vown * y;
     }
-    block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+    if (blk == 0 && tid == 0) {
+      a.sc[a.it] = sc_tot;
+      if (!a.first) a.sc[3 * a.n_iter + 1 + a.it - 1] = beta;
+    }
   } else if constexpr (MODE == M_JACOBI) {
     if (valid) {
       if (a.y) a.y[gme] = y;     // A x_k, kept by the adjoint's recompute
@@ -313,26 +358,27 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
   const bool valid = lx < t.nvx && ly < t.nvy;
   double* sc = a.sc;
   const int nit = a.n_iter, it = a.it;
-  // gather this thread's fields first, then the pAp total
-  size_t gi = 0;
-  double x = 0.0, p = 0.0, r = 0.0, ap = 0.0, im = 0.0;
-  if (valid) {
-    gi = t.gidx(s, n);
-    x = a.xout[gi]; p = a.pold[gi]; r = a.r ? a.r[gi] : a.rout[gi]; ap = a.y[gi]; im = a.invM[gi];
-  }
-  const double rz_it = sc[it];   // loaded with the partials, not after them
+  // the partial sums first, then this thread's fields with no branch around
+  // the loads (an invalid thread reads node 0 and stores nothing)
+  double pre[1];
+  grid_prefetch<1>(a.part_in, nblk, pre);
+  const size_t gi = valid ? t.gidx(s, n) : 0;
+  const double* rin = a.r ? a.r : a.rout;
+  const double x = a.xout[gi], p = a.pold[gi], r = rin[gi], ap = a.y[gi], im = a.invM[gi];
+  const double rz_it = sc[it];
   double tot[1];
-  grid_total<1>(a.part_in, nblk, tot, red);
+  grid_finish<1>(a.part_in, nblk, pre, tot);
   const double ak = rz_it / (tot[0] + 1e-30);
-  if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
-  double red_v[1] = {0.0};
-  if (valid) {
-    a.xout[gi] = x + ak * p;
-    const double rn = r - ak * ap;
-    a.rout[gi] = rn;
-    red_v[0] = rn * (rn * im);
-  }
+  const double xn = x + ak * p;
+  const double rn = r - ak * ap;
+  // the reduction's barrier first: stores ahead of a barrier are waited for
+  double red_v[1] = {valid ? rn * (rn * im) : 0.0};
   block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+  if (valid) {
+    a.xout[gi] = xn;
+    a.rout[gi] = rn;
+  }
+  if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
 }
 template <int NP, int TX, int TY>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
