@@ -202,44 +202,63 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const size_t gme = valid ? t.gidx(s, n) : 0;
   const FS fs(a.n_iter);
   const int k = a.k;
+  // basis in LDS: read per lane below, each access to the argument block
+  // would be a memory round trip in the middle of the kernel
+  __shared__ double sD[NP * NP], sW[NP];
+  const double bD = B.D[tid < NP * NP ? tid : 0];
+  const double bW = B.w[tid < NP ? tid : 0];
 
   // reverse scalars of this iteration, from the dot products the previous
   // kernel left as partial sums (every workgroup forms the same totals)
   double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
-  // PCG_ITER: load the tile + halo fields first so they are in flight with
-  // the partial sums (the staging loop below then only combines them)
+  double r_bb = 0.0, r_abk = 0.0, r_carry = 0.0;   // reverse scalars block 0 records at the end
+  // PCG_ITER: the previous kernel's partial sums are requested first; then
+  // the tile + halo fields and this thread's own-node inputs, every address
+  // first and the loads with no branch between them (an empty slot reads
+  // node 0 and is zeroed afterwards), so all are in flight together
+  double pre4[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (MODE == RV_PCG_ITER) grid_prefetch<4>(a.part_in, gridDim.x, pre4);
   constexpr int NL = (NS * NP2 + NT - 1) / NT;
   double l_ir[NL], l_iM[NL], l_rc[NL], l_pb[NL], l_R[NL], l_p[NL];
+  double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0, f_ab = 0.0;
   if constexpr (MODE == RV_PCG_ITER) {
+    size_t gis[NL];
+    bool oks[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       const int idx = tid + l * NT;
-      l_ir[l] = 0.0; l_iM[l] = 0.0; l_rc[l] = 0.0; l_pb[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0;
-      size_t gi;
-      if (idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi)) {
-        l_ir[l] = a.invR[gi];
-        l_iM[l] = a.invM[gi];
-        if (!a.first) { l_rc[l] = a.rbar_in[gi]; l_pb[l] = a.pbar_in[gi]; }
-        l_R[l] = a.R[(size_t)(k + 1) * N + gi];
-        l_p[l] = a.P[(size_t)k * N + gi];
-      }
+      size_t gi = 0;
+      const bool ok = idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi);
+      gis[l] = ok ? gi : 0;
+      oks[l] = ok;
     }
-  }
-  // own-node inputs of the outputs, loaded here so their latency overlaps
-  // the tile loads instead of following the LDS phases
-  double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0;
-  if constexpr (MODE == RV_PCG_ITER) {
-    if (valid) {
-      f_irb = a.irbar[gme];
-      f_imb = a.imbar[gme];
-      f_xb = a.xbar[gme];
-      f_apk = a.AP[(size_t)k * N + gme];
-      if (k >= 1) {
-        f_pm = a.P[(size_t)(k - 1) * N + gme];
-        f_apm = a.AP[(size_t)(k - 1) * N + gme];
-        f_Rk = a.R[(size_t)k * N + gme];
-      }
+    // r-bar and p-bar of the next iteration do not exist in a first reverse
+    // iteration, nor P_{k-1}, AP_{k-1}, R_k at k = 0: valid stand-ins are read
+    // and the values dropped
+    const double* rcin = a.first ? a.invR : a.rbar_in;
+    const double* pbin = a.first ? a.invR : a.pbar_in;
+    const double* Rk1 = a.R + (size_t)(k + 1) * N;
+    const double* Pk = a.P + (size_t)k * N;
+    const double* APk = a.AP + (size_t)k * N;
+    const double* Pm = k >= 1 ? a.P + (size_t)(k - 1) * N : a.P;
+    const double* APm = k >= 1 ? a.AP + (size_t)(k - 1) * N : a.AP;
+    const double* Rk = a.R + (size_t)k * N;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const size_t gi = gis[l];
+      l_ir[l] = a.invR[gi]; l_iM[l] = a.invM[gi]; l_rc[l] = rcin[gi]; l_pb[l] = pbin[gi];
+      l_R[l] = Rk1[gi]; l_p[l] = Pk[gi];
     }
+    f_irb = a.irbar[gme]; f_imb = a.imbar[gme]; f_xb = a.xbar[gme]; f_apk = APk[gme];
+    f_pm = Pm[gme]; f_apm = APm[gme]; f_Rk = Rk[gme];
+    f_ab = a.abar_blk[blockIdx.x];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      if (!oks[l]) { l_ir[l] = 0.0; l_iM[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0; }
+      if (!oks[l] || a.first) { l_rc[l] = 0.0; l_pb[l] = 0.0; }
+    }
+    if (!valid) { f_irb = 0.0; f_imb = 0.0; f_xb = 0.0; f_apk = 0.0; }
+    if (!valid || k < 1) { f_pm = 0.0; f_apm = 0.0; f_Rk = 0.0; }
   }
   if constexpr (MODE == RV_PCG_ITER) {
     // the stored scalars are loaded with the partials, not after them
@@ -248,19 +267,15 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
     const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
     double d[4];
-    grid_total<4>(a.part_in, gridDim.x, d, red);
+    grid_finish<4>(a.part_in, gridDim.x, pre4, d);
     const double eps = 1e-30;
     const double bb = d[0];
     rzb = carry_next + bb / (rzk + eps);
     const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
     papb = -abk * rzk / ((papk + eps) * (papk + eps));
-    if (blockIdx.x == 0 && tid == 0) {
-      a.rs[RS::bbar(k)] = bb;
-      a.rs[RS::abar(k)] = abk;
-      a.rs[RS::papbar(k)] = papb;
-      a.rs[RS::rzbar_next(k)] = rzb;
-      a.rs[RS::carry(k)] = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
-    }
+    // stored after the last barrier (a barrier waits for earlier stores)
+    r_bb = bb; r_abk = abk;
+    r_carry = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
   } else if constexpr (FIN) {
     rzb = a.n_iter > 0 ? a.rs[RS::carry(0)] : 0.0;
   }
@@ -338,6 +353,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     ws[idx] = w; vs[idx] = v; irs[idx] = ir;
     if constexpr (FIN) mbs[idx] = mb;
   }
+  if (tid < NP * NP) sD[tid] = bD;
+  if (tid < NP) sW[tid] = bW;
   __syncthreads();
 
   const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
@@ -348,14 +365,14 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     double dxw = 0.0, dyw = 0.0, dxv = 0.0, dyv = 0.0;
 #pragma unroll
     for (int kk = 0; kk < NP; ++kk) {
-      dxw += B.D[i * NP + kk] * ws[s * NP2 + kk + NP * j];
-      dyw += B.D[j * NP + kk] * ws[s * NP2 + i + NP * kk];
-      dxv += B.D[i * NP + kk] * vs[s * NP2 + kk + NP * j];
-      dyv += B.D[j * NP + kk] * vs[s * NP2 + i + NP * kk];
+      dxw += sD[i * NP + kk] * ws[s * NP2 + kk + NP * j];
+      dyw += sD[j * NP + kk] * ws[s * NP2 + i + NP * kk];
+      dxv += sD[i * NP + kk] * vs[s * NP2 + kk + NP * j];
+      dyv += sD[j * NP + kk] * vs[s * NP2 + i + NP * kk];
     }
-    const double kx = al * B.w[i] * B.w[j] * g.Jy * g.invJx;
-    const double ky = al * B.w[i] * B.w[j] * g.Jx * g.invJy;
-    const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+    const double kx = al * sW[i] * sW[j] * g.Jy * g.invJx;
+    const double ky = al * sW[i] * sW[j] * g.Jx * g.invJy;
+    const double cm = sW[i] * sW[j] * g.Jx * g.Jy;
     gx[s * NP2 + n] = kx * ir * dxw;
     gy[s * NP2 + n] = ky * ir * dyw;
     irb = cm * vown * wown + kx * dxv * dxw + ky * dyv * dyw;
@@ -369,8 +386,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
         const int ln = (dir == 0) ? kk + NP * j : i + NP * kk;
-        dnw += B.D[drow * NP + kk] * ws[sn * NP2 + ln];
-        dnv += B.D[drow * NP + kk] * vs[sn * NP2 + ln];
+        dnw += sD[drow * NP + kk] * ws[sn * NP2 + ln];
+        dnv += sD[drow * NP + kk] * vs[sn * NP2 + ln];
       }
       const double irn = irs[sn * NP2 + nnode];
       const double wn = ws[sn * NP2 + nnode], vn = vs[sn * NP2 + nnode];
@@ -392,10 +409,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       irb += -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
       if (own_L == L_chosen) irb += al * g.eta / h * jv * wf * jw;
     };
-    if (i == 0) face(false, sL, 0, g.invJx, g.dx, B.w[j] * g.Jy, &sxl[s * NP + j]);
-    if (i == NP - 1) face(true, sR, 0, g.invJx, g.dx, B.w[j] * g.Jy, &sxr[s * NP + j]);
-    if (j == 0) face(false, sB, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syb[s * NP + i]);
-    if (j == NP - 1) face(true, sT, 1, g.invJy, g.dy, B.w[i] * g.Jx, &syt[s * NP + i]);
+    if (i == 0) face(false, sL, 0, g.invJx, g.dx, sW[j] * g.Jy, &sxl[s * NP + j]);
+    if (i == NP - 1) face(true, sR, 0, g.invJx, g.dx, sW[j] * g.Jy, &sxr[s * NP + j]);
+    if (j == 0) face(false, sB, 1, g.invJy, g.dy, sW[i] * g.Jx, &syb[s * NP + i]);
+    if (j == NP - 1) face(true, sT, 1, g.invJy, g.dy, sW[i] * g.Jx, &syt[s * NP + i]);
     if constexpr (FIN) {
       // IGR source b = c alpha [(uxx+uyy)^2 + uxx^2 + 2 uxy uyx + uyy^2]; b-bar is r0-bar = -w
       // (PCG, Chebyshev) plus what the Jacobi/Chebyshev iterations accumulated
@@ -404,10 +421,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
-        uxx += B.D[i * NP + kk] * us[s * NP2 + kk + NP * j];
-        uyx += B.D[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
-        uxy += B.D[j * NP + kk] * us[s * NP2 + i + NP * kk];
-        uyy += B.D[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
+        uxx += sD[i * NP + kk] * us[s * NP2 + kk + NP * j];
+        uyx += sD[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
+        uxy += sD[j * NP + kk] * us[s * NP2 + i + NP * kk];
+        uyy += sD[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
       }
       uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
       const double f = bown * cm * al;
@@ -427,17 +444,17 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
 
   double y = 0.0, ab = 0.0;
   if (valid) {
-    const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+    const double cm = sW[i] * sW[j] * g.Jx * g.Jy;
     y = cm * wown * ir;
     double st = 0.0;
 #pragma unroll
     for (int kk = 0; kk < NP; ++kk) {
-      st += B.D[kk * NP + i] * gx[s * NP2 + kk + NP * j];
-      st += B.D[kk * NP + j] * gy[s * NP2 + i + NP * kk];
+      st += sD[kk * NP + i] * gx[s * NP2 + kk + NP * j];
+      st += sD[kk * NP + j] * gy[s * NP2 + i + NP * kk];
     }
     y += st;
-    y += sxl[s * NP + j] * B.D[i] + sxr[s * NP + j] * B.D[(NP - 1) * NP + i];
-    y += syb[s * NP + i] * B.D[j] + syt[s * NP + i] * B.D[(NP - 1) * NP + j];
+    y += sxl[s * NP + j] * sD[i] + sxr[s * NP + j] * sD[(NP - 1) * NP + i];
+    y += syb[s * NP + i] * sD[j] + syt[s * NP + i] * sD[(NP - 1) * NP + j];
     y += fc;
     // d/d alpha of <w, A v> = <v, K w>, K = (A - mass)/alpha
     if (al != 0.0) ab = vown * (y - cm * wown * ir) / al;
@@ -453,13 +470,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     if (tid == 0) a.abar_blk[blockIdx.x] += rv[0];
   } else if constexpr (MODE == RV_PCG_ITER) {
     double dv[4] = {0.0, 0.0, 0.0, 0.0};
+    double pbk = 0.0;
     if (valid) {
-      a.irbar[gme] = f_irb + irb;
-      a.imbar[gme] = f_imb + o_zb * o_R;
-      a.rbar_out[gme] = o_rb;
       const double xb = f_xb;
-      const double pbk = bk * o_pb + ak * xb + papb * f_apk + y;
-      a.pbar_out[gme] = pbk;
+      pbk = bk * o_pb + ak * xb + papb * f_apk + y;
       if (k >= 1) {
         const double pm = f_pm, apm = f_apm;
         const double iM = l_iM[0];      // own node: slot tid of the tile loads
@@ -470,10 +484,24 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       }
     }
     // the four dot products for the next iteration and this block's alpha-bar
-    // share one reduction (the alpha-bar total is not stored as a partial)
+    // share one reduction (the alpha-bar total is not stored as a partial);
+    // its barrier comes before the stores, which a barrier would wait for
     double dv5[5] = {dv[0], dv[1], dv[2], dv[3], ab};
     block_partials_store<5, 4>(dv5, red, a.part_out, gridDim.x, blockIdx.x);
-    if (tid == 0) a.abar_blk[blockIdx.x] += dv5[4];
+    if (tid == 0) a.abar_blk[blockIdx.x] = f_ab + dv5[4];
+    if (valid) {
+      a.irbar[gme] = f_irb + irb;
+      a.imbar[gme] = f_imb + o_zb * o_R;
+      a.rbar_out[gme] = o_rb;
+      a.pbar_out[gme] = pbk;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+      a.rs[RS::bbar(k)] = r_bb;
+      a.rs[RS::abar(k)] = r_abk;
+      a.rs[RS::papbar(k)] = papb;
+      a.rs[RS::rzbar_next(k)] = rzb;
+      a.rs[RS::carry(k)] = r_carry;
+    }
   } else if constexpr (MODE == RV_JAC_ITER) {
     if (valid) {
       const double xb1 = o_pb, iM = a.invM[gme];
@@ -505,7 +533,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     double ab_extra = 0.0;
     if (valid) {
       a.x0bar[gme] = a.xbar[gme] + y;    // x0-bar = x-bar - A r0-bar, w = -r0-bar
-      const double cm = B.w[i] * B.w[j] * g.Jx * g.Jy;
+      const double cm = sW[i] * sW[j] * g.Jx * g.Jy;
       // diag(A) reverse: M = cm/rho + x/y stiffness self-coupling + face terms
       const double iM = a.invM[gme];
       const double Mown = 1.0 / iM;
@@ -515,10 +543,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
         // x-stiffness of node (kk, j) couples to this node (i, j): alpha w_i w_j Jy/Jx D[i][kk]^2
-        const double Dx = B.D[i * NP + kk];
-        irbd += al * B.w[i] * B.w[j] * g.Jy * g.invJx * Dx * Dx * mbs[s * NP2 + kk + NP * j];
-        const double Dy = B.D[j * NP + kk];
-        irbd += al * B.w[i] * B.w[j] * g.Jx * g.invJy * Dy * Dy * mbs[s * NP2 + i + NP * kk];
+        const double Dx = sD[i * NP + kk];
+        irbd += al * sW[i] * sW[j] * g.Jy * g.invJx * Dx * Dx * mbs[s * NP2 + kk + NP * j];
+        const double Dy = sD[j * NP + kk];
+        irbd += al * sW[i] * sW[j] * g.Jx * g.invJy * Dy * Dy * mbs[s * NP2 + i + NP * kk];
       }
       auto dface = [&](bool own_L, int sn, int nnode, double invJ, double h, double wf, double Dself) {
         const double irn = irs[sn * NP2 + nnode], mbn = mbs[sn * NP2 + nnode];
@@ -527,16 +555,16 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
         const bool L_chosen = !(irR > irL);
         if (own_L == L_chosen) irbd += (mbo + mbn) * al * wf * g.eta / h;
       };
-      if (i == NP - 1) dface(true, sR, NP * j, g.invJx, g.dx, B.w[j] * g.Jy, B.D[(NP - 1) * NP + NP - 1]);
-      if (i == 0) dface(false, sL, (NP - 1) + NP * j, g.invJx, g.dx, B.w[j] * g.Jy, B.D[0]);
-      if (j == NP - 1) dface(true, sT, i, g.invJy, g.dy, B.w[i] * g.Jx, B.D[(NP - 1) * NP + NP - 1]);
-      if (j == 0) dface(false, sB, i + NP * (NP - 1), g.invJy, g.dy, B.w[i] * g.Jx, B.D[0]);
+      if (i == NP - 1) dface(true, sR, NP * j, g.invJx, g.dx, sW[j] * g.Jy, sD[(NP - 1) * NP + NP - 1]);
+      if (i == 0) dface(false, sL, (NP - 1) + NP * j, g.invJx, g.dx, sW[j] * g.Jy, sD[0]);
+      if (j == NP - 1) dface(true, sT, i, g.invJy, g.dy, sW[i] * g.Jx, sD[(NP - 1) * NP + NP - 1]);
+      if (j == 0) dface(false, sB, i + NP * (NP - 1), g.invJy, g.dy, sW[i] * g.Jx, sD[0]);
       // IGR source reverse: u-bar, v-bar gathered from G
       double ub = 0.0, vb = 0.0;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
-        ub += g.invJx * B.D[kk * NP + i] * Gxx[s * NP2 + kk + NP * j] + g.invJy * B.D[kk * NP + j] * Gxy[s * NP2 + i + NP * kk];
-        vb += g.invJx * B.D[kk * NP + i] * Gyx[s * NP2 + kk + NP * j] + g.invJy * B.D[kk * NP + j] * Gyy[s * NP2 + i + NP * kk];
+        ub += g.invJx * sD[kk * NP + i] * Gxx[s * NP2 + kk + NP * j] + g.invJy * sD[kk * NP + j] * Gxy[s * NP2 + i + NP * kk];
+        vb += g.invJx * sD[kk * NP + i] * Gyx[s * NP2 + kk + NP * j] + g.invJy * sD[kk * NP + j] * Gyy[s * NP2 + i + NP * kk];
       }
       const double rho = a.q[2 * N + gme];
       const double u = us[s * NP2 + n], v = vvs[s * NP2 + n];
@@ -545,10 +573,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
         double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
         for (int kk = 0; kk < NP; ++kk) {
-          uxx += B.D[i * NP + kk] * us[s * NP2 + kk + NP * j];
-          uyx += B.D[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
-          uxy += B.D[j * NP + kk] * us[s * NP2 + i + NP * kk];
-          uyy += B.D[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
+          uxx += sD[i * NP + kk] * us[s * NP2 + kk + NP * j];
+          uyx += sD[i * NP + kk] * vvs[s * NP2 + kk + NP * j];
+          uxy += sD[j * NP + kk] * us[s * NP2 + i + NP * kk];
+          uyy += sD[j * NP + kk] * vvs[s * NP2 + i + NP * kk];
         }
         uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
         const double dvg = uxx + uyy;
