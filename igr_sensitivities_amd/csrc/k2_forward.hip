@@ -429,6 +429,10 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
+  // D in LDS: read per lane after the first barrier, each access to the
+  // argument block would be a memory round trip there
+  __shared__ double sD[NP * NP];
+  const double bD = B.D[threadIdx.x < NP * NP ? threadIdx.x : 0];
   const double gam = a.gamma;
   TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
@@ -474,6 +478,7 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
     wsx[tid] = fabs(qo[0] / qo[2]) + c;
     wsy[tid] = fabs(qo[1] / qo[2]) + c;
   }
+  if (threadIdx.x < NP * NP) sD[threadIdx.x] = bD;
   __syncthreads();
 
   // volume term: dq = -(1/Jx) Dx Fx - (1/Jy) Dy Fy, x then y per k (hyperbolic.jl:55-66)
@@ -482,9 +487,9 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * B.D[i * NP + k] * Fx[v][s * NP2 + k + NP * j];
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * sD[i * NP + k] * Fx[v][s * NP2 + k + NP * j];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * B.D[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * sD[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
     }
     // face fluxes, one evaluation per face inside the tile
     if (i == NP - 1) {
