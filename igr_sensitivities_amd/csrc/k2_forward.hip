@@ -442,30 +442,37 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
   const size_t N = (size_t)g.Nex * g.Ney * NP2;
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
-  double qo[5] = {0.0, 0.0, 1.0, 1.0, 0.0};
-  double qn[4] = {0.0, 0.0, 0.0, 0.0};   // q^n of stages 2 and 3, loaded with the stage state
-  if (valid) {
+  // Own node and (threads below NH*NP) one halo face node: both sets of
+  // loads are issued, with no branch between them, before either is written
+  // to LDS (an invalid thread or empty halo slot reads node 0 and takes the
+  // default state instead); q^n of stages 2 and 3 comes with them.
+  static_assert(NH * NP <= NT, "one halo face node per thread");
+  const int hidx = tid;
+  size_t hgi = 0;
+  const bool hok = hidx < NH * NP && t.halo_face_global(hidx / NP, hidx % NP, hgi);
+  if (!hok) hgi = 0;
+  const double* qnp = a.mode >= 2 ? a.qn : a.q;
+  double lo[5], ln[4], lh[5];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) qo[v] = a.q[v * N + gme];
-    qo[4] = a.S[gme];
-    if (a.mode >= 2) {
+  for (int v = 0; v < 4; ++v) lo[v] = a.q[v * N + gme];
+  lo[4] = a.S[gme];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) qn[v] = a.qn[v * N + gme];
-    }
-  }
+  for (int v = 0; v < 4; ++v) ln[v] = qnp[v * N + gme];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) lh[v] = a.q[v * N + hgi];
+  lh[4] = a.S[hgi];
+  const double qdef[5] = {0.0, 0.0, 1.0, 1.0, 0.0};
+  double qo[5];
+  double qn[4];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qo[v] = valid ? lo[v] : qdef[v];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) qn[v] = (valid && a.mode >= 2) ? ln[v] : 0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
-  // halo face nodes of the neighbour elements
-  for (int idx = tid; idx < NH * NP; idx += NT) {
-    const int h = idx / NP, k = idx % NP;
-    size_t gi;
-    if (t.halo_face_global(h, k, gi)) {
+  if (hidx < NH * NP) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) hq[v][idx] = a.q[v * N + gi];
-      hq[4][idx] = a.S[gi];
-    } else {
-      hq[0][idx] = 0.0; hq[1][idx] = 0.0; hq[2][idx] = 1.0; hq[3][idx] = 1.0; hq[4][idx] = 0.0;
-    }
+    for (int v = 0; v < 5; ++v) hq[v][hidx] = hok ? lh[v] : qdef[v];
   }
   double fxo[4], fyo[4];
   flux_x2(gam, qo, fxo);
