@@ -112,6 +112,8 @@ def c5_leg(igr, configs, budget_slots=40):
     out = {"workload": "2D 256x256 p=3, Gaussian blast (peak 100, sigma 0.03) beside a counter-rotating Oseen "
                        "pair (core 0.05, vortex Mach 0.5), PCG-60, 400 SSP-RK3 steps; J = kinetic energy in a disc",
            "steps": ns}
+    out["tiles"] = {"sip": ctx.get_option(igr.core.OPT_TILE), "hyperbolic": ctx.get_option(igr.core.OPT_HTILE),
+                    "reverse": ctx.get_option(igr.core.OPT_RTILE)}
     try:
         ctx.set_state(cfg["q"])
         ctx.run_steps(2, dt, gam, al, n_iter, "pcg")

@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 
 SOLVERS = {"pcg": 0, "jacobi": 1, "chebyshev": 2}
-OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE = 0, 1, 2, 3
+OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE, OPT_RTILE = 0, 1, 2, 3, 4
 
 
 class DomainError(ArithmeticError):

@@ -31,6 +31,7 @@ void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
 int grid2_blocks(const Ctx& c);
+int grid2_rblocks(const Ctx& c);
 int grid2_hblocks(const Ctx& c);
 Geo2 geo2(const Ctx& c);
 SipArgs sip_base(const Ctx& c, double alpha);
@@ -749,7 +750,13 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
     HIPCHK(*c, hipMemGetInfo(&fr, &tot));
     const size_t per = (step_set_doubles(N, n_iter) + 64) * sizeof(double);
     const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes + pool_idle_bytes()));
-    const int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
+    int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
+    // the block grows only when it would at least double: growing it frees
+    // and obtains it again (about 30 ms per GB of a just-released block),
+    // while a few more kept steps save a few step recomputes (the free
+    // memory seen here can differ by a set between calls)
+    const int K_have = (int)run.W->sets.size();
+    if (K_have > 0 && K > K_have && K < 2 * K_have) K = K_have;
     if (K >= 1 && alloc_sets(*c, *run.W, K) == IGR_OK) run.kfirst = n_steps - 1 - K;
   }
   run.save(0);
@@ -758,7 +765,7 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
   if (c->dim == 1) {
     qb = run.W1->base.QBnext;
   } else {
-    const int nb = grid2_blocks(*c);
+    const int nb = grid2_rblocks(*c);
     launch_sum_blocks(*c, run.W->gblk, grid2_hblocks(*c), dsc + 3);
     launch_sum_blocks(*c, run.W->ablk, nb, dsc + 4);
     qb = run.W->QBnext;
@@ -835,7 +842,7 @@ int igr_apply_sip_vjp(igr_ctx* c, double alpha, const double* x, const double* r
   if (rc) return rc;
   // alpha-bar: <y-bar, K x> = (<y-bar, A x> - mass)/alpha, summed on the host from the block partials
   if (alphabar) {
-    std::vector<double> blk(grid2_blocks(*c));
+    std::vector<double> blk(grid2_rblocks(*c));
     HIPCHK(*c, hipMemcpy(blk.data(), W->ablk, blk.size() * sizeof(double), hipMemcpyDeviceToHost));
     double s = 0.0;
     for (double v : blk) s += v;
@@ -866,7 +873,7 @@ int igr_solve_elliptic_vjp(igr_ctx* c, double alpha, int n_iter, int solver, con
   launch_fill(*c, W->ablk, c->partials_n, 0.0);
   launch_fill(*c, W->dsc, 16, 0.0);
   solve2_rev(*c, *W, W->Q[0], W->Sg[0], W->Xbar, alpha, n_iter, solver, W->st[0], W->QBn, W->Scarry);
-  launch_sum_blocks(*c, W->ablk, grid2_blocks(*c), W->dsc + 4);
+  launch_sum_blocks(*c, W->ablk, grid2_rblocks(*c), W->dsc + 4);
   if (qbar) HIPCHK(*c, hipMemcpyAsync(qbar, W->QBn, 4 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (sigma0bar) HIPCHK(*c, hipMemcpyAsync(sigma0bar, W->Scarry, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   double ab = 0.0;

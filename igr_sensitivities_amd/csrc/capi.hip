@@ -224,14 +224,20 @@ void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
 // shape on MI355X at 16^2 .. 256^2: up to ~64^2 elements the launches are
 // latency-bound and 4x4 tiles are fastest; from ~128^2 the 8x4 tiles win
 // (fewer partial sums for every workgroup to read, less halo per node); from
-// ~256^2 the SIP family prefers 8x8.  The shape of a given mesh is fixed, so
-// repeated runs give bit-identical results; the autotuner may choose others.
+// ~256^2 the SIP family prefers 8x8.  Its reverse stays at 8x4 there: the
+// reverse PCG iteration holds 122 VGPRs, so a CU runs 16 of its waves with
+// either shape, and two 8-wave workgroups per CU hide each other's barriers
+// (the 256^2 gradient: 9.4 s with 8x8, 8.8 s with 8x4).  The shape of a given
+// mesh is fixed, so repeated runs give bit-identical results; the autotuner
+// may choose other forward shapes.
 void default_tiles(Ctx& c) {
   if (c.np != 4) return;
   const long long E = (long long)c.Nex * c.Ney;
   if (E < 96 * 96) { c.tile_x = 4; c.tile_y = 4; c.htile_x = 4; c.htile_y = 4; }
   else if (E < 192 * 192) { c.tile_x = 8; c.tile_y = 4; c.htile_x = 8; c.htile_y = 4; }
   else { c.tile_x = 8; c.tile_y = 8; c.htile_x = 8; c.htile_y = 4; }
+  c.rtile_x = c.tile_x == 8 ? 8 : c.tile_x;
+  c.rtile_y = c.tile_x == 8 ? 4 : c.tile_y;
 }
 
 // The autotuner: on the current state, time a 30-iteration PCG solve for
@@ -372,6 +378,13 @@ int igr_set_option(igr_ctx* c, int option, int value) {
     if (c->np == 4) set_tiles(*c, h ? c->tile_x : tx, h ? c->tile_y : ty, h ? tx : c->htile_x, h ? ty : c->htile_y);
     return IGR_OK;
   }
+  if (option == IGR_OPT_RTILE) {
+    // the reverse kernels are launched directly (no captured step holds them)
+    const int tx = value / 100, ty = value % 100;
+    if (!tile_ok(tx, ty, false)) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
+    if (c->np == 4) { c->rtile_x = tx; c->rtile_y = ty; }
+    return IGR_OK;
+  }
   if (option == IGR_OPT_AUTOTUNE) {
     if (c->dim != 2 || c->np != 4) return IGR_OK;   // one tile shape exists for the other cases
     int rc = begin_call(*c);
@@ -388,6 +401,7 @@ int igr_get_option(const igr_ctx* c, int option, int* value) {
   if (option == IGR_OPT_GRAPHS) *value = c->use_graphs;
   else if (option == IGR_OPT_TILE) *value = 100 * c->tile_x + c->tile_y;
   else if (option == IGR_OPT_HTILE) *value = 100 * c->htile_x + c->htile_y;
+  else if (option == IGR_OPT_RTILE) *value = 100 * c->rtile_x + c->rtile_y;
   else return IGR_EINVAL;
   return IGR_OK;
 }

@@ -67,11 +67,12 @@ struct Ctx {
   double* k1face = nullptr;   // 6*Ne face values
   double* k1args = nullptr;   // device copy of the 1D kernel's argument block (K1Args)
 
-  // 2D elements per workgroup (p = 3): tile_* for the SIP/solver kernels and
-  // their reverse (the kernels with grid reductions), htile_* for the
+  // 2D elements per workgroup (p = 3): tile_* for the SIP/solver kernels (the
+  // kernels with grid reductions), rtile_* for their reverse, htile_* for the
   // hyperbolic, tangent and objective kernels; chosen by a size table or by
-  // the autotuner (igr_set_option IGR_OPT_AUTOTUNE)
+  // the autotuner (igr_set_option IGR_OPT_AUTOTUNE, forward families only)
   int tile_x = 4, tile_y = 4;
+  int rtile_x = 4, rtile_y = 4;
   int htile_x = 4, htile_y = 4;
   int use_graphs = 1;
 

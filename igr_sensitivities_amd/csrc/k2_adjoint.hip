@@ -761,7 +761,7 @@ struct LaunchRev {
 
 void launch2_hyp_rev(const Ctx& c, const HypRevArgs& a) { IGR_HREV_DISPATCH(c.np, c.htile_x, c.htile_y, hyp(a, c.stream)); }
 void launch2_rev(const Ctx& c, int mode, const RevArgs& a) {
-  IGR_REV_DISPATCH(c.np, c.tile_x, c.tile_y, rev(mode, a, c.stream));
+  IGR_REV_DISPATCH(c.np, c.rtile_x, c.rtile_y, rev(mode, a, c.stream));
 }
 void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad) {
   IGR_HREV_DISPATCH(c.np, c.htile_x, c.htile_y, obj(a, grad, c.stream));

@@ -767,6 +767,14 @@ int grid2_blocks(const Ctx& c) {
   else if (c.np == 4) { tx = c.tile_x; ty = c.tile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
+// workgroups of one reverse SIP-family launch
+int grid2_rblocks(const Ctx& c) {
+  int tx = 4, ty = 4;
+  if (c.np == 2) { tx = 8; ty = 8; }
+  else if (c.np == 5) { tx = 2; ty = 4; }
+  else if (c.np == 4) { tx = c.rtile_x; ty = c.rtile_y; }
+  return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
+}
 void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit) {
   if (g_rec) { g_rec->recs.push_back({R_FINAL, 0, g_rec->fin.size()}); g_rec->fin.push_back({part, nb, sc, nit}); return; }
   hipLaunchKernelGGL(k2_pcg_final, dim3(1), dim3(256), 0, c.stream, part, nb, sc, nit);

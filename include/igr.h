@@ -56,16 +56,17 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
 /* Options:
  *  IGR_OPT_GRAPHS   1 = capture each SSP-RK3 step in a hipGraph and replay it,
  *                   0 = launch kernels directly;
- *  IGR_OPT_TILE     2D elements per workgroup of the SIP / solver kernels and
- *                   their reverse, tile_x*100 + tile_y (p = 3: 404, 804, 808, 202);
+ *  IGR_OPT_TILE     2D elements per workgroup of the SIP / solver kernels,
+ *                   tile_x*100 + tile_y (p = 3: 404, 804, 808, 202);
  *  IGR_OPT_HTILE    the same for the hyperbolic, tangent and objective kernels
  *                   (p = 3: 404, 804, 202);
- *  IGR_OPT_AUTOTUNE (value ignored) time the shapes on the current state and
- *                   keep the fastest of each family.
- * The defaults come from a size table measured by the autotuner.  The shapes
- * change results only through the summation order of the solver's dot
- * products. */
-enum igr_option { IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3 };
+ *  IGR_OPT_AUTOTUNE (value ignored) time the forward shapes on the current
+ *                   state and keep the fastest of each family;
+ *  IGR_OPT_RTILE    the same for the reverse SIP / solver kernels of the
+ *                   gradient and the VJPs (p = 3: 404, 804, 808, 202).
+ * The defaults come from a size table measured on MI355X.  The shapes change
+ * results only through the summation order of the solvers' dot products. */
+enum igr_option { IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4 };
 int igr_set_option(igr_ctx* ctx, int option, int value);
 int igr_get_option(const igr_ctx* ctx, int option, int* value);
 

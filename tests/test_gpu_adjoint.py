@@ -135,12 +135,14 @@ def test_gradient_alpha_zero_and_deterministic(orc):
     assert ab == 0.0
 
 
-@pytest.mark.parametrize("tile,htile", [(808, 804), (404, 404), (202, 202)])
-def test_gradient_dot_product_tiles(orc, tile, htile):
+@pytest.mark.parametrize("tile,htile,rtile", [(808, 804, 808), (808, 804, 804), (404, 404, 808), (202, 202, 202)])
+def test_gradient_dot_product_tiles(orc, tile, htile, rtile):
     nx, ny, ns = 12, 10, 2
     c = ctx2(3, nx, ny)
     c.set_option(igr.core.OPT_TILE, tile)
     c.set_option(igr.core.OPT_HTILE, htile)
+    c.set_option(igr.core.OPT_RTILE, rtile)
+    assert c.get_option(igr.core.OPT_RTILE) == rtile
     g = orc.Grid2D(3, nx, ny)
     q0 = configs.smooth_random2(3, nx, ny, seed=120, amp=0.3)
     gam, a = 1.4, 4e-3
@@ -152,6 +154,39 @@ def test_gradient_dot_product_tiles(orc, tile, htile):
     Jr, dJ, _, _ = g.jvp_objective_run(gam, 0.4, a, 0.3 * a, 15, orc.PCG, ns, dt, 3, q0, u, p4)
     close(J, Jr, 1e-12)
     close(dJ, np.vdot(qb, u) + 0.4 * gb + 0.3 * a * ab, 1e-10)
+
+
+def test_reverse_tile_defaults_and_rejects():
+    # the reverse kernels keep 8x4 workgroups where the forward SIP family
+    # takes 8x8 (from 192^2 elements); smaller meshes share the forward shape
+    for n, want in ((32, (404, 404)), (128, (804, 804)), (256, (808, 804))):
+        c = ctx2(3, n, n)
+        assert (c.get_option(igr.core.OPT_TILE), c.get_option(igr.core.OPT_RTILE)) == want
+    c = ctx2(3, 8, 8)
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_RTILE, 303)
+    assert c.get_option(igr.core.OPT_RTILE) == 404
+
+
+@pytest.mark.parametrize("solver", ["pcg", "jacobi", "chebyshev"])
+def test_gradient_reverse_tiles_agree(solver):
+    # the reverse shape changes only the summation order of the reverse dot
+    # products: J is bit-equal (same forward), the gradient agrees to rounding
+    nx, ny, ns = 20, 12, 3
+    q0 = configs.smooth_random2(3, nx, ny, seed=130, amp=0.3)
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
+    res = []
+    for rtile in (808, 804, 404, 202):
+        c = ctx2(3, nx, ny)
+        c.set_option(igr.core.OPT_TILE, 808)
+        c.set_option(igr.core.OPT_RTILE, rtile)
+        c.set_state(q0)
+        res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 12, solver, prm=(0.55, 0.5, 0.15, 0.0)))
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        np.testing.assert_allclose(r[1], res[0][1], rtol=0, atol=1e-11 * np.abs(res[0][1]).max())
+        close(r[2], res[0][2], 1e-10)
+        close(r[3], res[0][3], 1e-10)
 
 
 @pytest.mark.parametrize("solver", ["pcg", "chebyshev"])
