@@ -127,22 +127,32 @@ __device__ __forceinline__ void grid_prefetch(const double* part, int nb, double
 #pragma unroll
   for (int k = 0; k < NV; ++k) pre[k] = part[(size_t)k * nb + b];
 }
-template <int NV>
+template <int NV, int U = 1>
 __device__ __forceinline__ void grid_finish(const double* part, int nb, const double (&pre)[NV], double (&out)[NV]) {
   __shared__ double sg[NV * MAX_WAVES];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
-  // one pass over the remaining workgroups loads all NV partials together;
-  // each value sums over b in increasing order, from 0.0
+  const int bs = blockDim.x;
+  // one pass over the remaining workgroups loads all NV partials of U chunks
+  // together (a chunk past nb reads entry nb - 1 and is not added); each
+  // value sums over b in increasing order, from 0.0, whatever U is
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = (int)threadIdx.x < nb ? 0.0 + pre[k] : 0.0;
-  for (int b = threadIdx.x + blockDim.x; b < nb; b += blockDim.x) {
-    double v[NV];
+  for (int b0 = threadIdx.x + bs; b0 < nb; b0 += U * bs) {
+    double v[U][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = part[(size_t)k * nb + b];
+    for (int u = 0; u < U; ++u) {
+      const int b = min(b0 + u * bs, nb - 1);
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] += v[k];
+      for (int k = 0; k < NV; ++k) v[u][k] = part[(size_t)k * nb + b];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = b0 + u * bs < nb;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) acc[k] = in ? acc[k] + v[u][k] : acc[k];
+    }
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);

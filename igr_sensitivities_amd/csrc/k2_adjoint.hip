@@ -267,7 +267,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
     const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
     double d[4];
-    grid_finish<4>(a.part_in, gridDim.x, pre4, d);
+    grid_finish<4, 4>(a.part_in, gridDim.x, pre4, d);
     const double eps = 1e-30;
     const double bb = d[0];
     rzb = carry_next + bb / (rzk + eps);

@@ -39,7 +39,9 @@ namespace igr {
 // ------------------------------------------------------------------------
 // SIP family
 // ------------------------------------------------------------------------
-template <int NP, int TX, int TY, int MODE>
+// GU: chunks of the previous kernel's partial sums loaded together; a batched
+// launch sums one instance's few partials and keeps the registers instead
+template <int NP, int TX, int TY, int MODE, int GU = 8>
 __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool INIT = (MODE == M_INIT_PCG || MODE == M_INIT_CHEB || MODE == M_INIT_JAC);
@@ -114,7 +116,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     const int nit = a.n_iter, it = a.it;
     const double rz_prev = a.first ? 0.0 : a.sc[it - 1];
     double tot[1];
-    grid_finish<1>(a.part_in, nblk, pre, tot);
+    grid_finish<1, GU>(a.part_in, nblk, pre, tot);
     if (!a.first) beta = tot[0] / (rz_prev + 1e-30);
     sc_tot = tot[0];   // stored with the outputs, after the last barrier
     (void)nit;
@@ -343,12 +345,12 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs*
                                                                   const int2* __restrict__ work) {
   const int2 w = work[blockIdx.x];
   const SipArgs& a = tab[w.x];
-  k2_sip_body<NP, TX, TY, MODE>(a, w.y, tiles_of<TX, TY>(a.g));
+  k2_sip_body<NP, TX, TY, MODE, 1>(a, w.y, tiles_of<TX, TY>(a.g));
 }
 
 
 // x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
-template <int NP, int TX, int TY>
+template <int NP, int TX, int TY, int GU = 8>
 __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
   __shared__ double red[NT / WAVE + 1];
@@ -367,7 +369,7 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
   const double x = a.xout[gi], p = a.pold[gi], r = rin[gi], ap = a.y[gi], im = a.invM[gi];
   const double rz_it = sc[it];
   double tot[1];
-  grid_finish<1>(a.part_in, nblk, pre, tot);
+  grid_finish<1, GU>(a.part_in, nblk, pre, tot);
   const double ak = rz_it / (tot[0] + 1e-30);
   const double xn = x + ak * p;
   const double rn = r - ak * ap;
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b_batch(const SipArg
                                                                     const int2* __restrict__ work) {
   const int2 w = work[blockIdx.x];
   const SipArgs& a = tab[w.x];
-  k2_pcg_b_body<NP, TX, TY>(a, w.y, tiles_of<TX, TY>(a.g));
+  k2_pcg_b_body<NP, TX, TY, 1>(a, w.y, tiles_of<TX, TY>(a.g));
 }
 
 

@@ -171,8 +171,10 @@ def test_reverse_tile_defaults_and_rejects():
 @pytest.mark.parametrize("solver", ["pcg", "jacobi", "chebyshev"])
 def test_gradient_reverse_tiles_agree(solver):
     # the reverse shape changes only the summation order of the reverse dot
-    # products: J is bit-equal (same forward), the gradient agrees to rounding
-    nx, ny, ns = 20, 12, 3
+    # products: J is bit-equal (same forward), the gradient agrees to rounding;
+    # with 2x2 tiles the 120 workgroups outnumber the 64 threads that sum
+    # their partials, so the chunked partial-sum loop runs too
+    nx, ny, ns = 24, 20, 3
     q0 = configs.smooth_random2(3, nx, ny, seed=130, amp=0.3)
     dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
     res = []
