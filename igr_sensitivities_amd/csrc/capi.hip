@@ -225,7 +225,7 @@ void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
 // latency-bound and 4x4 tiles are fastest; from ~128^2 the 8x4 tiles win
 // (fewer partial sums for every workgroup to read, less halo per node); from
 // ~256^2 the SIP family prefers 8x8.  Its reverse stays at 8x4 there: the
-// reverse PCG iteration holds 122 VGPRs, so a CU runs 16 of its waves with
+// reverse PCG iteration holds 116 VGPRs, so a CU runs 16 of its waves with
 // either shape, and two 8-wave workgroups per CU hide each other's barriers
 // (the 256^2 gradient: 9.4 s with 8x8, 8.8 s with 8x4).  The shape of a given
 // mesh is fixed, so repeated runs give bit-identical results; the autotuner
