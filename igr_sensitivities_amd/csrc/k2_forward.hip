@@ -48,7 +48,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   __shared__ double vs[NS * NP2];
   __shared__ double irs[NS * NP2];
   __shared__ double gx[T * NP2], gy[T * NP2];
-  __shared__ double sxl[T * NP], sxr[T * NP], syb[T * NP], syt[T * NP];
+  __shared__ double fP[T * 4 * NP], fQ[T * 4 * NP], fS[T * 4 * NP];   // face-node terms
+  __shared__ double fG[INIT ? T * 4 * NP : 1];                        // penalty coefficients
   __shared__ double us[INIT ? T * NP2 : 1], ws_[INIT ? T * NP2 : 1];
   __shared__ double red[4 * (NT / WAVE + 1)];
 
@@ -160,8 +161,6 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   const double al = a.alpha;
   const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
   const double vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
-  double fc = 0.0;        // this node's own face terms (consistency + penalty)
-  double sigx_l = 0.0, sigx_r = 0.0, sigy_b = 0.0, sigy_t = 0.0;
   double bval = 0.0;
   if (valid) {
     double dxi = 0.0, deta = 0.0;
@@ -172,58 +171,6 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     }
     gx[s * NP2 + n] = al * sW[i] * sW[j] * g.Jy * g.invJx * ir * dxi;
     gy[s * NP2 + n] = al * sW[i] * sW[j] * g.Jx * g.invJy * ir * deta;
-    if (i == 0) {            // x-face between left neighbour (L) and this element (R)
-      const double xL = vs[sL * NP2 + (NP - 1) + NP * j];
-      const double irL = irs[sL * NP2 + (NP - 1) + NP * j];
-      double dL = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dL += sD[(NP - 1) * NP + k] * vs[sL * NP2 + k + NP * j];
-      const double jump = xL - vown, wf = sW[j] * g.Jy;
-      const double avg = g.invJx * (irL * dL + ir * dxi) / 2.0;
-      sigx_l = g.eta * fmax(irL, ir) / g.dx;
-      fc += al * avg * wf;
-      fc -= al * sigx_l * jump * wf;
-      sxl[s * NP + j] = -al * (ir / 2.0) * g.invJx * jump * wf;
-    }
-    if (i == NP - 1) {       // x-face between this element (L) and right neighbour (R)
-      const double xR = vs[sR * NP2 + NP * j];
-      const double irR = irs[sR * NP2 + NP * j];
-      double dR = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dR += sD[k] * vs[sR * NP2 + k + NP * j];
-      const double jump = vown - xR, wf = sW[j] * g.Jy;
-      const double avg = g.invJx * (ir * dxi + irR * dR) / 2.0;
-      sigx_r = g.eta * fmax(ir, irR) / g.dx;
-      fc += -al * avg * wf;
-      fc += al * sigx_r * jump * wf;
-      sxr[s * NP + j] = -al * (ir / 2.0) * g.invJx * jump * wf;
-    }
-    if (j == 0) {            // y-face between bottom neighbour (L) and this element (R)
-      const double xL = vs[sB * NP2 + i + NP * (NP - 1)];
-      const double irL = irs[sB * NP2 + i + NP * (NP - 1)];
-      double dL = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dL += sD[(NP - 1) * NP + k] * vs[sB * NP2 + i + NP * k];
-      const double jump = xL - vown, wf = sW[i] * g.Jx;
-      const double avg = g.invJy * (irL * dL + ir * deta) / 2.0;
-      sigy_b = g.eta * fmax(irL, ir) / g.dy;
-      fc += al * avg * wf;
-      fc -= al * sigy_b * jump * wf;
-      syb[s * NP + i] = -al * (ir / 2.0) * g.invJy * jump * wf;
-    }
-    if (j == NP - 1) {       // y-face between this element (L) and top neighbour (R)
-      const double xR = vs[sT * NP2 + i];
-      const double irR = irs[sT * NP2 + i];
-      double dR = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) dR += sD[k] * vs[sT * NP2 + i + NP * k];
-      const double jump = vown - xR, wf = sW[i] * g.Jx;
-      const double avg = g.invJy * (ir * deta + irR * dR) / 2.0;
-      sigy_t = g.eta * fmax(ir, irR) / g.dy;
-      fc += -al * avg * wf;
-      fc += al * sigy_t * jump * wf;
-      syt[s * NP + i] = -al * (ir / 2.0) * g.invJy * jump * wf;
-    }
     if constexpr (INIT) {    // IGR source R = alpha[(div u)^2 + tr((Du)^2)]
       double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
@@ -240,17 +187,63 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       if (a.b_in) bval = a.b_in[gme];
     }
   }
-  if (!valid) {
-    // keep LDS reads of never-written face factors deterministic
-    if (i == 0) sxl[s * NP + j] = 0.0;
-    if (i == NP - 1) sxr[s * NP + j] = 0.0;
-    if (j == 0) syb[s * NP + i] = 0.0;
-    if (j == NP - 1) syt[s * NP + i] = 0.0;
+  // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
+  // faces, NP nodes each) and NP^2 threads: thread n computes the terms of
+  // face nodes n, n + NP^2, ..., so a wave runs every face in one pass with
+  // all its lanes (a per-face branch would run four times with a quarter of
+  // them).  Each face node f = F NP + q gets its consistency term P, penalty
+  // term Q and symmetry factor S in LDS; the owner node adds its own after
+  // the barrier.  The owner's normal derivative is recomputed from the staged
+  // tile.  Faces F = 0, 2 (left, bottom) have the owner on their right side,
+  // F = 1, 3 on their left.
+  constexpr int NF = 4 * NP, NFR = (NF + NP2 - 1) / NP2;
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int f = n + r * NP2;
+    if (f >= NF) continue;
+    double P = 0.0, Q = 0.0, S = 0.0, G = 0.0;
+    if (valid) {
+      const int F = f / NP, q = f - F * NP;
+      const bool xf = F < 2, ownL = (F & 1) != 0;
+      const int io = xf ? (ownL ? NP - 1 : 0) : q, jo = xf ? q : (ownL ? NP - 1 : 0);
+      const int ion = xf ? (ownL ? 0 : NP - 1) : io, jon = xf ? jo : (ownL ? 0 : NP - 1);
+      const int sN = F == 0 ? sL : F == 1 ? sR : F == 2 ? sB : sT;
+      const int dro = xf ? io : jo, drn = xf ? ion : jon, stp = xf ? 1 : NP;
+      const int bo = s * NP2 + (xf ? NP * jo : io), bn = sN * NP2 + (xf ? NP * jon : ion);
+      double dno = 0.0, dnn = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        dno += sD[dro * NP + k] * vs[bo + stp * k];
+        dnn += sD[drn * NP + k] * vs[bn + stp * k];
+      }
+      const double vo = vs[s * NP2 + io + NP * jo], iro = irs[s * NP2 + io + NP * jo];
+      const double xn = vs[sN * NP2 + ion + NP * jon], irn = irs[sN * NP2 + ion + NP * jon];
+      const double wf = xf ? sW[jo] * g.Jy : sW[io] * g.Jx;
+      const double invJ = xf ? g.invJx : g.invJy, h = xf ? g.dx : g.dy;
+      const double jump = ownL ? vo - xn : xn - vo;
+      const double irLs = ownL ? iro : irn, irRs = ownL ? irn : iro;
+      const double dLs = ownL ? dno : dnn, dRs = ownL ? dnn : dno;
+      const double avg = invJ * (irLs * dLs + irRs * dRs) / 2.0;
+      G = g.eta * fmax(irLs, irRs) / h;
+      P = al * avg * wf;
+      Q = al * G * jump * wf;
+      S = -al * (iro / 2.0) * invJ * jump * wf;
+    }
+    fP[s * NF + f] = P;
+    fQ[s * NF + f] = Q;
+    fS[s * NF + f] = S;
+    if constexpr (INIT) fG[s * NF + f] = G;
   }
   __syncthreads();
 
+  const int fb = s * NF;
   double y = 0.0;
   if (valid) {
+    double fc = 0.0;        // this node's own face terms (consistency + penalty)
+    if (i == 0) { fc += fP[fb + j]; fc -= fQ[fb + j]; }
+    if (i == NP - 1) { fc -= fP[fb + NP + j]; fc += fQ[fb + NP + j]; }
+    if (j == 0) { fc += fP[fb + 2 * NP + i]; fc -= fQ[fb + 2 * NP + i]; }
+    if (j == NP - 1) { fc -= fP[fb + 3 * NP + i]; fc += fQ[fb + 3 * NP + i]; }
     y = sW[i] * sW[j] * g.Jx * g.Jy * vown * ir;
     double st = 0.0;
 #pragma unroll
@@ -259,8 +252,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       st += sD[k * NP + j] * gy[s * NP2 + i + NP * k];
     }
     y += st;
-    y += sxl[s * NP + j] * sD[i] + sxr[s * NP + j] * sD[(NP - 1) * NP + i];
-    y += syb[s * NP + i] * sD[j] + syt[s * NP + i] * sD[(NP - 1) * NP + j];
+    y += fS[fb + j] * sD[i] + fS[fb + NP + j] * sD[(NP - 1) * NP + i];
+    y += fS[fb + 2 * NP + i] * sD[j] + fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j];
     y += fc;
   }
 
@@ -282,10 +275,10 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         const double D2 = sD[k * NP + j] * sD[k * NP + j];
         Md += al * sW[i] * sW[k] * g.Jx * g.invJy * D2 * irs[s * NP2 + i + NP * k];
       }
-      if (i == NP - 1) Md += (-al * g.invJx * sD[(NP - 1) * NP + NP - 1] * ir + al * sigx_r) * (sW[j] * g.Jy);
-      if (i == 0) Md += (al * g.invJx * sD[0] * ir + al * sigx_l) * (sW[j] * g.Jy);
-      if (j == NP - 1) Md += (-al * g.invJy * sD[(NP - 1) * NP + NP - 1] * ir + al * sigy_t) * (sW[i] * g.Jx);
-      if (j == 0) Md += (al * g.invJy * sD[0] * ir + al * sigy_b) * (sW[i] * g.Jx);
+      if (i == NP - 1) Md += (-al * g.invJx * sD[(NP - 1) * NP + NP - 1] * ir + al * fG[fb + NP + j]) * (sW[j] * g.Jy);
+      if (i == 0) Md += (al * g.invJx * sD[0] * ir + al * fG[fb + j]) * (sW[j] * g.Jy);
+      if (j == NP - 1) Md += (-al * g.invJy * sD[(NP - 1) * NP + NP - 1] * ir + al * fG[fb + 3 * NP + i]) * (sW[i] * g.Jx);
+      if (j == 0) Md += (al * g.invJy * sD[0] * ir + al * fG[fb + 2 * NP + i]) * (sW[i] * g.Jx);
       const double iM = 1.0 / Md;
       o_iM = iM;
       if (!(ir > 0.0) || !isfinite(ir)) flag_domain(a.flag);
