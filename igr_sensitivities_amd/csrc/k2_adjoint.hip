@@ -184,7 +184,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   __shared__ double vs[NS * NP2];   // vector v of the forward product A v
   __shared__ double irs[NS * NP2];
   __shared__ double gx[T * NP2], gy[T * NP2];
-  __shared__ double sxl[T * NP], sxr[T * NP], syb[T * NP], syt[T * NP];
+  __shared__ double fC[T * 4 * NP], fS[T * 4 * NP], fR[T * 4 * NP];   // face-node terms
   __shared__ double mbs[FIN ? NS * NP2 : 1];                 // M-bar
   __shared__ double us[FIN ? T * NP2 : 1], vvs[FIN ? T * NP2 : 1];
   __shared__ double Gxx[FIN ? T * NP2 : 1], Gxy[FIN ? T * NP2 : 1], Gyx[FIN ? T * NP2 : 1], Gyy[FIN ? T * NP2 : 1];
@@ -376,43 +376,6 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     gx[s * NP2 + n] = kx * ir * dxw;
     gy[s * NP2 + n] = ky * ir * dyw;
     irb = cm * vown * wown + kx * dxv * dxw + ky * dyv * dyw;
-    // one face: own node is L (own_L) or R; neighbour slot sn, neighbour node nn,
-    // normal-derivative row of the neighbour along direction dir
-    auto face = [&](bool own_L, int sn, int dir, double invJ, double h, double wf, double* symf) {
-      const int nnode = (dir == 0) ? (own_L ? NP * j : (NP - 1) + NP * j)
-                                   : (own_L ? i : i + NP * (NP - 1));
-      const int drow = own_L ? 0 : NP - 1;        // derivative row of the neighbour's face node
-      double dnw = 0.0, dnv = 0.0;
-#pragma unroll
-      for (int kk = 0; kk < NP; ++kk) {
-        const int ln = (dir == 0) ? kk + NP * j : i + NP * kk;
-        dnw += sD[drow * NP + kk] * ws[sn * NP2 + ln];
-        dnv += sD[drow * NP + kk] * vs[sn * NP2 + ln];
-      }
-      const double irn = irs[sn * NP2 + nnode];
-      const double wn = ws[sn * NP2 + nnode], vn = vs[sn * NP2 + nnode];
-      const double dow = (dir == 0) ? dxw : dyw, dov = (dir == 0) ? dxv : dyv;
-      const double irL = own_L ? ir : irn, irR = own_L ? irn : ir;
-      const double jw = own_L ? (wown - wn) : (wn - wown);
-      const double jv = own_L ? (vown - vn) : (vn - vown);
-      const double dLw = own_L ? dow : dnw, dRw = own_L ? dnw : dow;
-      const double dLv = own_L ? dov : dnv, dRv = own_L ? dnv : dov;
-      const double sig = g.eta * fmax(irL, irR) / h;
-      // apply A to w: this node's consistency + penalty, and the symmetry factor
-      const double avg = invJ * (irL * dLw + irR * dRw) / 2.0;
-      if (own_L) { fc += -al * avg * wf; fc += al * sig * jw * wf; }
-      else { fc += al * avg * wf; fc -= al * sig * jw * wf; }
-      *symf = -al * (ir / 2.0) * invJ * jw * wf;
-      // d/d(1/rho_own) of the face's bilinear form in (w, v)
-      const bool L_chosen = !(irR > irL);
-      const double dv_own = own_L ? dLv : dRv, dw_own = own_L ? dLw : dRw;
-      irb += -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
-      if (own_L == L_chosen) irb += al * g.eta / h * jv * wf * jw;
-    };
-    if (i == 0) face(false, sL, 0, g.invJx, g.dx, sW[j] * g.Jy, &sxl[s * NP + j]);
-    if (i == NP - 1) face(true, sR, 0, g.invJx, g.dx, sW[j] * g.Jy, &sxr[s * NP + j]);
-    if (j == 0) face(false, sB, 1, g.invJy, g.dy, sW[i] * g.Jx, &syb[s * NP + i]);
-    if (j == NP - 1) face(true, sT, 1, g.invJy, g.dy, sW[i] * g.Jx, &syt[s * NP + i]);
     if constexpr (FIN) {
       // IGR source b = c alpha [(uxx+uyy)^2 + uxx^2 + 2 uxy uyx + uyy^2]; b-bar is r0-bar = -w
       // (PCG, Chebyshev) plus what the Jacobi/Chebyshev iterations accumulated
@@ -434,16 +397,71 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       Gyx[s * NP2 + n] = f * 2.0 * uxy;
     }
   }
-  if (!valid) {
-    if (i == 0) sxl[s * NP + j] = 0.0;
-    if (i == NP - 1) sxr[s * NP + j] = 0.0;
-    if (j == 0) syb[s * NP + i] = 0.0;
-    if (j == NP - 1) syt[s * NP + i] = 0.0;
+  // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
+  // faces, NP nodes each) and NP^2 threads: thread n computes the terms of
+  // face nodes n, n + NP^2, ..., so a wave runs every face in one pass with
+  // all its lanes.  Face node f = F NP + q leaves in LDS its part of the
+  // owner's A w (consistency + penalty), its symmetry factor, and its part of
+  // d/d(1/rho_owner) of the face's bilinear form in (w, v); the owner adds its
+  // own after the barrier.  Faces F = 0, 2 (left, bottom) have the owner on
+  // their right side, F = 1, 3 on their left.
+  constexpr int NF = 4 * NP, NFR = (NF + NP2 - 1) / NP2;
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int f = n + r * NP2;
+    if (f >= NF) continue;
+    double C = 0.0, Sy = 0.0, Rb = 0.0;
+    if (valid) {
+      const int F = f / NP, q = f - F * NP;
+      const bool xf = F < 2, own_L = (F & 1) != 0;
+      const int io = xf ? (own_L ? NP - 1 : 0) : q, jo = xf ? q : (own_L ? NP - 1 : 0);
+      const int ion = xf ? (own_L ? 0 : NP - 1) : io, jon = xf ? jo : (own_L ? 0 : NP - 1);
+      const int sN = F == 0 ? sL : F == 1 ? sR : F == 2 ? sB : sT;
+      const int dro = xf ? io : jo, drn = xf ? ion : jon, stp = xf ? 1 : NP;
+      const int bo = s * NP2 + (xf ? NP * jo : io), bn = sN * NP2 + (xf ? NP * jon : ion);
+      double dow = 0.0, dov = 0.0, dnw = 0.0, dnv = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) {
+        const double Do = sD[dro * NP + kk], Dn = sD[drn * NP + kk];
+        dow += Do * ws[bo + stp * kk];
+        dov += Do * vs[bo + stp * kk];
+        dnw += Dn * ws[bn + stp * kk];
+        dnv += Dn * vs[bn + stp * kk];
+      }
+      const int oo = s * NP2 + io + NP * jo, on = sN * NP2 + ion + NP * jon;
+      const double wo = ws[oo], vo = vs[oo], iro = irs[oo];
+      const double wn = ws[on], vn = vs[on], irn = irs[on];
+      const double wf = xf ? sW[jo] * g.Jy : sW[io] * g.Jx;
+      const double invJ = xf ? g.invJx : g.invJy, h = xf ? g.dx : g.dy;
+      const double irL = own_L ? iro : irn, irR = own_L ? irn : iro;
+      const double jw = own_L ? (wo - wn) : (wn - wo);
+      const double jv = own_L ? (vo - vn) : (vn - vo);
+      const double dLw = own_L ? dow : dnw, dRw = own_L ? dnw : dow;
+      const double dLv = own_L ? dov : dnv, dRv = own_L ? dnv : dov;
+      const double sig = g.eta * fmax(irL, irR) / h;
+      // apply A to w: the owner's consistency + penalty, and the symmetry factor
+      const double avg = invJ * (irL * dLw + irR * dRw) / 2.0;
+      C = own_L ? (-al * avg * wf + al * sig * jw * wf) : (al * avg * wf - al * sig * jw * wf);
+      Sy = -al * (iro / 2.0) * invJ * jw * wf;
+      // d/d(1/rho_owner) of the face's bilinear form in (w, v)
+      const bool L_chosen = !(irR > irL);
+      const double dv_own = own_L ? dLv : dRv, dw_own = own_L ? dLw : dRw;
+      Rb = -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
+      if (own_L == L_chosen) Rb += al * g.eta / h * jv * wf * jw;
+    }
+    fC[s * NF + f] = C;
+    fS[s * NF + f] = Sy;
+    fR[s * NF + f] = Rb;
   }
   __syncthreads();
 
+  const int fb = s * NF;
   double y = 0.0, ab = 0.0;
   if (valid) {
+    if (i == 0) { fc += fC[fb + j]; irb += fR[fb + j]; }
+    if (i == NP - 1) { fc += fC[fb + NP + j]; irb += fR[fb + NP + j]; }
+    if (j == 0) { fc += fC[fb + 2 * NP + i]; irb += fR[fb + 2 * NP + i]; }
+    if (j == NP - 1) { fc += fC[fb + 3 * NP + i]; irb += fR[fb + 3 * NP + i]; }
     const double cm = sW[i] * sW[j] * g.Jx * g.Jy;
     y = cm * wown * ir;
     double st = 0.0;
@@ -453,8 +471,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       st += sD[kk * NP + j] * gy[s * NP2 + i + NP * kk];
     }
     y += st;
-    y += sxl[s * NP + j] * sD[i] + sxr[s * NP + j] * sD[(NP - 1) * NP + i];
-    y += syb[s * NP + i] * sD[j] + syt[s * NP + i] * sD[(NP - 1) * NP + j];
+    y += fS[fb + j] * sD[i] + fS[fb + NP + j] * sD[(NP - 1) * NP + i];
+    y += fS[fb + 2 * NP + i] * sD[j] + fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j];
     y += fc;
     // d/d alpha of <w, A v> = <v, K w>, K = (A - mass)/alpha
     if (al != 0.0) ab = vown * (y - cm * wown * ir) / al;
