@@ -136,6 +136,28 @@ def test_one_step_and_run_2d(orc, solver):
     assert rel(ST, STr) < 1e-9
 
 
+def test_default_tiles_with_more_workgroups_than_threads(orc):
+    # 80x80 elements take the default 4x4 tiles: 400 workgroups of 256
+    # threads, so every workgroup sums the previous kernel's partials in more
+    # than one chunk (the chunked loads of the grid total run)
+    c = ctx2(3, 80, 80)
+    assert c.get_option(igr.core.OPT_TILE) == 404
+    g = orc.Grid2D(3, 80, 80)
+    q0 = configs.smooth_random2(3, 80, 80, seed=140, amp=0.3)
+    gam, a = 1.4, configs.alpha_of(1 / 80, 3)
+    dt = 0.4 * (1 / 80) / (7 * configs.max_wavespeed2(q0, gam))
+    c.set_state(q0)
+    c.run_steps(1, dt, gam, a, 30, "pcg")
+    q1, S1 = c.get_state()
+    q1r, S1r = g.run_steps(gam, a, 30, orc.PCG, 1, dt, q0)
+    assert rel(q1, q1r) < 1e-12
+    # Sigma is an inexact 30-iteration PCG iterate after three warm-started
+    # solves; on 102,400 nodes the dot products' summation order (a tree here,
+    # serial in the oracle) shows in it at about 1e-11.  A partial-sum chunk
+    # left out or added twice would move alpha by tens of percent.
+    assert rel(S1, S1r) < 1e-10
+
+
 def test_graph_replay_matches_eager_and_is_deterministic():
     # several replays of the captured step, with eager work and syncs between them,
     # must equal the eager launches bit for bit
