@@ -16,6 +16,7 @@
 namespace orc {
 thread_local int g_domain_error = 0;
 thread_local int g_adj_mutation = 0;
+thread_local int g_fwd_mutation = 0;
 }
 using namespace orc;
 
@@ -310,6 +311,7 @@ int orc1_jvp_sensitivity_run(G1ARGS, double g, double alpha, int n_iter, int sol
 // Snapshot arrays hold (n_steps + 1) records of (state, Sigma): 5N doubles in
 // 2D, 4N in 1D; `cap` is the number of records the caller allocated.
 void orc_set_adj_mutation(int m) { g_adj_mutation = m; }
+void orc_set_fwd_mutation(int m) { g_fwd_mutation = m; }
 
 int orc2_adj_hyp_rhs_local(G2ARGS, double g, const double* q, const double* S, const double* a, double* da) {
   MAKE_G2; adj_hyp_rhs_local2(G, g, q, S, a, da); return finish(0);

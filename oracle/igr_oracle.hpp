@@ -79,6 +79,14 @@ inline double tan_(const Dual& x) { return x.d; }
 // records it and carries on with NaN; the C wrapper reports it.
 extern thread_local int g_domain_error;
 
+// Test-only switch that plants one deliberate mistake in the forward
+// restatement (0 = none), so the tests can show that their checks of the
+// solvers and the y-direction flux reject it:
+//   1  damped Jacobi with omega = 0.5 instead of 2/3
+//   2  Chebyshev lambda_max = 2.2 at p = 3 instead of 2.1
+//   3  polytropic_flux_y with its two momentum components swapped
+extern thread_local int g_fwd_mutation;
+
 inline double sqrt_(double x) {
   if (x < 0.0) g_domain_error = 1;
   return std::sqrt(x);
@@ -221,6 +229,7 @@ template <class T> inline void flux_y2(T g, T mx, T my, T r, T E, T S, T f[4]) {
   T pb = P + S;
   T uy = my / r;
   f[0] = mx * uy; f[1] = my * uy + pb; f[2] = my; f[3] = (E + pb) * uy;
+  if (g_fwd_mutation == 3) { T t = f[0]; f[0] = f[1]; f[1] = t; }
 }
 // euler.jl:193-214
 template <class T> inline T sound2(T g, T mx, T my, T r, T E) {
@@ -518,6 +527,7 @@ void chebyshev_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const 
   std::vector<T> Ax(N), r(N), z(N), d(N);
   static const double lmax_tab[4] = {1.5, 2.0, 2.1, 2.2};
   double lmin = 1.0, lmax = lmax_tab[std::min(p, 4) - 1];
+  if (g_fwd_mutation == 2 && p == 3) lmax = 2.2;
   double theta = (lmax + lmin) / 2.0;
   double delta = (lmax - lmin) / 2.0;
   double sigma = theta / std::max(delta, 1e-10);
@@ -551,7 +561,7 @@ template <class T, class Apply>
 int run_solver(int solver, Apply A, std::vector<T>& x, const std::vector<T>& b,
                const std::vector<T>& Md, int n_iter, int p, SolverTrace* tr) {
   if (solver == PCG) pcg_fixed(A, x, b, Md, n_iter, tr);
-  else if (solver == JACOBI) jacobi_fixed(A, x, b, Md, n_iter, 2.0 / 3.0);
+  else if (solver == JACOBI) jacobi_fixed(A, x, b, Md, n_iter, g_fwd_mutation == 1 ? 0.5 : 2.0 / 3.0);
   else if (solver == CHEBYSHEV) chebyshev_fixed(A, x, b, Md, n_iter, p);
   else return -1;   // error("Unknown solver: ...")
   return 0;

@@ -86,6 +86,24 @@ class adj_mutation:
         return False
 
 
+class fwd_mutation:
+    """Context manager that plants one deliberate mistake in the oracle's
+    forward map (1: Jacobi omega 0.5; 2: Chebyshev lambda_max 2.2 at p = 3;
+    3: flux_y with its momentum components swapped), for tests that show
+    their checks reject it."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().orc_set_fwd_mutation(C.c_int(self.m))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_fwd_mutation(C.c_int(0))
+        return False
+
+
 class DomainError(OracleError):
     """Negative pressure under sqrt (Julia DomainError)."""
 
