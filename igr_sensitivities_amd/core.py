@@ -50,6 +50,26 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _sized(a, count, name):
+    """a as contiguous float64 holding exactly `count` values: the C side reads
+    and writes that many through the raw pointer, so a wrong size is refused
+    here rather than overrunning the array."""
+    a = _f64(a)
+    if a.size != count:
+        raise ValueError(f"{name}: expected {count} values, got {a.size} (shape {a.shape})")
+    return a
+
+
+def _prm4(prm):
+    """Objective parameters as the 4 doubles the C side reads (zero-padded)."""
+    if prm is None:
+        return None
+    p = np.ascontiguousarray(prm, dtype=np.float64).ravel()
+    if p.size > 4:
+        raise ValueError(f"objective parameters: at most 4 values, got {p.size}")
+    return np.concatenate([p, np.zeros(4 - p.size)])
+
+
 class DGBasis:
     """Polynomial degree and LGL node count (DGBasis, IGR_adjoints_2d/basis.jl:7-12)."""
 
@@ -105,6 +125,7 @@ class Context:
         self.h = h
         self.shape = mesh.shape
         self.nv = mesh.nv
+        self.N = int(np.prod(mesh.shape))
 
     def close(self):
         if self.h:
@@ -144,8 +165,8 @@ class Context:
 
     # state transfer
     def set_state(self, q, sigma=None):
-        q = _f64(q)
-        s = None if sigma is None else _f64(sigma)
+        q = _sized(q, self.nv * self.N, "state q")
+        s = None if sigma is None else _sized(sigma, self.N, "Sigma")
         self.check(self._L.igr_set_state(self.h, _p(q), _p(s)))
 
     def get_state(self):
@@ -156,25 +177,25 @@ class Context:
 
     # operators on host arrays
     def hyperbolic_rhs(self, gamma, q, sigma):
-        q, sigma = _f64(q), _f64(sigma)
+        q, sigma = _sized(q, self.nv * self.N, "state q"), _sized(sigma, self.N, "Sigma")
         dq = np.empty_like(q)
         self.check(self._L.igr_hyperbolic_rhs(self.h, float(gamma), _p(q), _p(sigma), _p(dq)))
         return dq
 
     def apply_sip(self, alpha, x, rho):
-        x, rho = _f64(x), _f64(rho)
+        x, rho = _sized(x, self.N, "x"), _sized(rho, self.N, "rho")
         y = np.empty_like(x)
         self.check(self._L.igr_apply_sip(self.h, float(alpha), _p(x), _p(rho), _p(y)))
         return y
 
     def igr_rhs(self, alpha, q):
-        q = _f64(q)
+        q = _sized(q, self.nv * self.N, "state q")
         b = np.empty(self.shape)
         self.check(self._L.igr_igr_rhs(self.h, float(alpha), _p(q), _p(b)))
         return b
 
     def sip_diagonal(self, alpha, rho):
-        rho = _f64(rho)
+        rho = _sized(rho, self.N, "rho")
         d = np.empty(self.shape)
         self.check(self._L.igr_sip_diagonal(self.h, float(alpha), _p(rho), _p(d)))
         return d
@@ -211,6 +232,7 @@ class Context:
         if s0.ndim == len(self.shape) + 1:
             s0 = s0[None]
         nt = s0.shape[0]
+        s0 = _sized(s0, nt * self.nv * self.N, "tangent states s0")
         so = np.empty_like(s0)
         vo = np.empty((nt,) + self.shape)
         ns = C.c_int(0)
@@ -222,7 +244,7 @@ class Context:
     # discrete adjoint
     def objective(self, obj, gamma, prm=None):
         J = C.c_double(0)
-        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        p = _prm4(prm)
         self.check(self._L.igr_objective(self.h, int(obj), _p(p), float(gamma), C.byref(J)))
         return J.value
 
@@ -243,28 +265,29 @@ class Context:
         """(J, dJ/dq0, dJ/dgamma, dJ/dalpha) of the objective after n_steps fixed steps from the state."""
         J, gb, ab = C.c_double(0), C.c_double(0), C.c_double(0)
         qb = np.empty((self.nv,) + self.shape)
-        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        p = _prm4(prm)
         self.check(self._L.igr_gradient(self.h, int(obj), _p(p), int(n_steps), float(dt), float(gamma),
                                         float(alpha), int(n_iter), _solver_code(solver), int(ckpt_slots),
                                         C.byref(J), _p(qb), C.byref(gb), C.byref(ab)))
         return J.value, qb, gb.value, ab.value
 
     def hyperbolic_vjp(self, gamma, q, sigma, dqbar):
-        q, sigma, dqbar = _f64(q), _f64(sigma), _f64(dqbar)
+        nvN = self.nv * self.N
+        q, sigma, dqbar = _sized(q, nvN, "state q"), _sized(sigma, self.N, "Sigma"), _sized(dqbar, nvN, "dq-bar")
         qb, sb, gb = np.empty_like(q), np.empty(self.shape), C.c_double(0)
         self.check(self._L.igr_hyperbolic_vjp(self.h, float(gamma), _p(q), _p(sigma), _p(dqbar), _p(qb), _p(sb),
                                               C.byref(gb)))
         return qb, sb, gb.value
 
     def apply_sip_vjp(self, alpha, x, rho, ybar):
-        x, rho, ybar = _f64(x), _f64(rho), _f64(ybar)
+        x, rho, ybar = _sized(x, self.N, "x"), _sized(rho, self.N, "rho"), _sized(ybar, self.N, "y-bar")
         xb, rb, ab = np.empty(self.shape), np.empty(self.shape), C.c_double(0)
         self.check(self._L.igr_apply_sip_vjp(self.h, float(alpha), _p(x), _p(rho), _p(ybar), _p(xb), _p(rb),
                                              C.byref(ab)))
         return xb, rb, ab.value
 
     def solve_elliptic_vjp(self, alpha, n_iter, solver, sigma0, sigmabar):
-        s0, sb = _f64(sigma0), _f64(sigmabar)
+        s0, sb = _sized(sigma0, self.N, "sigma0"), _sized(sigmabar, self.N, "Sigma-bar")
         qb, s0b, ab = np.empty((self.nv,) + self.shape), np.empty(self.shape), C.c_double(0)
         self.check(self._L.igr_solve_elliptic_vjp(self.h, float(alpha), int(n_iter), _solver_code(solver), _p(s0),
                                                   _p(sb), _p(qb), _p(s0b), C.byref(ab)))
@@ -292,6 +315,9 @@ class Context:
     def set_snapshots(self, records):
         """records: array (n_steps + 1, nv + 1, ...) of (state, Sigma)."""
         r = _f64(records)
+        if r.ndim < 1 or r.shape[0] < 1:
+            raise ValueError("snapshots: need at least one record")
+        r = _sized(r, r.shape[0] * (self.nv + 1) * self.N, "snapshot records")
         self.check(self._L.igr_set_snapshots(self.h, r.shape[0] - 1, _p(r)))
 
     def run_adjoint_conservative(self, obj, dt, T, gamma, alpha, n_iter, solver="pcg", n_steps=None, prm=None):
@@ -299,7 +325,7 @@ class Context:
         if n_steps is None:
             n_steps = self.snapshot_count()
         a = np.empty((self.nv,) + self.shape)
-        p = None if prm is None else np.ascontiguousarray(prm, dtype=np.float64)
+        p = _prm4(prm)
         self.check(self._L.igr_run_adjoint_conservative(self.h, int(obj), _p(p), int(n_steps), float(dt), float(T),
                                                         float(gamma), float(alpha), int(n_iter),
                                                         _solver_code(solver), _p(a)))
@@ -421,6 +447,7 @@ def run_sensitivity_batch(runs, n_iter=10, solver="pcg", elliptic_rhs="form2"):
         if s0.ndim == len(c.shape) + 1:
             s0 = s0[None]
         nt = s0.shape[0]
+        s0 = _sized(s0, nt * c.nv * c.N, "tangent states s0")
         so = np.empty_like(s0)
         vo = np.empty((nt,) + c.shape)
         keep += [s0, so, vo]

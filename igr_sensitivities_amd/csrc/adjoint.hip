@@ -149,8 +149,20 @@ struct GradWork {
   void* set_block = nullptr;
   size_t set_block_bytes = 0;
   bool set_block_pooled = false;
+  // every other buffer of this workspace, freed with it (not with the context)
+  std::vector<void*> bufs;
+  double* alloc(size_t n) {
+    void* p = nullptr;
+    if (dev_malloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
+    bufs.push_back(p);
+    return static_cast<double*>(p);
+  }
   StepView view() { return {st, Q, Sg}; }
-  ~GradWork() { pool_give(set_block, set_block_pooled); }
+  // the owner synchronizes the context stream before deleting a workspace
+  ~GradWork() {
+    pool_give(set_block, set_block_pooled);
+    for (void* p : bufs) (void)hipFree(p);
+  }
 };
 
 // Doubles of one StepSet for n solver iterations on N nodes.
@@ -194,36 +206,47 @@ static int alloc_sets(Ctx& c, GradWork& W, int K) {
 static std::map<const Ctx*, GradWork*> g_work;
 static std::mutex g_work_mu;
 
+// A workspace sized for n solver iterations serves every call with n or fewer
+// (the per-iteration stores and scalar records are indexed with the call's
+// own n_iter), so the objective and VJP entry points (n_iter = 0) reuse the
+// gradient's.  A call with more iterations replaces it, and the old one's
+// buffers are freed once the context stream has drained.
 static int alloc_work(Ctx& c, int n_iter, int slots, GradWork*& W) {
   GradWork*& w = [&]() -> GradWork*& { std::lock_guard<std::mutex> lk(g_work_mu); return g_work[&c]; }();
   const size_t N = c.N;
   const int n = std::max(n_iter, 0);
-  if (!w || w->n_iter != n) {
-    delete w;             // its c.alloc buffers stay owned by the context until igr_destroy
-    w = new GradWork();
-    w->n_iter = n;
-    for (int s = 0; s < 3; ++s) {
-      StageStore& st = w->st[s];
-      st.P = c.alloc(std::max<size_t>((size_t)n * N, 1));
-      st.AP = c.alloc(std::max<size_t>((size_t)n * N, 1));
-      st.R = c.alloc((size_t)(n + 1) * N);
-      st.invR = c.alloc(N); st.invM = c.alloc(N); st.b = c.alloc(N);
-      st.sc = c.alloc(4 * (size_t)std::max(n, 1) + 1);
-      st.rs = c.alloc(5 * (size_t)std::max(n, 1) + 1);
-      if (!st.P || !st.AP || !st.R || !st.invR || !st.invM || !st.b || !st.sc || !st.rs) return IGR_ENOMEM;
+  if (!w || w->n_iter < n) {
+    if (w) {
+      (void)hipStreamSynchronize(c.stream);
+      delete w;
+      w = nullptr;
     }
-    for (int s = 0; s < 3; ++s) if (!(w->Q[s] = c.alloc(4 * N))) return IGR_ENOMEM;
-    for (int s = 0; s < 4; ++s) if (!(w->Sg[s] = c.alloc(N))) return IGR_ENOMEM;
-    double** one[] = {&w->Xbar, &w->Scarry, &w->RB[0], &w->RB[1], &w->PB[0], &w->PB[1], &w->irbar, &w->imbar,
-                      &w->bbar, &w->XB[0], &w->XB[1]};
-    for (auto p : one) if (!(*p = c.alloc(N))) return IGR_ENOMEM;
-    double** four[] = {&w->QBn, &w->QBa, &w->QBb, &w->QBnext};
-    for (auto p : four) if (!(*p = c.alloc(4 * N))) return IGR_ENOMEM;
+    GradWork* nw = new GradWork();
+    nw->n_iter = n;
+    auto fail = [&]() { delete nw; return IGR_ENOMEM; };
+    for (int s = 0; s < 3; ++s) {
+      StageStore& st = nw->st[s];
+      st.P = nw->alloc(std::max<size_t>((size_t)n * N, 1));
+      st.AP = nw->alloc(std::max<size_t>((size_t)n * N, 1));
+      st.R = nw->alloc((size_t)(n + 1) * N);
+      st.invR = nw->alloc(N); st.invM = nw->alloc(N); st.b = nw->alloc(N);
+      st.sc = nw->alloc(4 * (size_t)std::max(n, 1) + 1);
+      st.rs = nw->alloc(5 * (size_t)std::max(n, 1) + 1);
+      if (!st.P || !st.AP || !st.R || !st.invR || !st.invM || !st.b || !st.sc || !st.rs) return fail();
+    }
+    for (int s = 0; s < 3; ++s) if (!(nw->Q[s] = nw->alloc(4 * N))) return fail();
+    for (int s = 0; s < 4; ++s) if (!(nw->Sg[s] = nw->alloc(N))) return fail();
+    double** one[] = {&nw->Xbar, &nw->Scarry, &nw->RB[0], &nw->RB[1], &nw->PB[0], &nw->PB[1], &nw->irbar,
+                      &nw->imbar, &nw->bbar, &nw->XB[0], &nw->XB[1]};
+    for (auto p : one) if (!(*p = nw->alloc(N))) return fail();
+    double** four[] = {&nw->QBn, &nw->QBa, &nw->QBb, &nw->QBnext};
+    for (auto p : four) if (!(*p = nw->alloc(4 * N))) return fail();
     const size_t nb = c.partials_n;
-    if (!(w->gblk = c.alloc(nb)) || !(w->ablk = c.alloc(nb)) || !(w->dsc = c.alloc(16))) return IGR_ENOMEM;
+    if (!(nw->gblk = nw->alloc(nb)) || !(nw->ablk = nw->alloc(nb)) || !(nw->dsc = nw->alloc(16))) return fail();
+    w = nw;
   }
   while ((int)w->ck.size() < slots) {
-    double* p = c.alloc(5 * N);
+    double* p = w->alloc(5 * N);
     if (!p) return IGR_ENOMEM;
     w->ck.push_back(p);
   }
@@ -236,6 +259,16 @@ struct GradWork1 {
   int n_iter = -1;
   K1AdjArgs base;
   std::vector<double*> ck;
+  std::vector<void*> bufs;   // every buffer above, freed with the workspace
+  double* alloc(size_t n) {
+    void* p = nullptr;
+    if (dev_malloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
+    bufs.push_back(p);
+    return static_cast<double*>(p);
+  }
+  ~GradWork1() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
 };
 static std::map<const Ctx*, GradWork1*> g_work1;
 
@@ -243,27 +276,34 @@ static int alloc_work1(Ctx& c, int n_iter, int slots, GradWork1*& W) {
   GradWork1*& w = [&]() -> GradWork1*& { std::lock_guard<std::mutex> lk(g_work_mu); return g_work1[&c]; }();
   const size_t N = c.N;
   const int n = std::max(n_iter, 0);
-  if (!w || w->n_iter != n) {
-    delete w;              // its c.alloc buffers stay owned by the context until igr_destroy
-    w = new GradWork1();
-    w->n_iter = n;
-    K1AdjArgs& a = w->base;
-    for (int s = 0; s < 3; ++s) {
-      a.Q[s] = c.alloc(3 * N); a.invR[s] = c.alloc(N); a.invM[s] = c.alloc(N); a.b[s] = c.alloc(N);
-      a.P[s] = c.alloc(std::max<size_t>((size_t)n * N, 1)); a.AP[s] = c.alloc(std::max<size_t>((size_t)n * N, 1));
-      a.R[s] = c.alloc((size_t)(n + 1) * N); a.sc[s] = c.alloc(4 * (size_t)std::max(n, 1) + 1);
-      if (!a.Q[s] || !a.invR[s] || !a.invM[s] || !a.b[s] || !a.P[s] || !a.AP[s] || !a.R[s] || !a.sc[s])
-        return IGR_ENOMEM;
+  if (!w || w->n_iter < n) {   // as alloc_work: a larger workspace serves smaller n_iter
+    if (w) {
+      (void)hipStreamSynchronize(c.stream);
+      delete w;
+      w = nullptr;
     }
-    for (int s = 0; s < 4; ++s) if (!(a.Sg[s] = c.alloc(N))) return IGR_ENOMEM;
+    GradWork1* nw = new GradWork1();
+    nw->n_iter = n;
+    auto fail = [&]() { delete nw; return IGR_ENOMEM; };
+    K1AdjArgs& a = nw->base;
+    for (int s = 0; s < 3; ++s) {
+      a.Q[s] = nw->alloc(3 * N); a.invR[s] = nw->alloc(N); a.invM[s] = nw->alloc(N); a.b[s] = nw->alloc(N);
+      a.P[s] = nw->alloc(std::max<size_t>((size_t)n * N, 1));
+      a.AP[s] = nw->alloc(std::max<size_t>((size_t)n * N, 1));
+      a.R[s] = nw->alloc((size_t)(n + 1) * N); a.sc[s] = nw->alloc(4 * (size_t)std::max(n, 1) + 1);
+      if (!a.Q[s] || !a.invR[s] || !a.invM[s] || !a.b[s] || !a.P[s] || !a.AP[s] || !a.R[s] || !a.sc[s])
+        return fail();
+    }
+    for (int s = 0; s < 4; ++s) if (!(a.Sg[s] = nw->alloc(N))) return fail();
     double** three[] = {&a.QBnext, &a.QBn, &a.QBa, &a.QBb, &a.Ab};
-    for (auto p : three) if (!(*p = c.alloc(3 * N))) return IGR_ENOMEM;
+    for (auto p : three) if (!(*p = nw->alloc(3 * N))) return fail();
     double** one[] = {&a.Xbar, &a.Scarry, &a.w, &a.y, &a.rb, &a.pb, &a.xb, &a.irbar, &a.imbar, &a.bbar};
-    for (auto p : one) if (!(*p = c.alloc(N))) return IGR_ENOMEM;
-    if (!(a.dsc = c.alloc(16))) return IGR_ENOMEM;
+    for (auto p : one) if (!(*p = nw->alloc(N))) return fail();
+    if (!(a.dsc = nw->alloc(16))) return fail();
+    w = nw;
   }
   while ((int)w->ck.size() < slots) {
-    double* p = c.alloc(4 * N);
+    double* p = w->alloc(4 * N);
     if (!p) return IGR_ENOMEM;
     w->ck.push_back(p);
   }

@@ -34,3 +34,18 @@ def test_python_prototypes_cover_header():
 def test_loader_sets_prototypes():
     L = _lib.lib()
     assert L.igr_create.restype is C.c_int
+
+
+def test_wrappers_refuse_wrong_sizes():
+    # the C side reads and writes exactly the counts it expects through raw
+    # pointers; the Python mirror refuses anything else before the call
+    import numpy as np
+    import pytest
+    from igr_sensitivities_amd import core
+    assert core._sized(np.zeros((2, 3)), 6, "x").shape == (2, 3)
+    with pytest.raises(ValueError):
+        core._sized(np.zeros(5), 6, "x")
+    np.testing.assert_array_equal(core._prm4((0.4, 0.6, 0.2)), [0.4, 0.6, 0.2, 0.0])
+    assert core._prm4(None) is None
+    with pytest.raises(ValueError):
+        core._prm4(np.zeros(5))

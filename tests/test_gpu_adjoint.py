@@ -5,6 +5,8 @@ whole gradient, plus central finite differences of the GPU forward map.
 This stands in for the reference's Enzyme checks (Enzyme == ForwardDiff to
 1e-8 in IGR_adjoints_1d/tests/run_enzyme_test.jl:87; vs FD to 1e-3 in
 IGR_adjoints_2d/tests/run_enzyme_test.jl:88)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -172,9 +174,9 @@ def test_reverse_tile_defaults_and_rejects():
 def test_gradient_reverse_tiles_agree(solver):
     # the reverse shape changes only the summation order of the reverse dot
     # products: J is bit-equal (same forward), the gradient agrees to rounding;
-    # with 2x2 tiles the 120 workgroups outnumber the 64 threads that sum
-    # their partials, so the chunked partial-sum loop runs too
-    nx, ny, ns = 24, 20, 3
+    # with 2x2 tiles the 210 workgroups are summed by 64 threads, so the grid
+    # total adds three chunks after the first (loaded together, four per pass)
+    nx, ny, ns = 30, 28, 3
     q0 = configs.smooth_random2(3, nx, ny, seed=130, amp=0.3)
     dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
     res = []
@@ -289,3 +291,28 @@ def test_gradient_across_contexts_and_release_cache():
     for J, qb, gb, ab in outs[1:]:
         assert J == outs[0][0] and gb == outs[0][2] and ab == outs[0][3]
         np.testing.assert_array_equal(qb, outs[0][1])
+
+
+def _hip_free_bytes():
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_objective_and_gradient_alternating_keep_memory_flat():
+    """The objective and VJP entry points (n_iter = 0) reuse the gradient's
+    workspace, and a workspace that is replaced frees its buffers: free
+    device memory does not fall across repeated objective/gradient calls."""
+    print("[case] workspace reuse 24x24", flush=True)
+    q0 = configs.smooth_random2(3, 24, 24, seed=91, amp=0.2)
+    dt = 0.3 * (1 / 24) / (7 * configs.max_wavespeed2(q0, 1.4))
+    c = ctx2(3, 24, 24)
+    free = []
+    for k in range(4):
+        c.set_state(q0)
+        c.objective(3, 1.4)
+        c.gradient(3, 2, dt, 1.4, 3e-3, 20 + 10 * (k % 2), "pcg", ckpt_slots=2)
+        free.append(_hip_free_bytes())
+    # one field is 3 MB here; a leaked workspace would be > 100 MB per cycle
+    assert max(free[1:]) - min(free[1:]) < 64 << 20, free

@@ -136,23 +136,30 @@ def test_one_step_and_run_2d(orc, solver):
     assert rel(ST, STr) < 1e-9
 
 
-def test_default_tiles_with_more_workgroups_than_threads(orc):
-    # 80x80 elements take the default 4x4 tiles: 400 workgroups of 256
-    # threads, so every workgroup sums the previous kernel's partials in more
-    # than one chunk (the chunked loads of the grid total run)
-    c = ctx2(3, 80, 80)
-    assert c.get_option(igr.core.OPT_TILE) == 404
-    g = orc.Grid2D(3, 80, 80)
-    q0 = configs.smooth_random2(3, 80, 80, seed=140, amp=0.3)
-    gam, a = 1.4, configs.alpha_of(1 / 80, 3)
-    dt = 0.4 * (1 / 80) / (7 * configs.max_wavespeed2(q0, gam))
+@pytest.mark.parametrize("n,tile", [(80, None), (30, 202)])
+def test_default_tiles_with_more_workgroups_than_threads(orc, n, tile):
+    # More partial sums than threads per workgroup, so the grid total adds
+    # chunks after the first (one value per thread each).  80x80 with the
+    # default 4x4 tiles: 400 workgroups of 256 threads, one further chunk.
+    # 30x28 with 2x2 tiles: 210 workgroups of 64 threads, three further
+    # chunks, loaded together in one pass of the grid total.
+    ny = n if tile is None else 28
+    c = ctx2(3, n, ny)
+    if tile is None:
+        assert c.get_option(igr.core.OPT_TILE) == 404
+    else:
+        c.set_option(igr.core.OPT_TILE, tile)
+    g = orc.Grid2D(3, n, ny)
+    q0 = configs.smooth_random2(3, n, ny, seed=140, amp=0.3)
+    gam, a = 1.4, configs.alpha_of(1 / n, 3)
+    dt = 0.4 * (1 / n) / (7 * configs.max_wavespeed2(q0, gam))
     c.set_state(q0)
     c.run_steps(1, dt, gam, a, 30, "pcg")
     q1, S1 = c.get_state()
     q1r, S1r = g.run_steps(gam, a, 30, orc.PCG, 1, dt, q0)
     assert rel(q1, q1r) < 1e-12
     # Sigma is an inexact 30-iteration PCG iterate after three warm-started
-    # solves; on 102,400 nodes the dot products' summation order (a tree here,
+    # solves; on 10^5 nodes the dot products' summation order (a tree here,
     # serial in the oracle) shows in it at about 1e-11.  A partial-sum chunk
     # left out or added twice would move alpha by tens of percent.
     assert rel(S1, S1r) < 1e-10
@@ -303,6 +310,13 @@ def test_errors():
     q1[2] = -1.0
     with pytest.raises(igr.DomainError):
         c1.hyperbolic_rhs(1.4, q1, np.zeros(q1.shape[1:]))
+    # arrays of the wrong size are refused before the C call
+    with pytest.raises(ValueError):
+        c.set_state(q[:, :2])
+    with pytest.raises(ValueError):
+        c.apply_sip(0.01, np.zeros(7), np.ones(q.shape[1:]))
+    with pytest.raises(ValueError):
+        c.objective(3, 1.4, prm=np.zeros(6))
 
 
 def test_mirror_api_run_forward(orc):
