@@ -58,6 +58,7 @@ PROTOTYPES = {
     "igr_run_adjoint_conservative": (_i, [_vp, _i, _pd, _i, _d, _d, _d, _d, _i, _i, _pd]),
     "igr_release_cache": (_i, []),
     "igr_build_ic": (_i, [_vp, _i, _d, _d, _pd, _i, _pd]),
+    "igr_solve_elliptic_noalloc": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

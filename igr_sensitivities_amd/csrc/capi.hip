@@ -108,10 +108,16 @@ SipArgs sip_base(const Ctx& c, double alpha) {
 // Sigma solve on stage state qs (device, 4N); warm start and result c.S.
 void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc, double* x,
             const double* b_in) {
+  solve2_fields(c, qs, qs + c.N, qs + 2 * c.N, alpha, n_iter, solver, sc, x, b_in);
+}
+
+// The same solve on separate momentum and density fields (device, N each).
+void solve2_fields(Ctx& c, const double* mx, const double* my, const double* rho, double alpha, int n_iter,
+                   int solver, double* sc, double* x, const double* b_in) {
   const size_t N = c.N;
   if (!x) x = c.S;
   SipArgs a = sip_base(c, alpha);
-  a.mx = qs; a.my = qs + N; a.rho = qs + 2 * N; a.x = x; a.b_in = b_in;
+  a.mx = mx; a.my = my; a.rho = rho; a.x = x; a.b_in = b_in;
   a.invR_out = c.invR; a.invM_out = c.invM; a.sc = sc; a.n_iter = n_iter;
   if (solver == IGR_SOLVER_PCG) {
     double* R0 = c.partials;

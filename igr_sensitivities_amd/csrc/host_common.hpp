@@ -64,6 +64,11 @@ int sens1_args(Ctx& c, TanBufs& T, int n_tan, double dt, double t_end, double ga
 void step2(Ctx& c, double dt, double gamma, double alpha, int n_iter, int solver);
 void solve2(Ctx& c, const double* qs, double alpha, int n_iter, int solver, double* sc, double* x = nullptr,
             const double* b_in = nullptr);
+void solve2_fields(Ctx& c, const double* mx, const double* my, const double* rho, double alpha, int n_iter,
+                   int solver, double* sc, double* x, const double* b_in);
+void make_basis(int p, double* xi, double* w, double* Dm);
+void default_tiles(Ctx& c);
+int grid2_blocks(const Ctx& c);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 K1Args k1_base(Ctx& c, double gamma, double alpha, int n_iter, int solver);
 int run1(Ctx& c, const K1Args& a);

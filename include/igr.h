@@ -269,6 +269,28 @@ int igr_run_adjoint_conservative(igr_ctx* ctx, int objective, const double* prm,
  * reference (Julia's GC owns its tape). */
 int igr_release_cache(void);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): solve_elliptic! on caller-owned DEVICE arrays with no
+ * context.  mx, my, rho (N doubles each, N = (p+1)^2 nex ney, 2D layout as
+ * above) are read; sigma (N) is the warm start and receives the result, as
+ * in igr_solve_elliptic.  The solver's work vectors, partial sums and scalar
+ * record live in a fixed device array of the library (16.1 MiB per GPU, kept
+ * until the process ends); nothing is allocated and nothing is synchronized:
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns.  Limits: N <= 262144 (e.g. 128 x 128 elements at
+ * p = 3), n_iter <= 1024; larger sizes, a bad solver code or a bad mesh
+ * return IGR_EINVAL without enqueuing anything.  domain_flag (device int,
+ * optional) is set to 0 and then to 1 if a density is non-positive (the
+ * reference's DomainError); read it after the stream completes.  Calls on
+ * one GPU share the array, so calls that overlap on one GPU (other streams,
+ * other host threads) race; consecutive calls on one stream are safe.  The
+ * first call on a device resolves the array's address and must not be made
+ * under graph capture. */
+int igr_solve_elliptic_noalloc(int p, int nex, int ney, double lx, double ly, double alpha, int n_iter,
+                               int solver, const double* mx, const double* my, const double* rho, double* sigma,
+                               int* domain_flag, void* stream);
+
 /* Mean device time (ms) per launch of one hot kernel on the current state,
  * each launch stamped with start/end events by hipExtLaunchKernelGGL on the
  * context stream.  kernel 0 (fused SIP apply + PCG direction update) and 1
