@@ -215,11 +215,14 @@ class Grid2D:
                                     _p(q), _p(S), C.byref(ns)))
         return q, S, ns.value
 
-    def run_steps(self, gamma, alpha, n_iter, solver, n_steps, dt, q0):
+    def run_steps(self, gamma, alpha, n_iter, solver, n_steps, dt, q0, omp=False):
+        """omp=True runs the OpenMP build of the same loops (the same bits,
+        tests/test_oracle_omp.py), for the large meshes."""
         q = q0.copy()
         S = self.field()
-        _chk(lib().orc2_run_steps(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _i(n_steps),
-                                  _d(dt), _p(q), _p(S)))
+        L = omp_lib() if omp else lib()
+        _chk(L.orc2_run_steps(*self._g(), _d(gamma), _d(alpha), _i(n_iter), _i(solver), _i(n_steps),
+                              _d(dt), _p(q), _p(S)))
         return q, S
 
     def objective(self, obj, gamma, q, prm=(0.0, 0.0, 1.0, 0.0)):
@@ -290,12 +293,13 @@ class Grid2D:
         return q, dq, S, dS
 
     def jvp_objective_run(self, gamma, dgamma, alpha, dalpha, n_iter, solver, n_steps, dt, obj, q0, dq0,
-                          prm=(0.0, 0.0, 1.0, 0.0)):
+                          prm=(0.0, 0.0, 1.0, 0.0), omp=False):
         q, dq = q0.copy(), dq0.copy()
         S, dS = self.field(), self.field()
         prm = np.ascontiguousarray(prm, dtype=np.float64)
         J, dJ = _d(0), _d(0)
-        _chk(lib().orc2_jvp_objective_run(*self._g(), _d(gamma), _d(dgamma), _d(alpha), _d(dalpha), _i(n_iter),
+        L = omp_lib() if omp else lib()
+        _chk(L.orc2_jvp_objective_run(*self._g(), _d(gamma), _d(dgamma), _d(alpha), _d(dalpha), _i(n_iter),
                                           _i(solver), _i(n_steps), _d(dt), _i(obj), _p(prm), _p(q), _p(dq), _p(S),
                                           _p(dS), C.byref(J), C.byref(dJ)))
         return J.value, dJ.value, q, dq

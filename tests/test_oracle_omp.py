@@ -19,3 +19,19 @@ def test_omp_build_bit_equal(Nex, Ney, solver):
     assert threads >= 1
     np.testing.assert_array_equal(q2, q1)
     np.testing.assert_array_equal(S2, S1)
+
+
+@pytest.mark.parametrize("obj", [O.OBJ_DISC_KE, O.OBJ_WINDOWED_DENSITY])
+def test_omp_build_jvp_bit_equal(obj):
+    # the Dual-number JVP of an objective run, which checks the GPU gradient
+    # at full mesh sizes through the OpenMP build
+    g = O.Grid2D(3, 7, 6)
+    q0 = configs.smooth_random2(3, 7, 6, seed=12, amp=0.2)
+    du = np.random.default_rng(13).standard_normal(q0.shape)
+    dt = 0.3 * (1 / 7) / (7 * configs.max_wavespeed2(q0, 1.4))
+    prm = (0.45, 0.5, 0.2, 0.0)
+    a = g.jvp_objective_run(1.4, 0.3, 2e-3, 1e-3, 12, O.PCG, 2, dt, obj, q0, du, prm)
+    b = g.jvp_objective_run(1.4, 0.3, 2e-3, 1e-3, 12, O.PCG, 2, dt, obj, q0, du, prm, omp=True)
+    assert a[0] == b[0] and a[1] == b[1]
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[3], b[3])
