@@ -110,7 +110,10 @@ def c5_leg(igr, configs, budget_slots=40):
     gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
     obj = igr.OBJECTIVES["disc_kinetic_energy"]
     out = {"workload": "2D 256x256 p=3, Gaussian blast (peak 100, sigma 0.03) beside a counter-rotating Oseen "
-                       "pair (core 0.05, vortex Mach 0.5), PCG-60, 400 SSP-RK3 steps; J = kinetic energy in a disc",
+                       "pair (core 0.05, vortex Mach 0.5), PCG-60, 400 SSP-RK3 steps; J = kinetic energy in a disc. "
+                       "The stated single Lamb-Oseen vortex (circulation 1, core 0.05) on rho = p = 1 has a negative "
+                       "core pressure and a net circulation that is not periodic, so the reference's own "
+                       "blast-vortex pair (build_oseen_pair_blast_ic!) stands in for it",
            "steps": ns}
     out["tiles"] = {"sip": ctx.get_option(igr.core.OPT_TILE), "hyperbolic": ctx.get_option(igr.core.OPT_HTILE),
                     "reverse": ctx.get_option(igr.core.OPT_RTILE)}
@@ -154,35 +157,49 @@ def c1_leg(igr, configs, cpu_steps=200):
     ns = ctx.run_forward(cfg["dt"], cfg["T"], cfg["gamma"], cfg["alpha"], cfg["n_iter"], cfg["solver"])
     t = time.perf_counter() - t0
     n_nodes = cfg["Ne"] * 4
-    out = {"workload": "1D blast (1000 / 0.01 / 100 with tanh edges), N_e = 128, p = 3, damped Jacobi-50, "
-                       "CFL 0.1 to t = 0.038", "steps": ns, "seconds": t,
+    out = {"workload": "1D blast (1000 / 0.01 / 100 with tanh edges of width 0.01), N_e = 128, p = 3, "
+                       "alpha = 10 h^2 (the reference's 1D choice), damped Jacobi-50, CFL 0.1 to t = 0.038; the "
+                       "stated sharp steps and alpha = 10 (h/4)^2 reach a negative pressure within the run in the "
+                       "reference's map", "steps": ns, "seconds": t,
            "dof_stage_per_s": 3 * n_nodes * 3 * ns / t}
     ctx.close()
     return out
 
 
-def size_sweep_leg(igr, configs, sizes=(128, 256, 512, 1024), reps=100):
-    """Per-launch time and algorithmic bandwidth of the three forward kernels
-    on the headline workload's data at growing mesh sizes.  At 128^2 a launch
+# Algorithmic bytes per node of the three forward kernels (each distinct
+# input and output field once, 8 B per node, halo re-reads excluded): PCG_A
+# reads r, 1/M, p_old, 1/rho and writes p, Ap; PCG_B reads x, p, r, Ap, 1/M
+# and writes x, r; the hyperbolic stage 2 reads q_stage (4), Sigma, q^n (4)
+# and writes 4.
+KERNEL_FIELDS = (("pcg_a", 3, 6), ("pcg_b", 4, 7), ("hyp", 5, 13))
+
+
+def kernel_rows(igr, configs, N, reps=200):
+    """Mean in-stream time per launch of the three forward kernels on the
+    headline workload's data at N x N (back-to-back launches between two HIP
+    events on the context's stream) and their fraction of the HBM peak."""
+    cfg = configs.config2(N=N)
+    ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(3)))
+    ctx.set_state(cfg["q"])
+    n_nodes = 16 * N * N
+    row = {"mesh": f"{N}x{N}", "nodes": n_nodes,
+           "tiles": {"sip": ctx.get_option(igr.core.OPT_TILE), "hyperbolic": ctx.get_option(igr.core.OPT_HTILE),
+                     "pcg_b_workgroups": ctx.get_option(igr.core.OPT_GROUPS)}}
+    for name, kern, fields in KERNEL_FIELDS:
+        ms = ctx.time_kernel(kern, cfg["gamma"], cfg["alpha"], reps)
+        alg = fields * 8 * n_nodes
+        row[name] = {"us": ms * 1e3, "alg_bytes": alg, "gbs": alg / (ms * 1e-3) / 1e9,
+                     "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ctx.close()
+    return row
+
+
+def size_sweep_leg(igr, configs, sizes=(128, 256, 512, 1024)):
+    """The three forward kernels at growing mesh sizes.  At 128^2 a launch
     moves about one latency-bandwidth product of data (12.6 MB), so the time
     is mostly latency; larger meshes show the streaming rate the same
-    kernels reach.  Algorithmic bytes per node: PCG_A 6 fields (read r, 1/M,
-    p_old, 1/rho; write p, Ap), PCG_B 7 (read x, p, r, Ap, 1/M; write x, r),
-    hyperbolic stage 2 13 (read q_stage, Sigma, q^n; write 4)."""
-    rows = []
-    for N in sizes:
-        cfg = configs.config2(N=N)
-        ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(3)))
-        ctx.set_state(cfg["q"])
-        n_nodes = 16 * N * N
-        row = {"mesh": f"{N}x{N}", "nodes": n_nodes}
-        for name, kern, fields in (("pcg_a", 0, 6), ("pcg_b", 1, 7), ("hyp", 2, 13)):
-            ms = ctx.time_kernel(kern, cfg["gamma"], cfg["alpha"], reps)
-            row[name + "_us"] = ms * 1e3
-            row[name + "_gbs"] = fields * 8 * n_nodes / (ms * 1e-3) / 1e9
-        ctx.close()
-        rows.append(row)
-    return rows
+    kernels reach."""
+    return [kernel_rows(igr, configs, N, reps=200 if N <= 512 else 50) for N in sizes]
 
 
 def adjoint_pde_leg(igr, configs, cfg, steps, prm, qbar, t_forward):
@@ -221,7 +238,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=8, help="SSP-RK3 steps of the CPU-oracle sample")
+    ap.add_argument("--cpu-steps", type=int, default=24, help="SSP-RK3 steps of the CPU-oracle sample")
+    ap.add_argument("--mesh", type=int, default=128,
+                    help="elements per direction of the headline workload (128; 256 is the 256x256 size the "
+                         "profiles of the larger mesh use)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the kernel size sweep")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
@@ -241,13 +261,14 @@ def main():
     global GRAPHS
     GRAPHS = not args.graphs_off
 
-    cfg = configs.config2(N=128)
+    cfg = configs.config2(N=args.mesh)
     N1 = cfg["N"]
     ctx = new_ctx(igr, igr.CartesianMesh2D(N1, N1, 1.0, 1.0, igr.DGBasis(3)))
     n_nodes = 16 * N1 * N1
     gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
 
-    tiles = (ctx.get_option(igr.core.OPT_TILE), ctx.get_option(igr.core.OPT_HTILE))
+    tiles = (ctx.get_option(igr.core.OPT_TILE), ctx.get_option(igr.core.OPT_HTILE),
+             ctx.get_option(igr.core.OPT_GROUPS))
     ctx.set_state(cfg["q"])
     ctx.run_steps(max(args.warmup, 1), dt, gam, al, n_iter, "pcg")   # also captures the step graph
     ctx.set_state(cfg["q"])
@@ -268,15 +289,17 @@ def main():
     ms_stamped = ctx.time_kernel(0, gam, al, reps)
     alg_bytes = 6 * 8 * n_nodes
     achieved = alg_bytes / (ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes of this workload (profiles/), when they cover this mesh
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pcg_a_2d_128x128_p3_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"pcg_a_2d_{args.mesh}x{args.mesh}_p3_traffic.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    ms_b = ctx.time_kernel(1, gam, al, reps)
-    ms_h = ctx.time_kernel(2, gam, al, reps)
+    ms_b = ctx.time_kernel(4, gam, al, reps)
+    ms_h = ctx.time_kernel(5, gam, al, reps)
 
     # adjoint gradient of J = int rho(T) w (Gaussian window, sigma 0.1) over the
     # same steps, w.r.t. the initial state, gamma and alpha; checkpoint slots
@@ -298,6 +321,10 @@ def main():
     c5 = None if args.no_c5 else c5_leg(igr, configs)
     blast1 = None if args.no_c1 else c1_leg(igr, configs)
     sweep = None if args.no_sweep else size_sweep_leg(igr, configs)
+    # the kernels at the 256x256 size of the blast-vortex workload
+    big = None
+    if args.mesh != 256:
+        big = next((r for r in (sweep or []) if r["mesh"] == "256x256"), None) or kernel_rows(igr, configs, 256)
 
     cpu = None
     if not args.no_cpu:
@@ -316,8 +343,8 @@ def main():
             "unit": "DOF-stage updates/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"first {args.cpu_steps} SSP-RK3 steps of the same 128x128 PCG-60 workload, "
-                      f"C++ restatement of the reference loops (-O2 -ffp-contract=off), 1 thread on "
+            "sample": f"first {args.cpu_steps} SSP-RK3 steps of the same {N1}x{N1} PCG-60 workload, "
+                      f"C++ restatement of the reference loops (-O3 -ffp-contract=off), 1 thread on "
                       f"{cpu_model()}; Julia reference not run (no Julia on the host)",
             "openmp": {"value": 4 * n_nodes * 3 * args.cpu_steps / (c3 - c2), "threads": omp_threads,
                        "note": "same sample, OpenMP build of the same loops (bit-identical results); "
@@ -338,10 +365,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded triple Gaussian energy deposit)",
         "config": {
-            "workload": "2D IGR Euler forward, 128x128 elements, p=3, PCG-60 Sigma solve per stage, "
-                        "SSP-RK3, triple energy deposit",
+            "workload": f"2D IGR Euler forward, {N1}x{N1} elements, p=3, PCG-60 Sigma solve per stage, "
+                        "SSP-RK3, triple energy deposit (unit total energy, sigma 0.02, seeded centres) on a "
+                        "background pressure of 1, not the stated 1e-5: from 1e-5 the reference's unlimited "
+                        "DG map reaches a negative pressure (Julia DomainError) at step 86 of 200; the work "
+                        "per step (mesh, p, PCG-60, FP64) is the stated one",
             "nodes": n_nodes, "n_iter": n_iter, "solver": "pcg", "dt": dt, "alpha": al, "gamma": gam,
-            "tiles": {"sip": tiles[0], "hyperbolic": tiles[1]},
+            "background_pressure": 1.0,
+            "tiles": {"sip": tiles[0], "hyperbolic": tiles[1], "pcg_b_workgroups": tiles[2]},
         },
         "roofline": {
             "bound": "hbm",
@@ -355,8 +386,15 @@ def main():
             "launch_us_kernel_stamped": ms_stamped * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },
-        "kernels_us": {"pcg_a_sip": ms_stamped * 1e3, "pcg_b_update": ms_b * 1e3, "hyp_rk3": ms_h * 1e3,
-                       "timing": "per-launch kernel-stamped events inside a solve"},
+        "kernels": {
+            "timing": "back-to-back launches between two HIP events on the context stream, mean per launch",
+            "pcg_a_sip": {"us": ms * 1e3, "frac": achieved / HBM_PEAK_GBS},
+            "pcg_b_update": {"us": ms_b * 1e3, "alg_bytes": 7 * 8 * n_nodes,
+                             "frac": 7 * 8 * n_nodes / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "hyp_rk3": {"us": ms_h * 1e3, "alg_bytes": 13 * 8 * n_nodes,
+                        "frac": 13 * 8 * n_nodes / (ms_h * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        },
+        "roofline_256x256": big,
         "gradient_over_forward": t_grad / elapsed,
         "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,
                      "J": J, "dJ_dgamma": dgam, "dJ_dalpha": dal},

@@ -29,7 +29,7 @@ void launch2_rev(const Ctx& c, int mode, const RevArgs& a);
 void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad);
 void launch_sum_blocks(const Ctx& c, const double* blk, int nb, double* out);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
-void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
+void launch_pcg_final(const Ctx& c, const double* part, double* sc, int nit);
 int grid2_blocks(const Ctx& c);
 int grid2_rblocks(const Ctx& c);
 int grid2_hblocks(const Ctx& c);
@@ -351,7 +351,7 @@ static void solve2_store_pcg(Ctx& c, const double* qs, double* x, double alpha, 
     m.part_in = R1; m.part_out = R0;
     launch2_sip(c, M_PCG_B, m);
   }
-  launch_pcg_final(c, R0, grid2_blocks(c), st.sc, n_iter);
+  launch_pcg_final(c, R0, st.sc, n_iter);
 }
 
 // Damped Jacobi (jacobi_fixed!, elliptic.jl:354-361) keeping x_k and A x_k.

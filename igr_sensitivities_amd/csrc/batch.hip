@@ -69,13 +69,18 @@ struct Program {
   CopyArgs* copy = nullptr;
   int2* work = nullptr;      // (instance, workgroup) items of a SIP-family launch
   int nblocks = 0;
+  int2* bwork = nullptr;     // and of a PCG x/r update (its own workgroup count)
+  int bnblocks = 0;
   int2* hwork = nullptr;     // and of a hyperbolic-family launch
   int hnblocks = 0;
 
   void run(hipStream_t st) const {
     for (const L& l : ls) {
       switch (l.kind) {
-        case R_SIP: launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, work, nblocks, st); break;
+        case R_SIP:
+          if (l.mode == M_PCG_B) launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, bwork, bnblocks, st);
+          else launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, work, nblocks, st);
+          break;
         case R_HYP: launch2_hyp_batch(np, htx, hty, hyp + l.off, hwork, hnblocks, st); break;
         case R_SENS_RHS: launch2_sens_rhs_batch(np, htx, hty, sens + l.off, hwork, hnblocks, st); break;
         case R_SENS_HYP: launch2_sens_hyp_batch(np, htx, hty, sens + l.off, hwork, hnblocks, st); break;
@@ -138,19 +143,23 @@ int build_programs(const std::vector<Ctx*>& ctxs, const std::vector<int>& ids, c
       }
       P.ls.push_back(l);
     }
-    std::vector<int2> work, hwork;
+    std::vector<int2> work, bwork, hwork;
     for (size_t g = 0; g < mem.size(); ++g) {
       const Ctx& c = *ctxs[ids[mem[g]]];
-      const int nb = ((c.Nex + P.tx - 1) / P.tx) * ((c.Ney + P.ty - 1) / P.ty);
+      // the workgroup counts of the instance's own launches
+      const int nb = grid2_blocks(c), nbb = grid2_bblocks(c);
       for (int b = 0; b < nb; ++b) work.push_back(make_int2((int)g, b));
+      for (int b = 0; b < nbb; ++b) bwork.push_back(make_int2((int)g, b));
       const int hb = ((c.Nex + P.htx - 1) / P.htx) * ((c.Ney + P.hty - 1) / P.hty);
       for (int b = 0; b < hb; ++b) hwork.push_back(make_int2((int)g, b));
     }
     P.nblocks = (int)work.size();
+    P.bnblocks = (int)bwork.size();
     P.hnblocks = (int)hwork.size();
     P.work = dev.put(work);
+    P.bwork = dev.put(bwork);
     P.hwork = dev.put(hwork);
-    if (!P.work || !P.hwork) return IGR_ENOMEM;
+    if (!P.work || !P.bwork || !P.hwork) return IGR_ENOMEM;
     if ((!hs.empty() && !(P.sip = dev.put(hs))) || (!hh.empty() && !(P.hyp = dev.put(hh))) ||
         (!hn.empty() && !(P.sens = dev.put(hn))) || (!hf.empty() && !(P.fin = dev.put(hf))) ||
         (!hl.empty() && !(P.fill = dev.put(hl))) || (!hc.empty() && !(P.copy = dev.put(hc))))

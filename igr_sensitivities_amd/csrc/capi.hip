@@ -22,7 +22,7 @@ void launch_fill(const Ctx& c, double* x, size_t n, double v);
 void launch_copy(const Ctx& c, double* dst, const double* src, size_t n);
 void launch1_run(const Ctx& c, const K1Args& a, int block);
 void set_kernel_timer(hipEvent_t t0, hipEvent_t t1);
-void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit);
+void launch_pcg_final(const Ctx& c, const double* part, double* sc, int nit);
 void free_grad_work(const Ctx* c);
 void free_tan_bufs(const Ctx* c);
 
@@ -138,7 +138,7 @@ void solve2_fields(Ctx& c, const double* mx, const double* my, const double* rho
       m.part_in = R1; m.part_out = R0;
       launch2_sip(c, M_PCG_B, m);
     }
-    launch_pcg_final(c, R0, grid2_blocks(c), sc, n_iter);
+    launch_pcg_final(c, R0, sc, n_iter);
   } else if (solver == IGR_SOLVER_JACOBI) {
     a.b_out = c.b;
     launch2_sip(c, M_INIT_JAC, a);
@@ -244,6 +244,14 @@ void default_tiles(Ctx& c) {
   else { c.tile_x = 8; c.tile_y = 8; c.htile_x = 8; c.htile_y = 4; }
   c.rtile_x = c.tile_x == 8 ? 8 : c.tile_x;
   c.rtile_y = c.tile_x == 8 ? 4 : c.tile_y;
+  // The PCG x/r update streams 7 fields per node and no halo: beyond 512
+  // tiles it runs on max(256, tiles / 16) workgroups that each loop over
+  // their tiles with the next tile's loads in flight, and the next PCG_A
+  // reads that many partial sums instead of one per tile (measured on
+  // MI355X at 128^2 .. 1024^2: -8% per step at 256^2, -16% at 512^2, -32%
+  // at 1024^2; no change up to 128^2).
+  const int nt = ((c.Nex + c.tile_x - 1) / c.tile_x) * ((c.Ney + c.tile_y - 1) / c.tile_y);
+  c.bgroups = nt > 512 ? std::max(256, (nt / 16 + 7) / 8 * 8) : 0;
 }
 
 // The autotuner: on the current state, time a 30-iteration PCG solve for
@@ -391,6 +399,15 @@ int igr_set_option(igr_ctx* c, int option, int value) {
     if (c->np == 4) { c->rtile_x = tx; c->rtile_y = ty; }
     return IGR_OK;
   }
+  if (option == IGR_OPT_GROUPS) {
+    if (value < 0) return seterr(*c, IGR_EINVAL, "workgroup count must be >= 0");
+    if (value != c->bgroups) {
+      (void)hipStreamSynchronize(c->stream);
+      destroy_graphs(*c);   // captured steps hold launches of the old grid
+      c->bgroups = value;
+    }
+    return IGR_OK;
+  }
   if (option == IGR_OPT_AUTOTUNE) {
     if (c->dim != 2 || c->np != 4) return IGR_OK;   // one tile shape exists for the other cases
     int rc = begin_call(*c);
@@ -408,6 +425,7 @@ int igr_get_option(const igr_ctx* c, int option, int* value) {
   else if (option == IGR_OPT_TILE) *value = 100 * c->tile_x + c->tile_y;
   else if (option == IGR_OPT_HTILE) *value = 100 * c->htile_x + c->htile_y;
   else if (option == IGR_OPT_RTILE) *value = 100 * c->rtile_x + c->rtile_y;
+  else if (option == IGR_OPT_GROUPS) *value = c->bgroups;
   else return IGR_EINVAL;
   return IGR_OK;
 }
@@ -596,15 +614,38 @@ int igr_run_forward(igr_ctx* c, double dt, double t_end, double gamma, double al
 }
 
 int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps, double* ms) {
-  if (!c || !ms || reps < 1 || reps > 4096 || kernel < 0 || kernel > 3 || c->dim != 2) return IGR_EINVAL;
+  if (!c || !ms || reps < 1 || reps > 4096 || kernel < 0 || kernel > 5 || c->dim != 2) return IGR_EINVAL;
   int rc = begin_call(*c);
   if (rc) return rc;
   const size_t N = c->N;
-  if (kernel == 2) {
+  // events around a batch of back-to-back launches: the mean in-stream time
+  // per launch, gaps included
+  auto batch_ms = [&](auto&& launch) -> int {
+    hipEvent_t e0, e1;
+    HIPCHK(*c, hipEventCreate(&e0));
+    HIPCHK(*c, hipEventCreate(&e1));
+    for (int w = 0; w < 3; ++w) launch();
+    HIPCHK(*c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHK(*c, hipEventRecord(e1, c->stream));
+    HIPCHK(*c, hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(*c, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return IGR_OK;
+  };
+  if (kernel == 2 || kernel == 5) {
     // the hyperbolic RHS + RK3 stage-2 update, launched back to back on the current state
     HypArgs h = hyp_base(*c, gamma);
     h.q = c->q; h.mode = 2; h.cdt = 1e-6; h.out = c->q2; h.qn = c->q1;
     launch_copy(*c, c->q1, c->q, c->nv * N);
+    if (kernel == 5) {
+      rc = batch_ms([&]() { launch2_hyp(*c, h); });
+      if (rc) return rc;
+      return end_call(*c);
+    }
     std::vector<hipEvent_t> ev(2 * (size_t)reps);
     for (auto& e : ev) HIPCHK(*c, hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
     for (int w = 0; w < 3; ++w) launch2_hyp(*c, h);
@@ -636,11 +677,13 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
   a.invR_out = c->invR; a.invM_out = c->invM; a.rout = c->r; a.sc = c->scal; a.n_iter = reps; a.part_out = R0;
   launch2_sip(*c, M_INIT_PCG, a);
   double* pbuf[2] = {c->pA, c->pB};
-  if (kernel == 3) {
-    // PCG_A of iteration 1 launched `reps` times back to back, events around
-    // the whole batch only: its inputs (r, 1/M, p_0, 1/rho, the pAp partials)
-    // do not change, so every launch does the same work, and the mean is the
-    // in-stream duration plus the gap between launches.
+  if (kernel == 3 || kernel == 4) {
+    // PCG_A (3) or PCG_B (4) of iteration 1 launched `reps` times back to
+    // back, events around the whole batch only.  PCG_A's inputs (r, 1/M,
+    // p_0, 1/rho, the pAp partials) do not change, so every launch does the
+    // same work; PCG_B updates x and r in place with the same fields and
+    // scalars each time.  The mean is the in-stream duration plus the gap
+    // between launches.
     const int nit = std::max(reps, 2);
     rc = ensure_scal(*c, nit);
     if (rc) return rc;
@@ -649,20 +692,9 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
       k.sc = c->scal; k.n_iter = nit; k.it = it; k.first = (it == 0);
       k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = pbuf[(it + 1) % 2]; k.pnew = pbuf[it % 2];
       k.y = c->Ap; k.part_in = R0; k.part_out = R1;
-      if (it == 1) {
-        hipEvent_t e0, e1;
-        HIPCHK(*c, hipEventCreate(&e0));
-        HIPCHK(*c, hipEventCreate(&e1));
-        for (int w = 0; w < 3; ++w) launch2_sip(*c, M_PCG_A, k);
-        HIPCHK(*c, hipEventRecord(e0, c->stream));
-        for (int r = 0; r < reps; ++r) launch2_sip(*c, M_PCG_A, k);
-        HIPCHK(*c, hipEventRecord(e1, c->stream));
-        HIPCHK(*c, hipEventSynchronize(e1));
-        float t = 0.f;
-        HIPCHK(*c, hipEventElapsedTime(&t, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *ms = (double)t / reps;
+      if (it == 1 && kernel == 3) {
+        rc = batch_ms([&]() { launch2_sip(*c, M_PCG_A, k); });
+        if (rc) return rc;
         return end_call(*c);
       }
       launch2_sip(*c, M_PCG_A, k);
@@ -670,6 +702,12 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
       m.sc = c->scal; m.n_iter = nit; m.it = it;
       m.xout = c->xA; m.pold = pbuf[it % 2]; m.y = c->Ap; m.rout = c->r; m.r = nullptr; m.invM = c->invM;
       m.part_in = R1; m.part_out = R0;
+      if (it == 1 && kernel == 4) {
+        // it reads PCG_A's partial sums (R1) and writes R0, which it does not read
+        rc = batch_ms([&]() { launch2_sip(*c, M_PCG_B, m); });
+        if (rc) return rc;
+        return end_call(*c);
+      }
       launch2_sip(*c, M_PCG_B, m);
     }
   }

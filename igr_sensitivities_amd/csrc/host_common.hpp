@@ -69,6 +69,7 @@ void solve2_fields(Ctx& c, const double* mx, const double* my, const double* rho
 void make_basis(int p, double* xi, double* w, double* Dm);
 void default_tiles(Ctx& c);
 int grid2_blocks(const Ctx& c);
+int grid2_bblocks(const Ctx& c);
 void launch_fill(const Ctx& c, double* x, size_t n, double v);
 K1Args k1_base(Ctx& c, double gamma, double alpha, int n_iter, int solver);
 int run1(Ctx& c, const K1Args& a);

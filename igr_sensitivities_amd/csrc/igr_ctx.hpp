@@ -75,6 +75,9 @@ struct Ctx {
   int rtile_x = 4, rtile_y = 4;
   int htile_x = 4, htile_y = 4;
   int use_graphs = 1;
+  // workgroups of the PCG x/r update (k2_pcg_b): 0 = one per tile; n > 0 =
+  // at most n, each running every n-th tile
+  int bgroups = 0;
 
   // run_forward_store snapshots: records of (state, Sigma), (nv + 1) N
   // doubles each, for the continuous adjoint (adjpde.hip)

@@ -33,6 +33,8 @@ struct SipArgs {
   const double* b_in = nullptr;  // INIT: use this right-hand side instead of the IGR source
   double* sc = nullptr;          // per-solve scalars (rz | pAp | a | beta)
   int first = 0, it = 0, n_iter = 0;
+  int ngroups = 0;               // workgroups of the launch (set by the launcher)
+  int np_in = 0;                 // partial sums the previous kernel wrote (set by the launcher)
   double c1 = 0.0, c2 = 0.0, theta = 1.0, omega = 0.0;
   double* y = nullptr;
   double* pnew = nullptr;

@@ -73,7 +73,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   double beta = 0.0, sc_tot = 0.0;
   // PCG_A: the previous kernel's partial sums are requested first
   double pre[1] = {0.0};
-  if constexpr (MODE == M_PCG_A) grid_prefetch<1>(a.part_in, nblk, pre);
+  if constexpr (MODE == M_PCG_A) grid_prefetch<1>(a.part_in, a.np_in, pre);
 
   // Tile + halo loads.  Every slot's node index first, then all the loads
   // with no branch between them (an empty slot reads node 0 and is zeroed
@@ -117,7 +117,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     const int nit = a.n_iter, it = a.it;
     const double rz_prev = a.first ? 0.0 : a.sc[it - 1];
     double tot[1];
-    grid_finish<1, GU>(a.part_in, nblk, pre, tot);
+    grid_finish<1, GU>(a.part_in, a.np_in, pre, tot);
     if (!a.first) beta = tot[0] / (rz_prev + 1e-30);
     sc_tot = tot[0];   // stored with the outputs, after the last barrier
     (void)nit;
@@ -343,36 +343,51 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs*
 
 
 // x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
+// Workgroup blk of nblk processes tiles blk, blk + nblk, ...; the next
+// tile's loads are issued before this tile's stores.
 template <int NP, int TX, int TY, int GU = 8>
 __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
   __shared__ double red[NT / WAVE + 1];
-  TileInfo<NP, TX, TY> t(a.g, blk, nblk);
+  const int ntiles = tiles_of<TX, TY>(a.g);
   const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
   const int lx = s % TX, ly = s / TX;
-  const bool valid = lx < t.nvx && ly < t.nvy;
   double* sc = a.sc;
   const int nit = a.n_iter, it = a.it;
   // the partial sums first, then this thread's fields with no branch around
   // the loads (an invalid thread reads node 0 and stores nothing)
   double pre[1];
-  grid_prefetch<1>(a.part_in, nblk, pre);
-  const size_t gi = valid ? t.gidx(s, n) : 0;
+  grid_prefetch<1>(a.part_in, a.np_in, pre);
   const double* rin = a.r ? a.r : a.rout;
-  const double x = a.xout[gi], p = a.pold[gi], r = rin[gi], ap = a.y[gi], im = a.invM[gi];
+  bool valid;
+  size_t gi;
+  double x, p, r, ap, im;
+  auto issue = [&](int tt) {
+    const TileInfo<NP, TX, TY> t(a.g, tt, ntiles);
+    valid = lx < t.nvx && ly < t.nvy;
+    gi = valid ? t.gidx(s, n) : 0;
+    x = a.xout[gi]; p = a.pold[gi]; r = rin[gi]; ap = a.y[gi]; im = a.invM[gi];
+  };
+  issue(blk);
   const double rz_it = sc[it];
   double tot[1];
-  grid_finish<1, GU>(a.part_in, nblk, pre, tot);
+  grid_finish<1, GU>(a.part_in, a.np_in, pre, tot);
   const double ak = rz_it / (tot[0] + 1e-30);
-  const double xn = x + ak * p;
-  const double rn = r - ak * ap;
-  // the reduction's barrier first: stores ahead of a barrier are waited for
-  double red_v[1] = {valid ? rn * (rn * im) : 0.0};
-  block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
-  if (valid) {
-    a.xout[gi] = xn;
-    a.rout[gi] = rn;
+  double acc = 0.0;
+  for (int tt = blk; tt < ntiles; tt += nblk) {
+    const double xn = x + ak * p;
+    const double rn = r - ak * ap;
+    const bool v = valid;
+    const size_t g0 = gi;
+    acc += v ? rn * (rn * im) : 0.0;
+    issue(tt + nblk < ntiles ? tt + nblk : tt);   // the last tile again after the last: no branch
+    if (v) {
+      a.xout[g0] = xn;
+      a.rout[g0] = rn;
+    }
   }
+  double red_v[1] = {acc};
+  block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
   if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
 }
 template <int NP, int TX, int TY>
@@ -384,7 +399,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b_batch(const SipArg
                                                                     const int2* __restrict__ work) {
   const int2 w = work[blockIdx.x];
   const SipArgs& a = tab[w.x];
-  k2_pcg_b_body<NP, TX, TY, 1>(a, w.y, tiles_of<TX, TY>(a.g));
+  k2_pcg_b_body<NP, TX, TY, 1>(a, w.y, a.ngroups);
 }
 
 
@@ -635,6 +650,9 @@ __global__ void k_copy_batch(const CopyArgs* __restrict__ tab) {
 // ------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------
+int grid2_blocks(const Ctx& c);
+int grid2_bblocks(const Ctx& c);
+
 // Kernel-timestamp events for the next SIP/hyperbolic launch (set by
 // igr_time_kernel only): hipExtLaunchKernelGGL stamps them at the start and
 // end of the kernel itself, so their elapsed time excludes launch gaps.
@@ -659,7 +677,7 @@ struct Launch2 {
   }
   static dim3 block() { return dim3(TX * TY * NP * NP); }
   static void sip(int mode, const SipArgs& a, hipStream_t st) {
-    dim3 G = grid(a.g), Bk = block();
+    dim3 G(a.ngroups), Bk = block();
     switch (mode) {
       case M_APPLY: launch_k(k2_sip<NP, TX, TY, M_APPLY>, G, Bk, st, a); break;
       case M_INIT_PCG: launch_k(k2_sip<NP, TX, TY, M_INIT_PCG>, G, Bk, st, a); break;
@@ -738,7 +756,14 @@ void launch_copy_batch(const CopyArgs* tab, int n, size_t maxn, hipStream_t st) 
   hipLaunchKernelGGL(k_copy_batch, dim3(std::max(gx, 1u), n), dim3(256), 0, st, tab);
 }
 
-void launch2_sip(const Ctx& c, int mode, const SipArgs& a) {
+// The x/r update of PCG (PCG_B) runs grid2_bblocks(c) workgroups, every
+// other SIP-family kernel one per tile (grid2_blocks(c)); each kernel is
+// told how many partial sums the kernel before it wrote.
+void launch2_sip(const Ctx& c, int mode, const SipArgs& a0) {
+  SipArgs a = a0;
+  const int nt = grid2_blocks(c), nbb = grid2_bblocks(c);
+  a.ngroups = mode == M_PCG_B ? nbb : nt;
+  a.np_in = mode == M_PCG_B ? nt : (mode == M_PCG_A ? (a.first ? nt : nbb) : 0);
   if (g_rec) { g_rec->recs.push_back({R_SIP, mode, g_rec->sip.size()}); g_rec->sip.push_back(a); return; }
   IGR_SIP_TILE_DISPATCH(Launch2, c.np, c.tile_x, c.tile_y, sip(mode, a, c.stream));
 }
@@ -754,13 +779,21 @@ int grid2_hblocks(const Ctx& c) {
   else if (c.np == 4) { tx = c.htile_x; ty = c.htile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
-// workgroups of one SIP-family launch (the number of partial sums)
+// workgroups of one SIP-family launch (the number of partial sums it
+// writes): one per tile
 int grid2_blocks(const Ctx& c) {
   int tx = 4, ty = 4;
   if (c.np == 2) { tx = 8; ty = 8; }
   else if (c.np == 5) { tx = 2; ty = 4; }
   else if (c.np == 4) { tx = c.tile_x; ty = c.tile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
+}
+// workgroups of the PCG x/r update: at most c.bgroups (0: one per tile),
+// each running every n-th tile with the next tile's loads issued before the
+// current tile's stores
+int grid2_bblocks(const Ctx& c) {
+  const int nt = grid2_blocks(c);
+  return c.bgroups > 0 ? std::min(nt, c.bgroups) : nt;
 }
 // workgroups of one reverse SIP-family launch
 int grid2_rblocks(const Ctx& c) {
@@ -770,7 +803,9 @@ int grid2_rblocks(const Ctx& c) {
   else if (c.np == 4) { tx = c.rtile_x; ty = c.rtile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
-void launch_pcg_final(const Ctx& c, const double* part, int nb, double* sc, int nit) {
+// the partial sums are the last PCG_B's (n_iter > 0) or the INIT kernel's
+void launch_pcg_final(const Ctx& c, const double* part, double* sc, int nit) {
+  const int nb = nit > 0 ? grid2_bblocks(c) : grid2_blocks(c);
   if (g_rec) { g_rec->recs.push_back({R_FINAL, 0, g_rec->fin.size()}); g_rec->fin.push_back({part, nb, sc, nit}); return; }
   hipLaunchKernelGGL(k2_pcg_final, dim3(1), dim3(256), 0, c.stream, part, nb, sc, nit);
 }

@@ -63,10 +63,18 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
  *  IGR_OPT_AUTOTUNE (value ignored) time the forward shapes on the current
  *                   state and keep the fastest of each family;
  *  IGR_OPT_RTILE    the same for the reverse SIP / solver kernels of the
- *                   gradient and the VJPs (p = 3: 404, 804, 808, 202).
+ *                   gradient and the VJPs (p = 3: 404, 804, 808, 202);
+ *  IGR_OPT_GROUPS   workgroups of the PCG x/r update kernel (2D): 0 = one per
+ *                   SIP tile, n > 0 = at most n, each running every n-th
+ *                   tile with the next tile's loads issued before the
+ *                   current tile's stores (the default comes from the size
+ *                   table; results change only in the dot products' order).
  * The defaults come from a size table measured on MI355X.  The shapes change
  * results only through the summation order of the solvers' dot products. */
-enum igr_option { IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4 };
+enum igr_option {
+  IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4,
+  IGR_OPT_GROUPS = 5
+};
 int igr_set_option(igr_ctx* ctx, int option, int value);
 int igr_get_option(const igr_ctx* ctx, int option, int* value);
 
@@ -298,7 +306,8 @@ int igr_solve_elliptic_noalloc(int p, int nex, int ney, double lx, double ly, do
  * kernel 2 (hyperbolic RHS + RK3 stage update) over `reps` launches.
  * kernel 3: kernel 0 of iteration 1 launched `reps` times back to back with
  * events around the batch only (per-launch stamps add their own overhead):
- * the mean in-stream time per launch, gaps included. */
+ * the mean in-stream time per launch, gaps included; kernel 4: the same for
+ * kernel 1; kernel 5: the same for kernel 2. */
 int igr_time_kernel(igr_ctx* ctx, int kernel, double gamma, double alpha, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus

@@ -23,8 +23,10 @@ def make_runs(solver_alpha, seed=11):
         gam = float(rng.uniform(1.2, 1.67))
         dt = 0.3 * (1.0 / n) / (7 * configs.max_wavespeed2(q0, gam))
         s0 = rng.standard_normal((nt, 4) + mesh.shape) * 0.1 if nt else None
+        # instances 1 and 2 run the PCG x/r update on fewer workgroups than
+        # tiles (9 tiles on 4, 16 on 5: uneven tiles per workgroup)
         runs.append(dict(mesh=mesh, q0=q0, s0=s0, dt=dt, T=(3 + k) * dt * 0.97, gamma=gam,
-                         alpha=solver_alpha[1] * (1.0 if k != 2 else 0.5)))
+                         alpha=solver_alpha[1] * (1.0 if k != 2 else 0.5), groups=(0, 4, 5, 0)[k]))
     # 1D: N_e = 64 (256 nodes, smaller workgroup) and 256/320 (1024-thread workgroups)
     for k, (ne, nt) in enumerate([(256, 2), (64, 1), (320, 4)]):
         mesh = igr.PeriodicMesh1D(ne, 1.0, igr.DGBasis(3))
@@ -47,6 +49,8 @@ def test_batch_equals_single_runs(solver_alpha):
     ref = []
     for r in runs:
         c = igr.Context(r["mesh"])
+        if r.get("groups"):
+            c.set_option(igr.core.OPT_GROUPS, r["groups"])
         c.set_state(r["q0"])
         s0 = r["s0"] if r["s0"] is not None else np.empty((0, c.nv) + c.shape)
         if s0.shape[0]:
@@ -63,6 +67,8 @@ def test_batch_equals_single_runs(solver_alpha):
     ctxs = []
     for r in runs:
         c = igr.Context(r["mesh"])
+        if r.get("groups"):
+            c.set_option(igr.core.OPT_GROUPS, r["groups"])
         c.set_state(r["q0"])
         ctxs.append(c)
     out = igr.run_sensitivity_batch([dict(r, ctx=c) for r, c in zip(runs, ctxs)], n_iter=n_iter, solver=solver)

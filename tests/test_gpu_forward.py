@@ -197,7 +197,21 @@ def test_tile_shapes_agree(orc, nx, ny):
         c.set_state(q0)
         c.run_steps(2, dt, 1.4, 2e-3, 60, "pcg")
         assert rel(c.get_state()[0], ref) < 1e-11, (tile, htile)
+    # the PCG x/r update on fewer workgroups than tiles, each running several
+    # (uneven counts: 36 or 15 tiles of 4x4 on 7 workgroups)
+    for groups in (7, 3):
+        c = ctx2(3, nx, ny)
+        c.set_option(igr.core.OPT_TILE, 404)
+        c.set_option(igr.core.OPT_GROUPS, groups)
+        assert c.get_option(igr.core.OPT_GROUPS) == groups
+        for solver in ("pcg", "jacobi"):
+            c.set_state(q0)
+            c.run_steps(2, dt, 1.4, 2e-3, 60, solver)
+            want = ref if solver == "pcg" else g.run_steps(1.4, 2e-3, 60, orc.JACOBI, 2, dt, q0)[0]
+            assert rel(c.get_state()[0], want) < 1e-11, (groups, solver)
     c = ctx2(3, nx, ny)
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_GROUPS, -1)
     with pytest.raises(ValueError):
         c.set_option(igr.core.OPT_HTILE, 808)     # no 8x8 hyperbolic tile
     with pytest.raises(ValueError):
