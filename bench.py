@@ -144,6 +144,48 @@ def c5_leg(igr, configs, budget_slots=40):
     return out
 
 
+def p4_leg(igr, configs, fwd_steps=60):
+    """The reference's p = 4 production run (the triple-Sedov heatmap of
+    IGR_adjoints_2d/paper_scripts/generate_sedov_heatmap_data.jl: 160 x 160,
+    PCG-60, T = 0.10): forward throughput over `fwd_steps` steps, the three
+    forward kernels' fraction of HBM peak, and the script's whole GPU work,
+    run_forward_store over T followed by run_adjoint_conservative of the
+    kinetic energy."""
+    cfg = configs.sedov_heatmap_p4()
+    N = cfg["N"]
+    ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(4)))
+    gam, al, dt, n_iter = cfg["gamma"], cfg["alpha"], cfg["dt"], cfg["n_iter"]
+    n_nodes = 25 * N * N
+    out = {"workload": "2D triple Sedov (the reference's p = 4 heatmap run), 160x160 elements, p=4, PCG-60, "
+                       "alpha = 3 (1/20)^2, CFL 0.4, T = 0.10", "nodes": n_nodes, "steps_to_T": cfg["n_steps"],
+           "tiles": {"sip": ctx.get_option(igr.core.OPT_TILE), "pcg_b_workgroups": ctx.get_option(igr.core.OPT_GROUPS)}}
+    try:
+        ctx.set_state(cfg["q"])
+        ctx.run_steps(2, dt, gam, al, n_iter, "pcg")
+        ctx.set_state(cfg["q"])
+        t0 = time.perf_counter()
+        ctx.run_steps(fwd_steps, dt, gam, al, n_iter, "pcg")
+        t = time.perf_counter() - t0
+        out["forward_ms_per_step"] = t / fwd_steps * 1e3
+        out["dof_stage_per_s"] = 4 * n_nodes * 3 * fwd_steps / t
+        for name, kern, fields in KERNEL_FIELDS:
+            ms = ctx.time_kernel(kern, gam, al, 200)
+            alg = fields * 8 * n_nodes
+            out[name] = {"us": ms * 1e3, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        ctx.set_state(cfg["q"])
+        t0 = time.perf_counter()
+        n = ctx.run_forward_store(dt, cfg["T"], gam, al, n_iter, "pcg")
+        t1 = time.perf_counter()
+        ctx.run_adjoint_conservative(igr.OBJECTIVES["kinetic_energy"], dt, cfg["T"], gam, al, n_iter, "pcg")
+        t2 = time.perf_counter()
+        out.update({"forward_store_steps": n, "forward_store_s": t1 - t0, "adjoint_s": t2 - t1,
+                    "heatmap_run_s": t2 - t0})
+    except (igr.DomainError, igr.IGRError, ValueError) as e:
+        out["error"] = f"{type(e).__name__}: {e}"
+    ctx.close()
+    return out
+
+
 def c1_leg(igr, configs, cpu_steps=200):
     """C1: the 1D blast to t = 0.038 (N_e = 128, p = 3, Jacobi-50, CFL 0.1,
     about 12,700 SSP-RK3 steps), one persistent workgroup; the CPU oracle
@@ -248,6 +290,7 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the 256x256 blast-vortex forward + gradient leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1D blast run")
     ap.add_argument("--no-adjpde", action="store_true", help="skip the continuous-adjoint leg")
+    ap.add_argument("--no-p4", action="store_true", help="skip the p = 4 heatmap-run leg")
     ap.add_argument("--graphs-off", action="store_true",
                     help="launch every kernel directly instead of replaying captured hipGraphs (for profilers)")
     args = ap.parse_args()
@@ -320,6 +363,7 @@ def main():
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)
     blast1 = None if args.no_c1 else c1_leg(igr, configs)
+    p4 = None if args.no_p4 else p4_leg(igr, configs)
     sweep = None if args.no_sweep else size_sweep_leg(igr, configs)
     # the kernels at the 256x256 size of the blast-vortex workload
     big = None
@@ -402,6 +446,7 @@ def main():
         "ensemble": ens,
         "blast_vortex_256": c5,
         "blast_1d": blast1,
+        "p4_sedov_heatmap_160": p4,
         "kernel_size_sweep": sweep,
         "cpu_baseline": cpu,
     }

@@ -324,6 +324,26 @@ def config5(N=256, p=3, seed=5, gamma=1.4, n_steps=400, cfl=0.4):
                 p=p, N=N, disc=disc)
 
 
+def sedov_heatmap_p4(N=160, p=4, gamma=1.4, cfl=0.4, T=0.10):
+    """The p = 4 workload of the reference's triple-Sedov heatmap runs
+    (IGR_adjoints_2d/paper_scripts/generate_sedov_heatmap_data.jl:41-96):
+    160 x 160 elements, alpha = 3 (L/20)^2, rho = 1, u = 0, p = 1 + three
+    Gaussian over-pressures (A = 6, 5, 5.5 at (0.30, 0.34), (0.70, 0.40),
+    (0.52, 0.74), sigma = 0.09), CFL 0.4 on max(lambda_x, lambda_y) over
+    (2p + 1), n_steps = ceil(T / dt), PCG-60."""
+    x, y = coords2(p, N, N)
+    pr = np.ones_like(x)
+    for xc, yc, A in ((0.30, 0.34, 6.0), (0.70, 0.40, 5.0), (0.52, 0.74, 5.5)):
+        pr += A * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / 0.09 ** 2)
+    q = np.zeros((4,) + x.shape)
+    q[2] = 1.0
+    q[3] = pr / (gamma - 1.0)
+    dt = cfl * (1.0 / N) / ((2 * p + 1) * max_wavespeed2(q, gamma))
+    n_steps = int(math.ceil(T / dt))
+    return dict(q=q, gamma=gamma, alpha=3.0 * (1.0 / 20) ** 2, dt=T / n_steps, n_steps=n_steps, T=T, n_iter=60,
+                solver="pcg", p=p, N=N)
+
+
 def taylor_green2(p, Nex, Ney, A=0.1, gamma=1.4, Lx=1.0, Ly=1.0):
     """build_taylor_green_ic! (IGR_adjoints_2d/initial_conditions.jl:105-121)."""
     x, y = coords2(p, Nex, Ney, Lx, Ly)

@@ -31,8 +31,12 @@ void launch1_batch(hipStream_t st, int np, const K1Args* dargs, int n_inst, int 
 
 namespace {
 
-int tile_x_of(const Ctx& c, bool h) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 2 : (h ? c.htile_x : c.tile_x); }
-int tile_y_of(const Ctx& c, bool h) { return c.np == 2 ? 8 : c.np == 3 ? 4 : c.np == 5 ? 4 : (h ? c.htile_y : c.tile_y); }
+int tile_x_of(const Ctx& c, bool h) {
+  return c.np == 2 ? 8 : c.np == 3 ? 4 : (c.np == 5 && h) ? 2 : (h ? c.htile_x : c.tile_x);
+}
+int tile_y_of(const Ctx& c, bool h) {
+  return c.np == 2 ? 8 : c.np == 3 ? 4 : (c.np == 5 && h) ? 4 : (h ? c.htile_y : c.tile_y);
+}
 
 struct DevBufs {
   std::vector<void*> p;

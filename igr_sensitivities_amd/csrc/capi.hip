@@ -219,6 +219,10 @@ int k1_block(const Ctx& c) {  // threads of the 1D whole-instance workgroup
 bool tile_ok(int tx, int ty, bool hyp) {
   return (tx == 4 && ty == 4) || (tx == 8 && ty == 4) || (tx == 2 && ty == 2) || (!hyp && tx == 8 && ty == 8);
 }
+// SIP-family tiles at p = 4
+bool tile_ok5(int tx, int ty) {
+  return (tx == 2 && ty == 4) || (tx == 4 && ty == 4) || (tx == 5 && ty == 5) || (tx == 8 && ty == 5);
+}
 void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
   if (tx == c.tile_x && ty == c.tile_y && htx == c.htile_x && hty == c.htile_y) return;
   (void)hipStreamSynchronize(c.stream);
@@ -237,6 +241,20 @@ void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
 // mesh is fixed, so repeated runs give bit-identical results; the autotuner
 // may choose other forward shapes.
 void default_tiles(Ctx& c) {
+  if (c.np == 5) {
+    // p = 4: the SIP family's tile by mesh size (the other families keep
+    // 2x4).  At 160^2 (the reference's p = 4 heatmap runs), one tile per x/r
+    // workgroup, a PCG-60 step takes 5.41 ms with 8x5 tiles, 5.81 (4x4),
+    // 5.86 (5x5) and 6.10 (2x4); with the x/r update on 256 workgroups 5x5
+    // tiles take 5.09 ms and 8x5 5.24.
+    const long long E = (long long)c.Nex * c.Ney;
+    if (E >= 96 * 96) { c.tile_x = 5; c.tile_y = 5; }
+    else if (E >= 32 * 32) { c.tile_x = 4; c.tile_y = 4; }
+    else { c.tile_x = 2; c.tile_y = 4; }
+    const int nt = ((c.Nex + c.tile_x - 1) / c.tile_x) * ((c.Ney + c.tile_y - 1) / c.tile_y);
+    c.bgroups = nt > 512 ? std::max(256, (nt / 16 + 7) / 8 * 8) : 0;
+    return;
+  }
   if (c.np != 4) return;
   const long long E = (long long)c.Nex * c.Ney;
   if (E < 96 * 96) { c.tile_x = 4; c.tile_y = 4; c.htile_x = 4; c.htile_y = 4; }
@@ -388,6 +406,12 @@ int igr_set_option(igr_ctx* c, int option, int value) {
   if (option == IGR_OPT_TILE || option == IGR_OPT_HTILE) {
     const int tx = value / 100, ty = value % 100;
     const bool h = option == IGR_OPT_HTILE;
+    if (c->np == 5) {
+      if (h) return seterr(*c, IGR_EINVAL, "the hyperbolic family has one tile shape at p = 4");
+      if (!tile_ok5(tx, ty)) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
+      set_tiles(*c, tx, ty, c->htile_x, c->htile_y);
+      return IGR_OK;
+    }
     if (!tile_ok(tx, ty, h)) return seterr(*c, IGR_EINVAL, "unsupported tile shape");
     if (c->np == 4) set_tiles(*c, h ? c->tile_x : tx, h ? c->tile_y : ty, h ? tx : c->htile_x, h ? ty : c->htile_y);
     return IGR_OK;

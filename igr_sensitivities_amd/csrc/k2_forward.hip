@@ -344,7 +344,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs*
 
 // x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
 // Workgroup blk of nblk processes tiles blk, blk + nblk, ...; the next
-// tile's loads are issued before this tile's stores.
+// tile's loads are issued before this tile's stores.  The first tile's loads
+// come before the grid total of the previous kernel's partial sums.
 template <int NP, int TX, int TY, int GU = 8>
 __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
@@ -380,7 +381,7 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
     const bool v = valid;
     const size_t g0 = gi;
     acc += v ? rn * (rn * im) : 0.0;
-    issue(tt + nblk < ntiles ? tt + nblk : tt);   // the last tile again after the last: no branch
+    if (tt + nblk < ntiles) issue(tt + nblk);      // uniform branch: the workgroup's next tile
     if (v) {
       a.xout[g0] = xn;
       a.rout[g0] = rn;
@@ -712,7 +713,9 @@ struct Launch2 {
 
 // (np, tile) -> the instantiated tile shape.  p = 3 (np = 4): the SIP /
 // solver family has 4x4, 8x4, 8x8 and 2x2 tiles, the hyperbolic family
-// 4x4, 8x4 and 2x2 (an 8x8 hyperbolic tile would not fit the LDS).
+// 4x4, 8x4 and 2x2 (an 8x8 hyperbolic tile would not fit the LDS).  p = 4
+// (np = 5, 25 nodes per element): the SIP family has 2x4 (200 threads),
+// 4x4 (400), 5x5 (625) and 8x5 (1000) tiles, the others 2x4.
 #define IGR_SIP_TILE_DISPATCH(L, np, tx, ty, CALL)                      \
   switch (np) {                                                         \
     case 2: L<2, 8, 8>::CALL; break;                                    \
@@ -723,7 +726,12 @@ struct Launch2 {
       else if ((tx) == 2 && (ty) == 2) L<4, 2, 2>::CALL;                \
       else L<4, 4, 4>::CALL;                                            \
       break;                                                            \
-    case 5: L<5, 2, 4>::CALL; break;                                    \
+    case 5:                                                             \
+      if ((tx) == 8 && (ty) == 5) L<5, 8, 5>::CALL;                     \
+      else if ((tx) == 5 && (ty) == 5) L<5, 5, 5>::CALL;                \
+      else if ((tx) == 4 && (ty) == 4) L<5, 4, 4>::CALL;                \
+      else L<5, 2, 4>::CALL;                                            \
+      break;                                                            \
   }
 #define IGR_HYP_TILE_DISPATCH(L, np, tx, ty, CALL)                      \
   switch (np) {                                                         \
@@ -784,8 +792,7 @@ int grid2_hblocks(const Ctx& c) {
 int grid2_blocks(const Ctx& c) {
   int tx = 4, ty = 4;
   if (c.np == 2) { tx = 8; ty = 8; }
-  else if (c.np == 5) { tx = 2; ty = 4; }
-  else if (c.np == 4) { tx = c.tile_x; ty = c.tile_y; }
+  else if (c.np == 4 || c.np == 5) { tx = c.tile_x; ty = c.tile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
 // workgroups of the PCG x/r update: at most c.bgroups (0: one per tile),

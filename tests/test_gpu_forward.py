@@ -431,3 +431,28 @@ def test_other_degrees_run_1d(orc, p):
         tq = max(1e-9, 10 * rel(qTp, qTr))
         tS = max(1e-9, 10 * rel(STp, STr))
         assert ns == nsr and rel(qT, qTr) < tq and rel(ST, STr) < tS, (solver, rel(qT, qTr), tq, rel(ST, STr), tS)
+
+
+def test_p4_tile_shapes_agree(orc):
+    """p = 4 (25 nodes per element): every SIP-family tile shape (2x4, 4x4,
+    5x5, 8x5: 200 to 1000 threads) against the oracle, on a mesh that none of
+    them divides evenly."""
+    nx, ny = 11, 9
+    q0 = configs.smooth_random2(4, nx, ny, seed=150, amp=0.3)
+    dt = 0.3 * (1 / nx) / (9 * configs.max_wavespeed2(q0, 1.4))
+    g = orc.Grid2D(4, nx, ny)
+    ref, Sref = g.run_steps(1.4, 3e-3, 40, orc.PCG, 2, dt, q0)
+    for tile in (204, 404, 505, 805):
+        c = ctx2(4, nx, ny)
+        c.set_option(igr.core.OPT_TILE, tile)
+        assert c.get_option(igr.core.OPT_TILE) == tile
+        c.set_state(q0)
+        c.run_steps(2, dt, 1.4, 3e-3, 40, "pcg")
+        q, S = c.get_state()
+        assert rel(q, ref) < 1e-11, tile
+        assert rel(S, Sref) < 1e-10, tile
+    c = ctx2(4, nx, ny)
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_TILE, 808)
+    with pytest.raises(ValueError):
+        c.set_option(igr.core.OPT_HTILE, 404)
