@@ -16,8 +16,40 @@ struct BasisK {
   double xi[MAXNP];
 };
 
-// Wavefront sum (64 lanes) by butterfly shuffles; every lane gets the sum.
+// v taken from another lane by a DPP lane pattern (two 32-bit DPP moves).
+// Lanes outside row_mask, or whose source lane is out of range, get 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_take(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wavefront sum (64 lanes); every lane gets the sum, in a fixed order.
+// A workgroup whose size is a multiple of 64 (every lane of every wave
+// active) sums by DPP lane moves: pairs, quads, half rows and rows of 16
+// (quad_perm, row_half_mirror, row_mirror), then the four row sums through
+// row_bcast:15 and row_bcast:31 into lane 63, read back as a uniform value.
+// These are VALU moves, not LDS round trips.  Otherwise (a partial last
+// wave, whose absent lanes DPP cannot skip) butterfly shuffles, where an
+// absent lane reads as 0.
 __device__ __forceinline__ double wave_sum(double v) {
+  if ((blockDim.x & (WAVE - 1)) == 0) {
+    v += dpp_take<0xB1>(v);         // quad_perm [1,0,3,2]
+    v += dpp_take<0x4E>(v);         // quad_perm [2,3,0,1]
+    v += dpp_take<0x141>(v);        // row_half_mirror
+    v += dpp_take<0x140>(v);        // row_mirror
+    v += dpp_take<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_take<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    return read_lane(v, WAVE - 1);
+  }
 #pragma unroll
   for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
   return v;

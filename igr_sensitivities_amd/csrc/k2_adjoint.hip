@@ -93,9 +93,6 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
       Fyb[v] = -g.invJy * sy;
     }
     double lxb = 0.0, lyb = 0.0;
-    const double ir = 1.0 / qo[2];
-    const double Po = (gam - 1.0) * (qo[3] - (qo[0] * qo[0] + qo[1] * qo[1]) / (2.0 * qo[2]));
-    const double co = sqrt(gam * Po * ir);
     const double wl = 1.0 / B.w[NP - 1], w0 = 1.0 / B.w[0];
     // neighbour face node (tile LDS or halo) -> q and D-bar
     auto nb = [&](int sl, int node, int hk, double qn[4], double dn[4]) {
@@ -108,6 +105,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
         for (int v = 0; v < 4; ++v) { qn[v] = hq[v][h * NP + hk]; dn[v] = hdb[v][h * NP + hk]; }
       }
     };
+    // |u| + c by the forward kernel's expression on both sides of a face: the
+    // reverse must break a tie of the two speeds the way the forward did
     auto wave = [&](const double qn[4], int dir) {
       const double Pn = (gam - 1.0) * (qn[3] - (qn[0] * qn[0] + qn[1] * qn[1]) / (2.0 * qn[2]));
       return fabs(qn[dir] / qn[2]) + sqrt(gam * Pn / qn[2]);
@@ -118,7 +117,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
       double qn[4], dn[4];
       if (own_is_L) nb(t.nb_right(s), NP * j, j, qn, dn);
       else nb(t.nb_left(s), (NP - 1) + NP * j, j, qn, dn);
-      const double lo = fabs(qo[0] * ir) + co, ln = wave(qn, 0);
+      const double lo = wave(qo, 0), ln = wave(qn, 0);
       const double lamL = own_is_L ? lo : ln, lamR = own_is_L ? ln : lo;
       const double lam = fmax_first(lamL, lamR);
       double lb = 0.0;
@@ -140,7 +139,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
       double qn[4], dn[4];
       if (own_is_L) nb(t.nb_top(s), i, i, qn, dn);
       else nb(t.nb_bottom(s), i + NP * (NP - 1), i, qn, dn);
-      const double lo = fabs(qo[1] * ir) + co, ln = wave(qn, 1);
+      const double lo = wave(qo, 1), ln = wave(qn, 1);
       const double lamL = own_is_L ? lo : ln, lamR = own_is_L ? ln : lo;
       const double lam = fmax_first(lamL, lamR);
       double lb = 0.0;
@@ -226,11 +225,9 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     bool oks[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
-      const int idx = tid + l * NT;
       size_t gi = 0;
-      const bool ok = idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi);
-      gis[l] = ok ? gi : 0;
-      oks[l] = ok;
+      oks[l] = t.round_global(l, s, n, gi);
+      gis[l] = gi;
     }
     // r-bar and p-bar of the next iteration do not exist in a first reverse
     // iteration, nor P_{k-1}, AP_{k-1}, R_k at k = 0: valid stand-ins are read

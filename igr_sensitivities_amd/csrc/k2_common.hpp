@@ -98,6 +98,7 @@ template <int NP, int TX, int TY>
 struct TileInfo {
   static constexpr int T = TX * TY, NP2 = NP * NP;
   int Nex, Ney, ex0, ey0, nvx, nvy;
+  int exL, exR, eyB, eyT;   // columns / rows of the four halo strips (periodic)
   __device__ TileInfo(const Geo2& g) : TileInfo(g, blockIdx.x, gridDim.x) {}
   // workgroup blk of the nblk workgroups that cover this mesh
   __device__ TileInfo(const Geo2& g, int blk, int nblk) {
@@ -107,6 +108,12 @@ struct TileInfo {
     Nex = g.Nex; Ney = g.Ney;
     ex0 = tx * TX; ey0 = ty * TY;
     nvx = min(TX, Nex - ex0); nvy = min(TY, Ney - ey0);
+    // periodic wrap by comparison (ex0 < Nex, ex0 + nvx <= Nex): a runtime
+    // integer modulo costs tens of instructions ahead of every halo load
+    exL = ex0 > 0 ? ex0 - 1 : Nex - 1;
+    exR = ex0 + nvx < Nex ? ex0 + nvx : ex0 + nvx - Nex;
+    eyB = ey0 > 0 ? ey0 - 1 : Ney - 1;
+    eyT = ey0 + nvy < Ney ? ey0 + nvy : ey0 + nvy - Ney;
   }
   __device__ size_t gidx(int s, int n) const {
     const int lx = s % TX, ly = s / TX;
@@ -116,24 +123,43 @@ struct TileInfo {
   __device__ int nb_right(int s) const { const int lx = s % TX, ly = s / TX; return lx < nvx - 1 ? s + 1 : T + TY + ly; }
   __device__ int nb_bottom(int s) const { const int lx = s % TX, ly = s / TX; return ly > 0 ? s - TX : T + 2 * TY + lx; }
   __device__ int nb_top(int s) const { const int lx = s % TX, ly = s / TX; return ly < nvy - 1 ? s + TX : T + 2 * TY + TX + lx; }
-  // element of LDS slot sl; false when the slot is outside this tile's rows/columns.
-  // Periodic wrap by comparison (ex0 < Nex, ex0 + nvx <= Nex): a runtime
-  // integer modulo costs tens of instructions ahead of every halo load.
+  // Element of LDS slot sl (tile slots 0..T-1, then the left, right, bottom
+  // and top halo strips); false when the slot lies outside this tile's rows
+  // or columns or past the last strip.  Selects only, no branches: a
+  // per-lane branch tree here ran ahead of every tile load.
   __device__ bool slot_elem(int sl, int& gex, int& gey) const {
-    if (sl < T) {
-      const int lx = sl % TX, ly = sl / TX;
-      if (lx >= nvx || ly >= nvy) return false;
-      gex = ex0 + lx; gey = ey0 + ly; return true;
+    const bool in = sl < T;
+    const int lx = sl % TX, ly = sl / TX;
+    const int h = sl - T;
+    const bool xs = h < 2 * TY;                        // left or right strip
+    const int k = xs ? (h < TY ? h : h - TY) : (h < 2 * TY + TX ? h - 2 * TY : h - 2 * TY - TX);
+    gex = in ? ex0 + lx : xs ? (h < TY ? exL : exR) : ex0 + k;
+    gey = in ? ey0 + ly : xs ? ey0 + k : (h < 2 * TY + TX ? eyB : eyT);
+    return in ? (lx < nvx && ly < nvy) : (h < 2 * (TX + TY) && k < (xs ? nvy : nvx));
+  }
+  // Node n of the slot staged by the thread of tile element s in load round l
+  // of a T * NP2-thread workgroup (slot s + l T): round 0 is the tile
+  // element s itself, round l >= 1 the halo slot h = s + (l - 1) T.  The
+  // address is 0 when the slot is empty.  With l known at compile time (an
+  // unrolled loop) this is a handful of selects.
+  __device__ bool round_global(int l, int s, int n, size_t& gi) const {
+    int gex, gey;
+    bool ok;
+    if (l == 0) {
+      const int lx = s % TX, ly = s / TX;
+      ok = lx < nvx && ly < nvy;
+      gex = ex0 + lx; gey = ey0 + ly;
+    } else {
+      const int h = s + (l - 1) * T;
+      const bool xs = h < 2 * TY;                               // left or right strip
+      const bool lo = xs ? h < TY : h < 2 * TY + TX;            // left / bottom
+      const int k = h - (xs ? (lo ? 0 : TY) : (lo ? 2 * TY : 2 * TY + TX));
+      gex = xs ? (lo ? exL : exR) : ex0 + k;
+      gey = xs ? ey0 + k : (lo ? eyB : eyT);
+      ok = h < 2 * (TX + TY) && k < (xs ? nvy : nvx);
     }
-    int h = sl - T;
-    if (h < TY) { if (h >= nvy) return false; gex = ex0 > 0 ? ex0 - 1 : Nex - 1; gey = ey0 + h; return true; }
-    h -= TY;
-    if (h < TY) { if (h >= nvy) return false; gex = ex0 + nvx < Nex ? ex0 + nvx : ex0 + nvx - Nex; gey = ey0 + h; return true; }
-    h -= TY;
-    if (h < TX) { if (h >= nvx) return false; gex = ex0 + h; gey = ey0 > 0 ? ey0 - 1 : Ney - 1; return true; }
-    h -= TX;
-    if (h >= nvx) return false;
-    gex = ex0 + h; gey = ey0 + nvy < Ney ? ey0 + nvy : ey0 + nvy - Ney; return true;
+    gi = ok ? (size_t)n + (size_t)NP2 * ((size_t)gex + (size_t)Nex * (size_t)gey) : 0;
+    return ok;
   }
   __device__ bool slot_global(int sl, int nn, size_t& gi) const {
     int gex, gey;

@@ -84,11 +84,9 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   bool oks[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
-    const int idx = tid + l * NT;
     size_t gi = 0;
-    const bool ok = idx < NS * NP2 && t.slot_global(idx / NP2, idx % NP2, gi);
-    gis[l] = ok ? gi : 0;
-    oks[l] = ok;
+    oks[l] = t.round_global(l, s, n, gi);
+    gis[l] = gi;
   }
   // PCG_A: r, 1/M, p_old, 1/rho; CHEB: r, 1/M, d, x, 1/rho; INIT: x, rho, mx, my;
   // APPLY: x, rho; JACOBI: x, 1/rho.  p_old and d are not used in a first
