@@ -231,15 +231,13 @@ void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
 }
 
 // Default shapes by mesh size (elements E = Nex * Ney), from timing every
-// shape on MI355X at 16^2 .. 256^2: up to ~64^2 elements the launches are
-// latency-bound and 4x4 tiles are fastest; from ~128^2 the 8x4 tiles win
-// (fewer partial sums for every workgroup to read, less halo per node); from
-// ~256^2 the SIP family prefers 8x8.  Its reverse stays at 8x4 there: the
-// reverse PCG iteration holds 116 VGPRs, so a CU runs 16 of its waves with
-// either shape, and two 8-wave workgroups per CU hide each other's barriers
-// (the 256^2 gradient: 9.4 s with 8x8, 8.8 s with 8x4).  The shape of a given
-// mesh is fixed, so repeated runs give bit-identical results; the autotuner
-// may choose other forward shapes.
+// shape on MI355X at 16^2 .. 1024^2: up to ~64^2 elements the launches are
+// latency-bound and 4x4 tiles are fastest; from ~96^2 8x4 tiles win for
+// every family (fewer partial sums for every workgroup to read than 4x4,
+// and two or more 8-wave workgroups per CU hide each other's barriers where
+// one 16-wave 8x8 workgroup cannot).  The shape of a given mesh is fixed, so
+// repeated runs give bit-identical results; the autotuner may choose other
+// forward shapes.
 void default_tiles(Ctx& c) {
   if (c.np == 5) {
     // p = 4: the SIP family's tile by mesh size (the other families keep
@@ -256,20 +254,24 @@ void default_tiles(Ctx& c) {
     return;
   }
   if (c.np != 4) return;
+  // p = 3: 4x4 tiles below 96^2 elements, 8x4 from there for every family
+  // (8x8 SIP tiles, two 16-wave workgroups per CU once PCG_A fits 64
+  // registers, ran PCG_A 24.4 us at 256^2 against 17.7 for 8x4, and the
+  // step 6.74 against 5.70 ms; 1024^2: 137 against 110 ms).
   const long long E = (long long)c.Nex * c.Ney;
   if (E < 96 * 96) { c.tile_x = 4; c.tile_y = 4; c.htile_x = 4; c.htile_y = 4; }
-  else if (E < 192 * 192) { c.tile_x = 8; c.tile_y = 4; c.htile_x = 8; c.htile_y = 4; }
-  else { c.tile_x = 8; c.tile_y = 8; c.htile_x = 8; c.htile_y = 4; }
-  c.rtile_x = c.tile_x == 8 ? 8 : c.tile_x;
-  c.rtile_y = c.tile_x == 8 ? 4 : c.tile_y;
+  else { c.tile_x = 8; c.tile_y = 4; c.htile_x = 8; c.htile_y = 4; }
+  c.rtile_x = c.tile_x;
+  c.rtile_y = c.tile_y;
   // The PCG x/r update streams 7 fields per node and no halo: beyond 512
-  // tiles it runs on max(256, tiles / 16) workgroups that each loop over
+  // tiles it runs on max(512, tiles / 16) workgroups that each loop over
   // their tiles with the next tile's loads in flight, and the next PCG_A
   // reads that many partial sums instead of one per tile (measured on
-  // MI355X at 128^2 .. 1024^2: -8% per step at 256^2, -16% at 512^2, -32%
-  // at 1024^2; no change up to 128^2).
+  // MI355X with 8x4 tiles: 256^2 step 5.57 ms on 512 workgroups against
+  // 6.82 on 256 and 5.95 on 1024; 512^2 22.2 ms on 512 against 23.3 on
+  // 1024; 1024^2 110 ms on 2048 against 112 on 4096; no change up to 128^2).
   const int nt = ((c.Nex + c.tile_x - 1) / c.tile_x) * ((c.Ney + c.tile_y - 1) / c.tile_y);
-  c.bgroups = nt > 512 ? std::max(256, (nt / 16 + 7) / 8 * 8) : 0;
+  c.bgroups = nt > 512 ? std::max(512, (nt / 16 + 7) / 8 * 8) : 0;
 }
 
 // The autotuner: on the current state, time a 30-iteration PCG solve for

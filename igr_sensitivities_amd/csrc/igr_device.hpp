@@ -207,6 +207,23 @@ __device__ __forceinline__ void grid_total(const double* part, int nb, double (&
   grid_finish<NV>(part, nb, pre, out);
 }
 
+// Writes D (NP x NP) and, when sW is given, w (NP) of the basis into LDS.
+// Thread 0 stores the entries from scalar loads of the argument block at
+// compile-time offsets: a per-lane (vector) read of the argument block is
+// not served by the scalar cache and costs a memory round trip.  The
+// caller's next barrier publishes them.
+template <int NP>
+__device__ __forceinline__ void stage_basis(const BasisK& B, double* sD, double* sW) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NP * NP; ++k) sD[k] = B.D[k];
+    if (sW) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) sW[k] = B.w[k];
+    }
+  }
+}
+
 // Julia max(x, y): returns x on ties.
 __device__ __forceinline__ double fmax_first(double x, double y) { return (y > x) ? y : x; }
 

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
   __shared__ double hq[4][NH * NP];
   __shared__ double hdb[4][NH * NP];
   __shared__ double red[NT / WAVE + 1];
+  __shared__ double sD[NP * NP];     // basis (stage_basis)
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
@@ -76,6 +77,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
       for (int v = 0; v < 4; ++v) { hq[v][idx] = (v >= 2) ? 1.0 : 0.0; hdb[v][idx] = 0.0; }
     }
   }
+  stage_basis<NP>(B, sD, nullptr);
   __syncthreads();
 
   double gb = 0.0;
@@ -86,8 +88,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
       double sx = 0.0, sy = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        sx += B.D[k * NP + i] * db[v][s * NP2 + k + NP * j];
-        sy += B.D[k * NP + j] * db[v][s * NP2 + i + NP * k];
+        sx += sD[k * NP + i] * db[v][s * NP2 + k + NP * j];
+        sy += sD[k * NP + j] * db[v][s * NP2 + i + NP * k];
       }
       Fxb[v] = -g.invJx * sx;
       Fyb[v] = -g.invJy * sy;
@@ -201,11 +203,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const size_t gme = valid ? t.gidx(s, n) : 0;
   const FS fs(a.n_iter);
   const int k = a.k;
-  // basis in LDS: read per lane below, each access to the argument block
-  // would be a memory round trip in the middle of the kernel
+  // basis in LDS (stage_basis), read per lane below
   __shared__ double sD[NP * NP], sW[NP];
-  const double bD = B.D[tid < NP * NP ? tid : 0];
-  const double bW = B.w[tid < NP ? tid : 0];
 
   // reverse scalars of this iteration, from the dot products the previous
   // kernel left as partial sums (every workgroup forms the same totals)
@@ -350,8 +349,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     ws[idx] = w; vs[idx] = v; irs[idx] = ir;
     if constexpr (FIN) mbs[idx] = mb;
   }
-  if (tid < NP * NP) sD[tid] = bD;
-  if (tid < NP) sW[tid] = bW;
+  stage_basis<NP>(B, sD, sW);
   __syncthreads();
 
   const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);

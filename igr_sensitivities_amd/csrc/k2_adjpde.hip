@@ -107,6 +107,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_stage(AdjStageArgs a
   __shared__ double wsx[NT], wsy[NT];
   __shared__ double hq[5][NH * NP], ha[4][NH * NP];
   __shared__ double fxr[4][T * NP], fxl[4][T * NP], fyt[4][T * NP], fyb[4][T * NP];
+  __shared__ double sD[NP * NP];     // basis (stage_basis)
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_stage(AdjStageArgs a
     wsx[tid] = fabs(qo[0] / qo[2]) + c;
     wsy[tid] = fabs(qo[1] / qo[2]) + c;
   }
+  stage_basis<NP>(B, sD, nullptr);
   __syncthreads();
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, Sl[4] = {0.0, 0.0, 0.0, 0.0};
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_stage(AdjStageArgs a
     for (int k = 0; k < NP; ++k) {
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        acc[v] += g.invJx * B.D[i * NP + k] * F[v][e0 + k + NP * j] + g.invJy * B.D[j * NP + k] * F[4 + v][e0 + i + NP * k];
+        acc[v] += g.invJx * sD[i * NP + k] * F[v][e0 + k + NP * j] + g.invJy * sD[j * NP + k] * F[4 + v][e0 + i + NP * k];
     }
     // adjoint LLF face fluxes, one evaluation per face inside the tile
     auto nbq = [&](int sl, int node, int hk, double qn[5], double an_[4]) {
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_stage(AdjStageArgs a
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
       const int kx = e0 + k + NP * j, ky = e0 + i + NP * k;
-      const double Dx = B.D[i * NP + k], Dy = B.D[j * NP + k];
+      const double Dx = sD[i * NP + k], Dy = sD[j * NP + k];
       d[0] += Dx * qs[0][kx]; d[1] += Dy * qs[0][ky];
       d[2] += Dx * qs[1][kx]; d[3] += Dy * qs[1][ky];
       d[4] += Dx * qs[2][kx]; d[5] += Dy * qs[2][ky];
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_stage(AdjStageArgs a
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
       const int kx = e0 + k + NP * j, ky = e0 + i + NP * k;
-      const double Dx = B.D[i * NP + k], Dy = B.D[j * NP + k];
+      const double Dx = sD[i * NP + k], Dy = sD[j * NP + k];
       dxPd += Dx * F[0][kx];
       dyPd += Dy * F[0][ky];
       dxPxx += Dx * F[1][kx];
@@ -333,6 +335,7 @@ template <int NP, int TX, int TY>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_b(AdjBArgs a) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
   __shared__ double ls[3][NT];   // a_mx, a_my, a_E
+  __shared__ double sD[NP * NP], sW[NP];     // basis (stage_basis)
   const Geo2 g = a.g;
   const BasisK& B = a.B;
   TileInfo<NP, TX, TY> t(g);
@@ -353,21 +356,22 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_adj_b(AdjBArgs a) {
   } else {
     ls[0][tid] = 0.0; ls[1][tid] = 0.0; ls[2][tid] = 0.0;
   }
+  stage_basis<NP>(B, sD, sW);
   __syncthreads();
   if (!valid) return;
   const int e0 = s * NP2;
   double dax = 0.0, day = 0.0, dEx = 0.0, dEy = 0.0;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    dax += B.D[i * NP + k] * ls[0][e0 + k + NP * j];
-    day += B.D[j * NP + k] * ls[1][e0 + i + NP * k];
-    dEx += B.D[i * NP + k] * ls[2][e0 + k + NP * j];
-    dEy += B.D[j * NP + k] * ls[2][e0 + i + NP * k];
+    dax += sD[i * NP + k] * ls[0][e0 + k + NP * j];
+    day += sD[j * NP + k] * ls[1][e0 + i + NP * k];
+    dEx += sD[i * NP + k] * ls[2][e0 + k + NP * j];
+    dEy += sD[j * NP + k] * ls[2][e0 + i + NP * k];
   }
   dax *= g.invJx; dEx *= g.invJx;
   day *= g.invJy; dEy *= g.invJy;
   const double bb = dax + day + dEx * ux + dEy * uy;
-  a.b[gme] = bb * (a.alpha * B.w[i] * B.w[j] * g.Jx * g.Jy);
+  a.b[gme] = bb * (a.alpha * sW[i] * sW[j] * g.Jx * g.Jy);
 }
 
 // terminal condition (adjoint_pde.jl:535-576; WINDOWED_DENSITY and DISC_KE

@@ -29,6 +29,7 @@
 //                  LLF flux computed once per tile (euler.jl:222-258).
 // Dot products finish with the deterministic last-workgroup reduction.
 #include <algorithm>
+#include <type_traits>
 #include <hip/hip_ext.h>
 #include "igr_ctx.hpp"
 #include "k2_common.hpp"
@@ -39,6 +40,11 @@ namespace igr {
 // ------------------------------------------------------------------------
 // SIP family
 // ------------------------------------------------------------------------
+// No implicit contraction: every fused multiply-add is written as fma(), so
+// the single and batched instantiations of a body round alike (with
+// contraction left to the compiler, a*b + c*d may fuse either product, and
+// the choice can differ between instantiations of the same source).
+#pragma clang fp contract(off)
 // GU: chunks of the previous kernel's partial sums loaded together; a batched
 // launch sums one instance's few partials and keeps the registers instead
 template <int NP, int TX, int TY, int MODE, int GU = 8>
@@ -62,12 +68,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   const bool valid = lx < t.nvx && ly < t.nvy;
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
-  // basis in LDS: read per lane below, each access to the argument block
-  // would be a memory round trip in the middle of the kernel.  Loaded now
-  // (clamped index, no branch) and written to LDS with the staged vector.
+  // basis in LDS (stage_basis), read per lane below
   __shared__ double sD[NP * NP], sW[NP];
-  const double bD = B.D[tid < NP * NP ? tid : 0];
-  const double bW = B.w[tid < NP ? tid : 0];
 
   double own_d = 0.0;
   double beta = 0.0, sc_tot = 0.0;
@@ -130,7 +132,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     if (oks[l]) {
       if constexpr (MODE == M_PCG_A) {
         const double lz = L0[l] * L1[l];
-        v = a.first ? lz : lz + beta * L2[l];
+        v = a.first ? lz : fma(beta, L2[l], lz);
         ir = L3[l];
       } else if constexpr (MODE == M_APPLY) {
         v = L0[l]; ir = 1.0 / L1[l];
@@ -143,7 +145,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         v = L0[l]; ir = L1[l];
       } else {   // M_CHEB
         const double z = L0[l] * L1[l];
-        const double dn = a.first ? z / a.theta : a.c1 * L2[l] + a.c2 * z;
+        const double dn = a.first ? z / a.theta : fma(a.c1, L2[l], a.c2 * z);
         v = L3[l] + dn;
         ir = L4[l];
         if (idx == tid) own_d = dn;
@@ -152,20 +154,18 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     vs[idx] = v;
     irs[idx] = ir;
   }
-  if (tid < NP * NP) sD[tid] = bD;
-  if (tid < NP) sW[tid] = bW;
+  stage_basis<NP>(B, sD, sW);
   __syncthreads();
 
   const double al = a.alpha;
-  const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
   const double vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
   double bval = 0.0;
   if (valid) {
     double dxi = 0.0, deta = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      dxi += sD[i * NP + k] * vs[s * NP2 + k + NP * j];
-      deta += sD[j * NP + k] * vs[s * NP2 + i + NP * k];
+      dxi = fma(sD[i * NP + k], vs[s * NP2 + k + NP * j], dxi);
+      deta = fma(sD[j * NP + k], vs[s * NP2 + i + NP * k], deta);
     }
     gx[s * NP2 + n] = al * sW[i] * sW[j] * g.Jy * g.invJx * ir * dxi;
     gy[s * NP2 + n] = al * sW[i] * sW[j] * g.Jx * g.invJy * ir * deta;
@@ -173,64 +173,76 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        uxx += sD[i * NP + k] * us[s * NP2 + k + NP * j];
-        uyx += sD[i * NP + k] * ws_[s * NP2 + k + NP * j];
-        uxy += sD[j * NP + k] * us[s * NP2 + i + NP * k];
-        uyy += sD[j * NP + k] * ws_[s * NP2 + i + NP * k];
+        uxx = fma(sD[i * NP + k], us[s * NP2 + k + NP * j], uxx);
+        uyx = fma(sD[i * NP + k], ws_[s * NP2 + k + NP * j], uyx);
+        uxy = fma(sD[j * NP + k], us[s * NP2 + i + NP * k], uxy);
+        uyy = fma(sD[j * NP + k], ws_[s * NP2 + i + NP * k], uyy);
       }
       uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
       const double dv = uxx + uyy;
-      const double R = al * (dv * dv + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
+      const double R = al * fma(uyy, uyy, fma(2.0 * uxy, uyx, fma(uxx, uxx, dv * dv)));
       bval = sW[i] * sW[j] * g.Jx * g.Jy * R;
       if (a.b_in) bval = a.b_in[gme];
     }
   }
   // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
-  // faces, NP nodes each) and NP^2 threads: thread n computes the terms of
-  // face nodes n, n + NP^2, ..., so a wave runs every face in one pass with
-  // all its lanes (a per-face branch would run four times with a quarter of
-  // them).  Each face node f = F NP + q gets its consistency term P, penalty
-  // term Q and symmetry factor S in LDS; the owner node adds its own after
-  // the barrier.  The owner's normal derivative is recomputed from the staged
-  // tile.  Faces F = 0, 2 (left, bottom) have the owner on their right side,
-  // F = 1, 3 on their left.
-  constexpr int NF = 4 * NP, NFR = (NF + NP2 - 1) / NP2;
-#pragma unroll
-  for (int r = 0; r < NFR; ++r) {
-    const int f = n + r * NP2;
-    if (f >= NF) continue;
+  // faces, NP nodes each); the tile's T 4 NP face nodes are spread over the
+  // workgroup face-major (item w: face F = w / (T NP), then element, then
+  // node q along the face), so a wave works on one face direction and its
+  // D rows, strides and left/right roles are compile-time constants (a
+  // per-lane mix of the four faces spent most of this phase on addressing).
+  // Each face node f = F NP + q gets its consistency term P, penalty term Q
+  // and symmetry factor S in LDS; the owner node adds its own after the
+  // barrier.  Faces F = 0, 2 (left, bottom) have the owner on their right
+  // side, F = 1, 3 on their left.
+  constexpr int NF = 4 * NP, NFI = T * NF, NFR = (NFI + NT - 1) / NT;
+  auto face_node = [&](auto Fc, const int se, const int q) {
+    constexpr int F = decltype(Fc)::value;
+    constexpr bool xf = F < 2, ownL = (F & 1) != 0;
+    constexpr int dro = ownL ? NP - 1 : 0, drn = ownL ? 0 : NP - 1;   // normal index, own / neighbour
+    constexpr int stp = xf ? 1 : NP;
     double P = 0.0, Q = 0.0, S = 0.0, G = 0.0;
-    if (valid) {
-      const int F = f / NP, q = f - F * NP;
-      const bool xf = F < 2, ownL = (F & 1) != 0;
-      const int io = xf ? (ownL ? NP - 1 : 0) : q, jo = xf ? q : (ownL ? NP - 1 : 0);
-      const int ion = xf ? (ownL ? 0 : NP - 1) : io, jon = xf ? jo : (ownL ? 0 : NP - 1);
-      const int sN = F == 0 ? sL : F == 1 ? sR : F == 2 ? sB : sT;
-      const int dro = xf ? io : jo, drn = xf ? ion : jon, stp = xf ? 1 : NP;
-      const int bo = s * NP2 + (xf ? NP * jo : io), bn = sN * NP2 + (xf ? NP * jon : ion);
+    const int lxe = se % TX, lye = se / TX;
+    if (lxe < t.nvx && lye < t.nvy) {
+      const int sN = F == 0 ? t.nb_left(se) : F == 1 ? t.nb_right(se) : F == 2 ? t.nb_bottom(se) : t.nb_top(se);
+      const int bo = se * NP2 + (xf ? NP * q : q), bn = sN * NP2 + (xf ? NP * q : q);
       double dno = 0.0, dnn = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        dno += sD[dro * NP + k] * vs[bo + stp * k];
-        dnn += sD[drn * NP + k] * vs[bn + stp * k];
+        dno = fma(B.D[dro * NP + k], vs[bo + stp * k], dno);
+        dnn = fma(B.D[drn * NP + k], vs[bn + stp * k], dnn);
       }
-      const double vo = vs[s * NP2 + io + NP * jo], iro = irs[s * NP2 + io + NP * jo];
-      const double xn = vs[sN * NP2 + ion + NP * jon], irn = irs[sN * NP2 + ion + NP * jon];
-      const double wf = xf ? sW[jo] * g.Jy : sW[io] * g.Jx;
+      const double vo = vs[bo + stp * dro], iro = irs[bo + stp * dro];
+      const double xn = vs[bn + stp * drn], irn = irs[bn + stp * drn];
+      const double wf = xf ? sW[q] * g.Jy : sW[q] * g.Jx;
       const double invJ = xf ? g.invJx : g.invJy, h = xf ? g.dx : g.dy;
       const double jump = ownL ? vo - xn : xn - vo;
       const double irLs = ownL ? iro : irn, irRs = ownL ? irn : iro;
       const double dLs = ownL ? dno : dnn, dRs = ownL ? dnn : dno;
-      const double avg = invJ * (irLs * dLs + irRs * dRs) / 2.0;
+      const double avg = invJ * fma(irLs, dLs, irRs * dRs) / 2.0;
       G = g.eta * fmax(irLs, irRs) / h;
       P = al * avg * wf;
       Q = al * G * jump * wf;
       S = -al * (iro / 2.0) * invJ * jump * wf;
     }
-    fP[s * NF + f] = P;
-    fQ[s * NF + f] = Q;
-    fS[s * NF + f] = S;
-    if constexpr (INIT) fG[s * NF + f] = G;
+    const int f = se * NF + F * NP + q;
+    fP[f] = P;
+    fQ[f] = Q;
+    fS[f] = S;
+    if constexpr (INIT) fG[f] = G;
+  };
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int w = tid + r * NT;
+    if (NFI % NT != 0 && w >= NFI) continue;
+    const int F = w / (T * NP), rem = w - F * (T * NP);
+    const int se = rem / NP, q = rem - se * NP;
+    switch (F) {
+      case 0: face_node(std::integral_constant<int, 0>{}, se, q); break;
+      case 1: face_node(std::integral_constant<int, 1>{}, se, q); break;
+      case 2: face_node(std::integral_constant<int, 2>{}, se, q); break;
+      default: face_node(std::integral_constant<int, 3>{}, se, q); break;
+    }
   }
   __syncthreads();
 
@@ -246,12 +258,12 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     double st = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      st += sD[k * NP + i] * gx[s * NP2 + k + NP * j];
-      st += sD[k * NP + j] * gy[s * NP2 + i + NP * k];
+      st = fma(sD[k * NP + i], gx[s * NP2 + k + NP * j], st);
+      st = fma(sD[k * NP + j], gy[s * NP2 + i + NP * k], st);
     }
     y += st;
-    y += fS[fb + j] * sD[i] + fS[fb + NP + j] * sD[(NP - 1) * NP + i];
-    y += fS[fb + 2 * NP + i] * sD[j] + fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j];
+    y += fma(fS[fb + j], sD[i], fS[fb + NP + j] * sD[(NP - 1) * NP + i]);
+    y += fma(fS[fb + 2 * NP + i], sD[j], fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j]);
     y += fc;
   }
 
@@ -266,17 +278,17 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const double D2 = sD[k * NP + i] * sD[k * NP + i];
-        Md += al * sW[k] * sW[j] * g.Jy * g.invJx * D2 * irs[s * NP2 + k + NP * j];
+        Md = fma(al * sW[k] * sW[j] * g.Jy * g.invJx * D2, irs[s * NP2 + k + NP * j], Md);
       }
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const double D2 = sD[k * NP + j] * sD[k * NP + j];
-        Md += al * sW[i] * sW[k] * g.Jx * g.invJy * D2 * irs[s * NP2 + i + NP * k];
+        Md = fma(al * sW[i] * sW[k] * g.Jx * g.invJy * D2, irs[s * NP2 + i + NP * k], Md);
       }
-      if (i == NP - 1) Md += (-al * g.invJx * sD[(NP - 1) * NP + NP - 1] * ir + al * fG[fb + NP + j]) * (sW[j] * g.Jy);
-      if (i == 0) Md += (al * g.invJx * sD[0] * ir + al * fG[fb + j]) * (sW[j] * g.Jy);
-      if (j == NP - 1) Md += (-al * g.invJy * sD[(NP - 1) * NP + NP - 1] * ir + al * fG[fb + 3 * NP + i]) * (sW[i] * g.Jx);
-      if (j == 0) Md += (al * g.invJy * sD[0] * ir + al * fG[fb + 2 * NP + i]) * (sW[i] * g.Jx);
+      if (i == NP - 1) Md = fma(fma(al, fG[fb + NP + j], -al * g.invJx * B.D[(NP - 1) * NP + NP - 1] * ir), sW[j] * g.Jy, Md);
+      if (i == 0) Md = fma(fma(al, fG[fb + j], al * g.invJx * B.D[0] * ir), sW[j] * g.Jy, Md);
+      if (j == NP - 1) Md = fma(fma(al, fG[fb + 3 * NP + i], -al * g.invJy * B.D[(NP - 1) * NP + NP - 1] * ir), sW[i] * g.Jx, Md);
+      if (j == 0) Md = fma(fma(al, fG[fb + 2 * NP + i], al * g.invJy * B.D[0] * ir), sW[i] * g.Jx, Md);
       const double iM = 1.0 / Md;
       o_iM = iM;
       if (!(ir > 0.0) || !isfinite(ir)) flag_domain(a.flag);
@@ -317,7 +329,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   } else if constexpr (MODE == M_JACOBI) {
     if (valid) {
       if (a.y) a.y[gme] = y;     // A x_k, kept by the adjoint's recompute
-      a.xout[gme] = vown + a.omega * (a.b[gme] - y) * a.invM[gme];
+      a.xout[gme] = fma(a.omega * (a.b[gme] - y), a.invM[gme], vown);
     }
   } else if constexpr (MODE == M_CHEB) {
     if (valid) {
@@ -374,8 +386,8 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
   const double ak = rz_it / (tot[0] + 1e-30);
   double acc = 0.0;
   for (int tt = blk; tt < ntiles; tt += nblk) {
-    const double xn = x + ak * p;
-    const double rn = r - ak * ap;
+    const double xn = fma(ak, p, x);
+    const double rn = fma(-ak, ap, r);
     const bool v = valid;
     const size_t g0 = gi;
     acc += v ? rn * (rn * im) : 0.0;
@@ -438,10 +450,8 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
-  // D in LDS: read per lane after the first barrier, each access to the
-  // argument block would be a memory round trip there
+  // D in LDS (stage_basis), read per lane after the first barrier
   __shared__ double sD[NP * NP];
-  const double bD = B.D[threadIdx.x < NP * NP ? threadIdx.x : 0];
   const double gam = a.gamma;
   TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
@@ -494,7 +504,7 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
     wsx[tid] = fabs(qo[0] / qo[2]) + c;
     wsy[tid] = fabs(qo[1] / qo[2]) + c;
   }
-  if (threadIdx.x < NP * NP) sD[threadIdx.x] = bD;
+  stage_basis<NP>(B, sD, nullptr);
   __syncthreads();
 
   // volume term: dq = -(1/Jx) Dx Fx - (1/Jy) Dy Fy, x then y per k (hyperbolic.jl:55-66)

@@ -17,10 +17,14 @@
 namespace igr {
 
 
+// No implicit contraction (explicit fma()): the single and batched
+// instantiations round alike.
+#pragma clang fp contract(off)
 template <int NP, int TX, int TY>
 __device__ __forceinline__ void k2_sens_rhs_body(const SensArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY;
   __shared__ double uu[T * NP2], vv[T * NP2], su[T * NP2], sv[T * NP2], Ss[T * NP2], sr[T * NP2];
+  __shared__ double sD[NP * NP], sW[NP];     // basis (stage_basis)
   const Geo2 g = a.g;
   const BasisK& B = a.B;
   TileInfo<NP, TX, TY> t(g, blk, nblk);
@@ -41,6 +45,7 @@ __device__ __forceinline__ void k2_sens_rhs_body(const SensArgs& a, const int bl
     Ss[tid] = a.S[gi];
     sr[tid] = sg / r;
   }
+  stage_basis<NP>(B, sD, sW);
   __syncthreads();
   if (!valid) return;
   double uxx = 0.0, uxy = 0.0, uyx = 0.0, uyy = 0.0, sxx = 0.0, sxy = 0.0, syx = 0.0, syy = 0.0;
@@ -48,13 +53,13 @@ __device__ __forceinline__ void k2_sens_rhs_body(const SensArgs& a, const int bl
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     const int kj = s * NP2 + k + NP * j, ik = s * NP2 + i + NP * k;
-    const double Di = B.D[i * NP + k], Dj = B.D[j * NP + k];
-    uxx += Di * uu[kj]; uyx += Di * vv[kj];
-    uxy += Dj * uu[ik]; uyy += Dj * vv[ik];
-    sxx += Di * su[kj]; syx += Di * sv[kj];
-    sxy += Dj * su[ik]; syy += Dj * sv[ik];
-    Sx += Di * Ss[kj]; Sy += Dj * Ss[ik];
-    dxs += Di * sr[kj]; dys += Dj * sr[ik];
+    const double Di = sD[i * NP + k], Dj = sD[j * NP + k];
+    uxx = fma(Di, uu[kj], uxx); uyx = fma(Di, vv[kj], uyx);
+    uxy = fma(Dj, uu[ik], uxy); uyy = fma(Dj, vv[ik], uyy);
+    sxx = fma(Di, su[kj], sxx); syx = fma(Di, sv[kj], syx);
+    sxy = fma(Dj, su[ik], sxy); syy = fma(Dj, sv[ik], syy);
+    Sx = fma(Di, Ss[kj], Sx); Sy = fma(Dj, Ss[ik], Sy);
+    dxs = fma(Di, sr[kj], dxs); dys = fma(Dj, sr[ik], dys);
   }
   uxx *= g.invJx; uyx *= g.invJx; uxy *= g.invJy; uyy *= g.invJy;
   sxx *= g.invJx; syx *= g.invJx; sxy *= g.invJy; syy *= g.invJy;
@@ -62,11 +67,11 @@ __device__ __forceinline__ void k2_sens_rhs_body(const SensArgs& a, const int bl
   dxs *= g.invJx; dys *= g.invJy;
   const double al = a.alpha;
   const double divu = uxx + uyy, divs = sxx + syy;
-  const double Rp = al * (divu * divu + uxx * uxx + 2.0 * uxy * uyx + uyy * uyy);
-  const double Rs = al * (2.0 * divu * divs + 2.0 * (uxx * sxx + uxy * syx + uyx * sxy + uyy * syy));
+  const double Rp = al * fma(uyy, uyy, fma(2.0 * uxy, uyx, fma(uxx, uxx, divu * divu)));
+  const double Rs = al * fma(2.0 * divu, divs, 2.0 * fma(uyy, syy, fma(uyx, sxy, fma(uxy, syx, uxx * sxx))));
   const double r = a.q[2 * N + gi];
-  const double bpt = Rs + sr[tid] * Rp - al * (dxs * Sx + dys * Sy) / r;
-  a.out[gi] = B.w[i] * B.w[j] * g.Jx * g.Jy * bpt;
+  const double bpt = fma(sr[tid], Rp, Rs) - al * fma(dxs, Sx, dys * Sy) / r;
+  a.out[gi] = sW[i] * sW[j] * g.Jx * g.Jy * bpt;
 }
 template <int NP, int TX, int TY>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sens_rhs(SensArgs a) {
@@ -119,6 +124,7 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
   __shared__ double Fx[4][T * NP2], Fy[4][T * NP2];
   __shared__ double hq[5][NH * NP], hs[5][NH * NP];
   __shared__ double fxr[4][T * NP], fxl[4][T * NP], fyt[4][T * NP], fyb[4][T * NP];
+  __shared__ double sD[NP * NP];     // basis (stage_basis)
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
@@ -161,6 +167,7 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
   sflux2(1, gam, qo, so, a.sgam, fyo);
 #pragma unroll
   for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
+  stage_basis<NP>(B, sD, nullptr);
   __syncthreads();
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -189,8 +196,8 @@ __device__ __forceinline__ void k2_sens_hyp_body(const SensArgs& a, const int bl
     for (int k = 0; k < NP; ++k)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        acc[v] -= g.invJx * B.D[i * NP + k] * Fx[v][s * NP2 + k + NP * j] +
-                  g.invJy * B.D[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
+        acc[v] -= g.invJx * sD[i * NP + k] * Fx[v][s * NP2 + k + NP * j] +
+                  g.invJy * sD[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
     if (i == NP - 1) {
       double qn[5], sn[5], f[4];
       nb(t.nb_right(s), NP * j, j, qn, sn);

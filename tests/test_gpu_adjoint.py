@@ -159,9 +159,8 @@ def test_gradient_dot_product_tiles(orc, tile, htile, rtile):
 
 
 def test_reverse_tile_defaults_and_rejects():
-    # the reverse kernels keep 8x4 workgroups where the forward SIP family
-    # takes 8x8 (from 192^2 elements); smaller meshes share the forward shape
-    for n, want in ((32, (404, 404)), (128, (804, 804)), (256, (808, 804))):
+    # the reverse kernels take the forward SIP family's shape by default
+    for n, want in ((32, (404, 404)), (128, (804, 804)), (256, (804, 804))):
         c = ctx2(3, n, n)
         assert (c.get_option(igr.core.OPT_TILE), c.get_option(igr.core.OPT_RTILE)) == want
     c = ctx2(3, 8, 8)

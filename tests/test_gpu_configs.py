@@ -42,7 +42,7 @@ def _ctx256():
 
 def test_c5_two_steps_vs_oracle(orc, c5):
     c = _ctx256()
-    assert c.get_option(igr.core.OPT_TILE) == 808      # the 256^2 default shapes
+    assert c.get_option(igr.core.OPT_TILE) == 804      # the 256^2 default shapes
     c.set_state(c5["q"])
     c.run_steps(2, c5["dt"], c5["gamma"], c5["alpha"], 60, "pcg")
     q, S = c.get_state()
