@@ -17,12 +17,12 @@ struct BasisK {
 };
 
 // v taken from another lane by a DPP lane pattern (two 32-bit DPP moves).
-// Lanes outside row_mask, or whose source lane is out of range, get 0.
-template <int CTRL, int ROW_MASK = 0xF>
+// A lane whose source is out of range keeps an undefined value.
+template <int CTRL>
 __device__ __forceinline__ double dpp_take(double v) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROW_MASK, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROW_MASK, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ __forceinline__ double read_lane(double v, int lane) {
@@ -32,27 +32,50 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Wavefront sum (64 lanes); every lane gets the sum, in a fixed order.
-// A workgroup whose size is a multiple of 64 (every lane of every wave
-// active) sums by DPP lane moves: pairs, quads, half rows and rows of 16
-// (quad_perm, row_half_mirror, row_mirror), then the four row sums through
-// row_bcast:15 and row_bcast:31 into lane 63, read back as a uniform value.
-// These are VALU moves, not LDS round trips.  Otherwise (a partial last
-// wave, whose absent lanes DPP cannot skip) butterfly shuffles, where an
-// absent lane reads as 0.
-__device__ __forceinline__ double wave_sum(double v) {
+// Wavefront sums of NV values, in a fixed order, valid in the last lane
+// (WAVE - 1).  A workgroup whose size is a multiple of 64 (every lane of
+// every wave active) sums by DPP lane moves, the NV values step by step
+// together: pairs and quads (quad_perm), half rows and rows of 16
+// (row_half_mirror, row_mirror), then the row sums through row_bcast:15 and
+// row_bcast:31 into lane 63 (the other lanes end with partial or undefined
+// values).  These are VALU moves, not LDS round trips.  Otherwise (a
+// partial last wave, whose absent lanes DPP cannot skip) butterfly
+// shuffles, where an absent lane reads as 0; there lane 0 holds the sum
+// (wave_sum_lane).
+template <int NV>
+__device__ __forceinline__ void wave_sums_last(double (&v)[NV]) {
   if ((blockDim.x & (WAVE - 1)) == 0) {
-    v += dpp_take<0xB1>(v);         // quad_perm [1,0,3,2]
-    v += dpp_take<0x4E>(v);         // quad_perm [2,3,0,1]
-    v += dpp_take<0x141>(v);        // row_half_mirror
-    v += dpp_take<0x140>(v);        // row_mirror
-    v += dpp_take<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
-    v += dpp_take<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
-    return read_lane(v, WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0xB1>(v[k]);    // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0x4E>(v[k]);    // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0x141>(v[k]);   // row_half_mirror
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0x140>(v[k]);   // row_mirror
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0x142>(v[k]);   // row_bcast:15 (rows 1, 3 matter)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += dpp_take<0x143>(v[k]);   // row_bcast:31 (row 3 matters)
+    return;
   }
 #pragma unroll
-  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, WAVE);
+  }
+}
+
+// The lane that holds wave_sums_last's sums in every wave: the last lane
+// for the DPP path; lane 0 for the shuffles, the one lane whose butterfly
+// partners are all present in a partial last wave (lanes 0 .. A-1 active).
+__device__ __forceinline__ int wave_sum_lane() { return (blockDim.x & (WAVE - 1)) == 0 ? WAVE - 1 : 0; }
+
+// Wavefront sum (64 lanes); every lane gets the sum (wave_sums_last's order).
+__device__ __forceinline__ double wave_sum(double v) {
+  double a[1] = {v};
+  wave_sums_last<1>(a);
+  return read_lane(a[0], wave_sum_lane());
 }
 
 // Workgroup sum of NV values; result valid in thread 0.  `red` is LDS of
@@ -61,9 +84,8 @@ template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0) {
+  wave_sums_last<NV>(v);
+  if (lane == wave_sum_lane()) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) red[k * nw + wid] = v[k];
   }
@@ -127,9 +149,8 @@ __device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*
   __shared__ double sp[NV * MAX_WAVES];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0) {
+  wave_sums_last<NV>(v);
+  if (lane == wave_sum_lane()) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) sp[k * nw + wid] = v[k];
   }
@@ -186,9 +207,8 @@ __device__ __forceinline__ void grid_finish(const double* part, int nb, const do
       for (int k = 0; k < NV; ++k) acc[k] = in ? acc[k] + v[u][k] : acc[k];
     }
   }
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
-  if (lane == 0) {
+  wave_sums_last<NV>(acc);
+  if (lane == wave_sum_lane()) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) sg[k * nw + wid] = acc[k];
   }

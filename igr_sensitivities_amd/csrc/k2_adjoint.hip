@@ -17,6 +17,7 @@
 //                   (enzyme_adjoint.jl:72-76, 108-113, 145-151, 184-190, 223-228)
 // Work decomposition, tiles, halo and XCD-contiguous order as in k2_forward.
 #include "igr_ctx.hpp"
+#include <type_traits>
 #include "k2_adjoint_common.hpp"
 
 namespace igr {
@@ -393,40 +394,39 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     }
   }
   // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
-  // faces, NP nodes each) and NP^2 threads: thread n computes the terms of
-  // face nodes n, n + NP^2, ..., so a wave runs every face in one pass with
-  // all its lanes.  Face node f = F NP + q leaves in LDS its part of the
-  // owner's A w (consistency + penalty), its symmetry factor, and its part of
-  // d/d(1/rho_owner) of the face's bilinear form in (w, v); the owner adds its
-  // own after the barrier.  Faces F = 0, 2 (left, bottom) have the owner on
-  // their right side, F = 1, 3 on their left.
-  constexpr int NF = 4 * NP, NFR = (NF + NP2 - 1) / NP2;
-#pragma unroll
-  for (int r = 0; r < NFR; ++r) {
-    const int f = n + r * NP2;
-    if (f >= NF) continue;
+  // faces, NP nodes each); the tile's T 4 NP face nodes are spread over the
+  // workgroup face-major (item w: face F = w / (T NP), then element, then
+  // node q along the face), so a wave works on one face direction with
+  // compile-time D rows, strides and left/right roles.  Face node
+  // f = F NP + q leaves in LDS its part of the owner's A w (consistency +
+  // penalty), its symmetry factor, and its part of d/d(1/rho_owner) of the
+  // face's bilinear form in (w, v); the owner adds its own after the
+  // barrier.  Faces F = 0, 2 (left, bottom) have the owner on their right
+  // side, F = 1, 3 on their left.
+  constexpr int NF = 4 * NP, NFI = T * NF, NFR = (NFI + NT - 1) / NT;
+  auto face_node = [&](auto Fc, const int se, const int q) {
+    constexpr int F = decltype(Fc)::value;
+    constexpr bool xf = F < 2, own_L = (F & 1) != 0;
+    constexpr int dro = own_L ? NP - 1 : 0, drn = own_L ? 0 : NP - 1;   // normal index, own / neighbour
+    constexpr int stp = xf ? 1 : NP;
     double C = 0.0, Sy = 0.0, Rb = 0.0;
-    if (valid) {
-      const int F = f / NP, q = f - F * NP;
-      const bool xf = F < 2, own_L = (F & 1) != 0;
-      const int io = xf ? (own_L ? NP - 1 : 0) : q, jo = xf ? q : (own_L ? NP - 1 : 0);
-      const int ion = xf ? (own_L ? 0 : NP - 1) : io, jon = xf ? jo : (own_L ? 0 : NP - 1);
-      const int sN = F == 0 ? sL : F == 1 ? sR : F == 2 ? sB : sT;
-      const int dro = xf ? io : jo, drn = xf ? ion : jon, stp = xf ? 1 : NP;
-      const int bo = s * NP2 + (xf ? NP * jo : io), bn = sN * NP2 + (xf ? NP * jon : ion);
+    const int lxe = se % TX, lye = se / TX;
+    if (lxe < t.nvx && lye < t.nvy) {
+      const int sN = F == 0 ? t.nb_left(se) : F == 1 ? t.nb_right(se) : F == 2 ? t.nb_bottom(se) : t.nb_top(se);
+      const int bo = se * NP2 + (xf ? NP * q : q), bn = sN * NP2 + (xf ? NP * q : q);
       double dow = 0.0, dov = 0.0, dnw = 0.0, dnv = 0.0;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
-        const double Do = sD[dro * NP + kk], Dn = sD[drn * NP + kk];
+        const double Do = B.D[dro * NP + kk], Dn = B.D[drn * NP + kk];
         dow += Do * ws[bo + stp * kk];
         dov += Do * vs[bo + stp * kk];
         dnw += Dn * ws[bn + stp * kk];
         dnv += Dn * vs[bn + stp * kk];
       }
-      const int oo = s * NP2 + io + NP * jo, on = sN * NP2 + ion + NP * jon;
+      const int oo = bo + stp * dro, on = bn + stp * drn;
       const double wo = ws[oo], vo = vs[oo], iro = irs[oo];
       const double wn = ws[on], vn = vs[on], irn = irs[on];
-      const double wf = xf ? sW[jo] * g.Jy : sW[io] * g.Jx;
+      const double wf = xf ? sW[q] * g.Jy : sW[q] * g.Jx;
       const double invJ = xf ? g.invJx : g.invJy, h = xf ? g.dx : g.dy;
       const double irL = own_L ? iro : irn, irR = own_L ? irn : iro;
       const double jw = own_L ? (wo - wn) : (wn - wo);
@@ -444,9 +444,23 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       Rb = -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
       if (own_L == L_chosen) Rb += al * g.eta / h * jv * wf * jw;
     }
-    fC[s * NF + f] = C;
-    fS[s * NF + f] = Sy;
-    fR[s * NF + f] = Rb;
+    const int f = se * NF + F * NP + q;
+    fC[f] = C;
+    fS[f] = Sy;
+    fR[f] = Rb;
+  };
+#pragma unroll
+  for (int r = 0; r < NFR; ++r) {
+    const int w = tid + r * NT;
+    if (NFI % NT != 0 && w >= NFI) continue;
+    const int F = w / (T * NP), rem = w - F * (T * NP);
+    const int se = rem / NP, q = rem - se * NP;
+    switch (F) {
+      case 0: face_node(std::integral_constant<int, 0>{}, se, q); break;
+      case 1: face_node(std::integral_constant<int, 1>{}, se, q); break;
+      case 2: face_node(std::integral_constant<int, 2>{}, se, q); break;
+      default: face_node(std::integral_constant<int, 3>{}, se, q); break;
+    }
   }
   __syncthreads();
 
