@@ -59,7 +59,7 @@ def lgl(p):
 
 
 def coords2(p, Nex, Ney, Lx=1.0, Ly=1.0):
-    """Node coordinates, shape (Ney, Nex, np, np) (CartesianMesh2D, mesh.jl:116-123)."""
+    """Node coordinates, shape (Ney, Nex, np, np) (CartesianMesh2D, IGR_adjoints_2d/mesh.jl:48-55)."""
     xi, _ = lgl(p)
     dx, dy = Lx / Nex, Ly / Ney
     ex = np.arange(Nex)[None, :, None, None]

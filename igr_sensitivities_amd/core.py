@@ -81,7 +81,7 @@ class DGBasis:
 
 
 class CartesianMesh2D:
-    """Uniform doubly periodic mesh (IGR_adjoints_2d/mesh.jl:89-125)."""
+    """Uniform doubly periodic mesh (IGR_adjoints_2d/mesh.jl:21-57)."""
 
     dim = 2
 
@@ -386,7 +386,7 @@ def solve_elliptic_(Sigma, q, basis, mesh, alpha, n_iter=10, solver="pcg"):
 
 
 def compute_igr_euler_rhs_(dq, q, Sigma, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
-    """compute_igr_euler_rhs! (IGR_adjoints_2d/forward.jl:141; 1D :23)."""
+    """compute_igr_euler_rhs! (IGR_adjoints_2d/forward.jl:22; 1D :23)."""
     c = _ctx(mesh)
     code = _solver_code(solver)
     c.set_state(q, Sigma)
@@ -395,7 +395,7 @@ def compute_igr_euler_rhs_(dq, q, Sigma, basis, mesh, gamma, alpha, n_iter=10, s
 
 
 def run_forward_(q, Sigma, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
-    """run_forward! (IGR_adjoints_2d/forward.jl:160; 1D :50): q and Sigma overwritten."""
+    """run_forward! (IGR_adjoints_2d/forward.jl:41; 1D :50): q and Sigma overwritten."""
     c = _ctx(mesh)
     code = _solver_code(solver)
     c.set_state(q)
@@ -503,7 +503,7 @@ def enzyme_adjoint_ic(q0, n_steps, dt, basis, mesh, gamma, alpha, n_iter=10, sol
 
 
 def run_forward(q0, dt, T, basis, mesh, gamma, alpha, n_iter=10, solver="pcg"):
-    """run_forward (IGR_adjoints_2d/forward.jl:203): returns (q, Sigma) at T."""
+    """run_forward (IGR_adjoints_2d/forward.jl:84): returns (q, Sigma) at T."""
     q = np.array(q0, dtype=np.float64, copy=True)
     S = np.zeros(mesh.shape)
     run_forward_(q, S, dt, T, basis, mesh, gamma, alpha, n_iter, solver)

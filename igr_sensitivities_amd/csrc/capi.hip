@@ -190,7 +190,7 @@ void stage2(Ctx& c, const double* qs, double gamma, double alpha, int n_iter, in
   launch2_hyp(c, h);
 }
 
-// One SSP-RK3 step (forward.jl:170-188).
+// One SSP-RK3 step (IGR_adjoints_2d/forward.jl:51-69).
 void step2(Ctx& c, double dt, double gamma, double alpha, int n_iter, int solver) {
   stage2(c, c.q, gamma, alpha, n_iter, solver, 1, dt, c.q1);
   stage2(c, c.q1, gamma, alpha, n_iter, solver, 2, 1.0 / 4.0 * dt, c.q2);

@@ -77,7 +77,8 @@ int k1_block(const Ctx& c);
 int ensure_scal(Ctx& c, int n_iter);
 
 // n_steps SSP-RK3 steps on the context state.  zero_sigma: Sigma <- 0 first
-// (forward.jl:161); final_solve: Sigma solve at the final state (:190-192).
+// (IGR_adjoints_2d/forward.jl:42); final_solve: Sigma solve at the final
+// state (:71-73).
 inline int run_steps_impl(Ctx& c, int n_steps, double dt, double gamma, double alpha, int n_iter, int solver,
                           int zero_sigma, int final_solve) {
   int rc = ensure_scal(c, n_iter);

@@ -3,7 +3,7 @@
 // (IGR_adjoints_2d/enzyme_adjoint.jl:270-296).
 //
 //   k2_hyp_rev      reverse of the hyperbolic RHS + SSP-RK3 stage update
-//                   (hyperbolic.jl:27-119, euler.jl:145-258, forward.jl:170-188)
+//                   (hyperbolic.jl:27-119, euler.jl:20-133, forward.jl:51-69)
 //   k2_sip_rev      reverse of the Sigma solve, one launch per PCG iteration
 //                   (pcg_fixed!, elliptic.jl:329-347): rebuilds the adjoint
 //                   of A p_k on the fly for own and halo nodes, applies A

@@ -182,7 +182,7 @@ struct TileInfo {
 
 // ---- pointwise Euler physics, reference evaluation order, no contraction
 #pragma clang fp contract(off)
-// polytropic_flux_x / _y, IGR_adjoints_2d/euler.jl:156-186.  q = (mx,my,rho,E,Sigma)
+// polytropic_flux_x / _y, IGR_adjoints_2d/euler.jl:31-61.  q = (mx,my,rho,E,Sigma)
 __device__ __forceinline__ void flux_x2(double gam, const double q[5], double f[4]) {
   const double P = (gam - 1.0) * (q[3] - (q[0] * q[0] + q[1] * q[1]) / (2.0 * q[2]));
   const double pb = P + q[4];
@@ -195,14 +195,14 @@ __device__ __forceinline__ void flux_y2(double gam, const double q[5], double f[
   const double uy = q[1] / q[2];
   f[0] = q[0] * uy; f[1] = q[1] * uy + pb; f[2] = q[1]; f[3] = (q[3] + pb) * uy;
 }
-// sound_speed, euler.jl:193-196; a negative argument is Julia's DomainError
+// sound_speed, euler.jl:68-71; a negative argument is Julia's DomainError
 __device__ __forceinline__ double sound2(double gam, const double q[5], int* flag) {
   const double P = (gam - 1.0) * (q[3] - (q[0] * q[0] + q[1] * q[1]) / (2.0 * q[2]));
   const double arg = gam * P / q[2];
   if (!(arg >= 0.0)) flag_domain(flag);
   return sqrt(arg);
 }
-// f* = (fL + fR)/2 + lam (qL - qR)/2, euler.jl:230-233
+// f* = (fL + fR)/2 + lam (qL - qR)/2, euler.jl:105-108
 __device__ __forceinline__ void llf_combine(const double fL[4], const double fR[4], const double qL[5],
                                             const double qR[5], double lam, double f[4]) {
 #pragma unroll

@@ -25,8 +25,8 @@
 //   k2_pcg_b       x += a p, r -= a Ap, <r, z>   (pcg_fixed!, :339-343)
 //   k2_hyp         hyperbolic RHS (compute_hyperbolic_rhs!,
 //                  IGR_adjoints_2d/hyperbolic.jl:27-119) fused with the
-//                  SSP-RK3 stage combination (forward.jl:170-188), each face
-//                  LLF flux computed once per tile (euler.jl:222-258).
+//                  SSP-RK3 stage combination (forward.jl:51-69), each face
+//                  LLF flux computed once per tile (euler.jl:97-133).
 // Dot products finish with the deterministic last-workgroup reduction.
 #include <algorithm>
 #include <type_traits>
@@ -608,7 +608,7 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
       acc[v] += g.invJy * (f - fyo[v]) / B.w[0];
     }
   }
-  // SSP-RK3 stage combination (forward.jl:172-187)
+  // SSP-RK3 stage combination (IGR_adjoints_2d/forward.jl:53-68)
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const size_t gi = v * N + gme;

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_build_ic(IcArgs a) {
       const int j = (int)((n / a.np) % a.np);
       const size_t e = n / ((size_t)a.np * a.np);
       const int ex = (int)(e % a.Nex), ey = (int)(e / a.Nex);
-      const double x = (double)ex * a.dx + a.Jx * (1.0 + a.xi[i]);   // mesh.jl:48-53
+      const double x = (double)ex * a.dx + a.Jx * (1.0 + a.xi[i]);   // IGR_adjoints_2d/mesh.jl:48-53
       const double y = (double)ey * a.dy + a.Jy * (1.0 + a.xi[j]);
       double rho = 1.0, ux = 0.0, uy = 0.0, p = 1.0;
       double drho = 0.0, dux = 0.0, duy = 0.0, dp = 0.0;

@@ -44,7 +44,7 @@ enum igr_solver { IGR_SOLVER_PCG = 0, IGR_SOLVER_JACOBI = 1, IGR_SOLVER_CHEBYSHE
 typedef struct igr_ctx igr_ctx;
 
 /* Mesh + basis set-up: DGBasis(p) (IGR_adjoints_2d/basis.jl:56-68) and
- * CartesianMesh2D(nex, ney, lx, ly) (IGR_adjoints_2d/mesh.jl:108-125) for
+ * CartesianMesh2D(nex, ney, lx, ly) (IGR_adjoints_2d/mesh.jl:40-57) for
  * dim = 2, or PeriodicMesh1D(nex, lx) (IGR_adjoints_1d/mesh.jl:21-33) for
  * dim = 1 (ney, ly ignored).  1 <= p <= 4. */
 int igr_create(igr_ctx** ctx, int dim, int p, int nex, int ney, double lx, double ly);
@@ -100,13 +100,13 @@ int igr_solve_elliptic(igr_ctx* ctx, double alpha, int n_iter, int solver);
 /* PCG scalars of the last PCG solve: a_k and beta_k, k < n_iter. */
 int igr_solver_trace(igr_ctx* ctx, int n_iter, double* a, double* beta);
 
-/* compute_igr_euler_rhs! on the context state (IGR_adjoints_2d/forward.jl:141):
+/* compute_igr_euler_rhs! on the context state (IGR_adjoints_2d/forward.jl:22):
  * Sigma solve when alpha > 0 (else Sigma = 0), then the hyperbolic RHS,
  * written to dq (host, nv*N; may be NULL). */
 int igr_euler_rhs(igr_ctx* ctx, double gamma, double alpha, int n_iter, int solver, double* dq);
 
 /* run_forward!(q..., Sigma, dt, T, basis, mesh, gamma, alpha, n_iter, solver)
- * (IGR_adjoints_2d/forward.jl:160-194; IGR_adjoints_1d/forward.jl:50-87):
+ * (IGR_adjoints_2d/forward.jl:41-75; IGR_adjoints_1d/forward.jl:50-87):
  * Sigma <- 0, n_steps = ceil(T/dt), dt = T/n_steps, SSP-RK3, final Sigma
  * solve when alpha > 0.  Advances the context state in place. */
 int igr_run_forward(igr_ctx* ctx, double dt, double t_end, double gamma, double alpha, int n_iter,

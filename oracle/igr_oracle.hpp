@@ -182,7 +182,7 @@ struct Mesh1D {
   Mesh1D(int ne, double l) : Ne(ne), L(l), dx(l / ne), J((l / ne) / 2.0) {}
   double x(const Basis& B, int i, int e) const { return e * dx + J * (1.0 + B.xi[i]); }
 };
-// CartesianMesh2D: IGR_adjoints_2d/mesh.jl:108-125.
+// CartesianMesh2D: IGR_adjoints_2d/mesh.jl:40-57.
 struct Mesh2D {
   int Nex, Ney; double Lx, Ly, dx, dy, Jx, Jy;
   Mesh2D(int nx, int ny, double lx, double ly)
@@ -212,18 +212,18 @@ struct Grid2 {
   }
 };
 
-// euler.jl:145-147
+// IGR_adjoints_2d/euler.jl:20-22
 template <class T> inline T pressure2(T g, T mx, T my, T r, T E) {
   return (g - 1.0) * (E - (mx * mx + my * my) / (2.0 * r));
 }
-// euler.jl:156-167
+// euler.jl:31-42
 template <class T> inline void flux_x2(T g, T mx, T my, T r, T E, T S, T f[4]) {
   T P = pressure2(g, mx, my, r, E);
   T pb = P + S;
   T ux = mx / r;
   f[0] = mx * ux + pb; f[1] = my * ux; f[2] = mx; f[3] = (E + pb) * ux;
 }
-// euler.jl:175-186
+// euler.jl:50-61
 template <class T> inline void flux_y2(T g, T mx, T my, T r, T E, T S, T f[4]) {
   T P = pressure2(g, mx, my, r, E);
   T pb = P + S;
@@ -231,14 +231,14 @@ template <class T> inline void flux_y2(T g, T mx, T my, T r, T E, T S, T f[4]) {
   f[0] = mx * uy; f[1] = my * uy + pb; f[2] = my; f[3] = (E + pb) * uy;
   if (g_fwd_mutation == 3) { T t = f[0]; f[0] = f[1]; f[1] = t; }
 }
-// euler.jl:193-214
+// euler.jl:68-89
 template <class T> inline T sound2(T g, T mx, T my, T r, T E) {
   T P = pressure2(g, mx, my, r, E);
   return sqrt_(g * P / r);
 }
 template <class T> inline T wave_x2(T g, T mx, T my, T r, T E) { return abs_(mx / r) + sound2(g, mx, my, r, E); }
 template <class T> inline T wave_y2(T g, T mx, T my, T r, T E) { return abs_(my / r) + sound2(g, mx, my, r, E); }
-// euler.jl:222-258, dir 0 = x, 1 = y
+// euler.jl:97-133, dir 0 = x, 1 = y
 template <class T>
 inline void llf2(int dir, T g, const T qL[5], const T qR[5], T f[4]) {
   T lam = dir == 0 ? max_(wave_x2(g, qL[0], qL[1], qL[2], qL[3]), wave_x2(g, qR[0], qR[1], qR[2], qR[3]))
@@ -556,7 +556,7 @@ void chebyshev_fixed(Apply A, std::vector<T>& x, const std::vector<T>& b, const 
 }
 
 // Solver dispatch shared by solve_elliptic! (2D elliptic.jl:413-427,
-// 1D elliptic.jl:451-467) and solve_sensitivity_elliptic!.
+// IGR_adjoints_1d/elliptic.jl:451-467) and solve_sensitivity_elliptic!.
 template <class T, class Apply>
 int run_solver(int solver, Apply A, std::vector<T>& x, const std::vector<T>& b,
                const std::vector<T>& Md, int n_iter, int p, SolverTrace* tr) {
@@ -582,7 +582,7 @@ int solve_elliptic2(const Grid2& G, T alpha, int n_iter, int solver, const T* q,
   return rc;
 }
 
-// compute_igr_euler_rhs!, IGR_adjoints_2d/forward.jl:141-150.
+// compute_igr_euler_rhs!, IGR_adjoints_2d/forward.jl:22-31.
 template <class T>
 int euler_rhs2(const Grid2& G, T g, T alpha, int n_iter, int solver, const T* q, T* S, T* dq) {
   if (alpha > 0.0) {
@@ -595,7 +595,7 @@ int euler_rhs2(const Grid2& G, T g, T alpha, int n_iter, int solver, const T* q,
   return 0;
 }
 
-// One SSP-RK3 step, IGR_adjoints_2d/forward.jl:170-188 (nv = 4 in 2D, 3 in 1D).
+// One SSP-RK3 step, IGR_adjoints_2d/forward.jl:51-69 (nv = 4 in 2D, 3 in 1D).
 template <class T, class Rhs>
 int ssp_rk3_step(Rhs rhs, size_t nvN, T* q, double dt, std::vector<T>& q1, std::vector<T>& q2,
                  std::vector<T>& dq) {
@@ -630,7 +630,7 @@ int run_steps2(const Grid2& G, T g, T alpha, int n_iter, int solver, int n_steps
   return 0;
 }
 
-// run_forward!, IGR_adjoints_2d/forward.jl:160-194: n_steps = ceil(T/dt),
+// run_forward!, IGR_adjoints_2d/forward.jl:41-75: n_steps = ceil(T/dt),
 // dt = T/n_steps, final Sigma solve when alpha > 0.
 template <class T>
 int run_forward2(const Grid2& G, T g, T alpha, int n_iter, int solver, double dt, double Tend,

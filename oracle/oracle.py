@@ -134,7 +134,7 @@ def basis(p):
 
 
 class Grid2D:
-    """Geometry handle; mirrors CartesianMesh2D (IGR_adjoints_2d/mesh.jl:108)."""
+    """Geometry handle; mirrors CartesianMesh2D (IGR_adjoints_2d/mesh.jl:40)."""
 
     def __init__(self, p, Nex, Ney, Lx=1.0, Ly=1.0):
         self.p, self.Nex, self.Ney, self.Lx, self.Ly = p, Nex, Ney, float(Lx), float(Ly)

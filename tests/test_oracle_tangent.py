@@ -66,7 +66,7 @@ def test_1d_sensitivity_vs_fd(orc, form):
     T, al, A = 0.05, 2e-3, 0.05
     _, _, s, _, _ = g.run_sensitivity(gam, al, 60, orc.PCG, dt, T, ic(A), s0, form=form)
     h = 1e-6
-    # the 1D tangent driver steps with dt = min(dt, T - t) (sensitivity.jl:419-420)
+    # the 1D tangent driver steps with dt = min(dt, T - t) (IGR_adjoints_1d/sensitivity.jl:419-420)
     fdp = _run_1d_like_tangent(g, gam, al, dt, T, ic(A + h), orc)
     fdm = _run_1d_like_tangent(g, gam, al, dt, T, ic(A - h), orc)
     fd = (fdp - fdm) / (2 * h)
