@@ -369,6 +369,13 @@ def main():
     big = None
     if args.mesh != 256:
         big = next((r for r in (sweep or []) if r["mesh"] == "256x256"), None) or kernel_rows(igr, configs, 256)
+        # measured HBM bytes per PCG_A launch at 256^2 (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy)
+        tp256 = os.path.join(ROOT, "profiles", "pcg_a_2d_256x256_p3_traffic.json")
+        if os.path.exists(tp256):
+            try:
+                big = dict(big, pcg_a=dict(big["pcg_a"], traffic=json.load(open(tp256)).get("hbm_bytes_per_launch")))
+            except (ValueError, OSError, KeyError):
+                pass
 
     cpu = None
     if not args.no_cpu:
