@@ -256,8 +256,8 @@ void default_tiles(Ctx& c) {
   if (c.np != 4) return;
   // p = 3: 4x4 tiles below 96^2 elements, 8x4 from there for every family
   // (8x8 SIP tiles, two 16-wave workgroups per CU once PCG_A fits 64
-  // registers, ran PCG_A 24.4 us at 256^2 against 17.7 for 8x4, and the
-  // step 6.74 against 5.70 ms; 1024^2: 137 against 110 ms).
+  // registers, ran PCG_A 24.4 us at 256^2 against 17.7 for 8x4; 1024^2 step
+  // 103.5 against 88.5 ms).
   const long long E = (long long)c.Nex * c.Ney;
   if (E < 96 * 96) { c.tile_x = 4; c.tile_y = 4; c.htile_x = 4; c.htile_y = 4; }
   else { c.tile_x = 8; c.tile_y = 4; c.htile_x = 8; c.htile_y = 4; }
@@ -268,8 +268,8 @@ void default_tiles(Ctx& c) {
   // their tiles with the next tile's loads in flight, and the next PCG_A
   // reads that many partial sums instead of one per tile (measured on
   // MI355X with 8x4 tiles: 256^2 step 5.57 ms on 512 workgroups against
-  // 6.82 on 256 and 5.95 on 1024; 512^2 22.2 ms on 512 against 23.3 on
-  // 1024; 1024^2 110 ms on 2048 against 112 on 4096; no change up to 128^2).
+  // 6.82 on 256 and 5.95 on 1024; 512^2 17.3 ms on 256 or 512; 1024^2
+  // 88.5 ms on 1024 or 2048; no change up to 128^2).
   const int nt = ((c.Nex + c.tile_x - 1) / c.tile_x) * ((c.Ney + c.tile_y - 1) / c.tile_y);
   c.bgroups = nt > 512 ? std::max(512, (nt / 16 + 7) / 8 * 8) : 0;
 }
