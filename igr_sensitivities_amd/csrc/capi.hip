@@ -405,6 +405,7 @@ int igr_sizes(const igr_ctx* c, long long* n_nodes, int* n_vars) {
 int igr_set_option(igr_ctx* c, int option, int value) {
   if (!c) return IGR_EINVAL;
   if (option == IGR_OPT_GRAPHS) { c->use_graphs = value ? 1 : 0; return IGR_OK; }
+  if (option == IGR_OPT_WAVE1D) { c->k1_wave = value ? 1 : 0; return IGR_OK; }
   if (option == IGR_OPT_TILE || option == IGR_OPT_HTILE) {
     const int tx = value / 100, ty = value % 100;
     const bool h = option == IGR_OPT_HTILE;
@@ -452,6 +453,7 @@ int igr_get_option(const igr_ctx* c, int option, int* value) {
   else if (option == IGR_OPT_HTILE) *value = 100 * c->htile_x + c->htile_y;
   else if (option == IGR_OPT_RTILE) *value = 100 * c->rtile_x + c->rtile_y;
   else if (option == IGR_OPT_GROUPS) *value = c->bgroups;
+  else if (option == IGR_OPT_WAVE1D) *value = c->k1_wave;
   else return IGR_EINVAL;
   return IGR_OK;
 }

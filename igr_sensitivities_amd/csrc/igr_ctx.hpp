@@ -75,6 +75,7 @@ struct Ctx {
   int rtile_x = 4, rtile_y = 4;
   int htile_x = 4, htile_y = 4;
   int use_graphs = 1;
+  int k1_wave = 1;            // 1D: one-wave kernel for small plain runs (k1_wave.hip)
   // workgroups of the PCG x/r update (k2_pcg_b): 0 = one per tile; n > 0 =
   // at most n, each running every n-th tile
   int bgroups = 0;

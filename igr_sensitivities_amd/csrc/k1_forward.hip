@@ -151,8 +151,21 @@ __global__ __launch_bounds__(1024) void k1_run_batch(const K1Args* __restrict__ 
 // for the kernel's static LDS.
 constexpr size_t K1_LDS_MAX_BYTES = 150 * 1024;
 
+// One-wave runs of small instances (k1_wave.hip)
+int k1_wave_epl(const K1Args& a, int np);
+void launch1_wave(hipStream_t st, int np, int epl, const K1Args* d);
+
 template <int NP>
 static void launch1_run_np(const Ctx& c, K1Args a, int block) {
+  K1Args* d = reinterpret_cast<K1Args*>(c.k1args);
+  if (c.k1_wave) {
+    const int epl = k1_wave_epl(a, NP);
+    if (epl > 0) {
+      (void)hipMemcpyAsync(d, &a, sizeof(K1Args), hipMemcpyHostToDevice, c.stream);
+      launch1_wave(c.stream, NP, epl, d);
+      return;
+    }
+  }
   const size_t bytes = k1_lds_doubles(a.Ne, NP) * sizeof(double);
   size_t smem = 0;
   if ((a.op == OP_RUN || a.op == OP_STORE) && bytes <= K1_LDS_MAX_BYTES) {
@@ -165,7 +178,6 @@ static void launch1_run_np(const Ctx& c, K1Args a, int block) {
     if (attr) { a.lds = 1; smem = bytes; }
   }
   // ordered on the context stream after the previous launch that read the block
-  K1Args* d = reinterpret_cast<K1Args*>(c.k1args);
   (void)hipMemcpyAsync(d, &a, sizeof(K1Args), hipMemcpyHostToDevice, c.stream);
   hipLaunchKernelGGL(k1_run<NP>, dim3(1), dim3(block), smem, c.stream, static_cast<const K1Args*>(d));
 }

@@ -68,12 +68,16 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
  *                   SIP tile, n > 0 = at most n, each running every n-th
  *                   tile with the next tile's loads issued before the
  *                   current tile's stores (the default comes from the size
- *                   table; results change only in the dot products' order).
+ *                   table; results change only in the dot products' order);
+ *  IGR_OPT_WAVE1D   1D: 1 (default) = a plain run of N_e = 64 elements, or
+ *                   128 up to p = 3, runs in one wavefront with no barrier
+ *                   (elements in registers, faces by lane rotates), 0 = the
+ *                   one-workgroup kernel; results agree to rounding.
  * The defaults come from a size table measured on MI355X.  The shapes change
  * results only through the summation order of the solvers' dot products. */
 enum igr_option {
   IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4,
-  IGR_OPT_GROUPS = 5
+  IGR_OPT_GROUPS = 5, IGR_OPT_WAVE1D = 6
 };
 int igr_set_option(igr_ctx* ctx, int option, int value);
 int igr_get_option(const igr_ctx* ctx, int option, int* value);

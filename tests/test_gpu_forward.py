@@ -295,6 +295,49 @@ def test_solve_and_run_1d(orc, solver):
     assert rel(qT, qTr) < 1e-9 and rel(ST, STr) < 1e-9
 
 
+@pytest.mark.parametrize("p,ne", [(3, 128), (3, 64), (1, 128), (2, 64), (4, 64)])
+@pytest.mark.parametrize("solver", ["pcg", "jacobi", "chebyshev"])
+def test_wave1d_run_matches_oracle_and_workgroup_kernel(orc, p, ne, solver):
+    """Plain 1D runs of N_e = 64 (and 128 up to p = 3) take the one-wave
+    kernel (IGR_OPT_WAVE1D): 12 steps (and a run_forward with its final
+    solve) against the oracle and against the one-workgroup kernel, with
+    alpha > 0 and alpha = 0.  PCG runs 200 iterations: 20 unconverged PCG
+    iterations on these systems move by up to 5e-9 when only the summation
+    order of the dot products changes (a dense restatement, summed forwards
+    and backwards), and 36 warm-started solves carry that into Sigma at
+    1e-5; converged, the same change moves them by <= 1e-13."""
+    g = orc.Grid1D(p, ne)
+    code = igr.SOLVERS[solver]
+    n_iter = 200 if solver == "pcg" else 20
+    q0 = configs.smooth_random1(p, ne, seed=40 + p, amp=0.3)
+    dt = 0.2 / ne / ((2 * p + 1) * configs.max_wavespeed1(q0, 1.4))
+    for alpha in (10.0 * (1.0 / ne) ** 2, 0.0):
+        out = {}
+        for wave in (1, 0):
+            c = ctx1(p, ne)
+            c.set_option(igr.core.OPT_WAVE1D, wave)
+            assert c.get_option(igr.core.OPT_WAVE1D) == wave
+            c.set_state(q0)
+            c.run_steps(12, dt, 1.4, alpha, n_iter, solver)
+            q, S = c.get_state()
+            c.set_state(q0)
+            ns = c.run_forward(dt, 7.5 * dt, 1.4, alpha, n_iter, solver)
+            qf, Sf = c.get_state()
+            out[wave] = (q, S, qf, Sf, ns)
+            c.close()
+        qr, Sr = g.run_steps(1.4, alpha, n_iter, code, 12, dt, q0)
+        qfr, Sfr, nsr = g.run_forward(1.4, alpha, n_iter, code, dt, 7.5 * dt, q0)
+        for wave in (1, 0):
+            q, S, qf, Sf, ns = out[wave]
+            assert ns == nsr
+            assert rel(q, qr) < 1e-11 and rel(qf, qfr) < 1e-11, (wave, alpha)
+            if alpha > 0:
+                assert rel(S, Sr) < 1e-10 and rel(Sf, Sfr) < 1e-10, (wave, alpha)
+            else:
+                assert np.all(S == 0.0) and np.all(Sf == 0.0)
+        assert rel(out[1][0], out[0][0]) < 1e-12 and rel(out[1][2], out[0][2]) < 1e-12
+
+
 def test_config1_run_matches_oracle(orc):
     # C1 (1D, 128 elements, Jacobi-50), first 300 steps compared with the oracle
     cfg = configs.config1()
