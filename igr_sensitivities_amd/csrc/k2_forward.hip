@@ -441,17 +441,30 @@ __global__ void k2_pcg_final_batch(const FinArgs* __restrict__ tab) {
 #pragma clang fp contract(off)
 template <int NP, int TX, int TY>
 __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, const int nblk) {
-  constexpr int NP2 = NP * NP, T = TX * TY, NH = 2 * (TX + TY), NT = T * NP2;
-  __shared__ double qs[5][T * NP2];
-  __shared__ double Fx[4][T * NP2], Fy[4][T * NP2];
-  __shared__ double wsx[T * NP2], wsy[T * NP2];
-  __shared__ double hq[5][NH * NP];          // halo face nodes
-  __shared__ double fxr[4][T * NP], fxl[4][T * NP], fyt[4][T * NP], fyb[4][T * NP];
+  constexpr int NP2 = NP * NP, T = TX * TY, NT = T * NP2;
+  // Halo face nodes: the left and right strips (x faces) first, then the
+  // bottom and top strips (y faces); the y-strip nodes start on a wave
+  // boundary where the workgroup has room, so each staging wave evaluates
+  // one flux direction.
+  constexpr int NHX = 2 * TY * NP, NHY = 2 * TX * NP;
+  constexpr int HY0A = (NHX + WAVE - 1) / WAVE * WAVE;
+  constexpr int HY0 = HY0A + NHY <= NT ? HY0A : NHX;
+  static_assert(NHX + NHY <= NT, "one halo face node per thread");
+  // Stage state, fluxes and face wave speeds |u| + c, |v| + c of the tile's
+  // nodes, followed by those of the halo face nodes (x strips in Fx and wsx,
+  // y strips in Fy and wsy; all of them in qs).  Sigma enters only through
+  // the fluxes, so it is not kept.
+  __shared__ double qs[4][NT + NHX + NHY];
+  __shared__ double Fx[4][NT + NHX], Fy[4][NT + NHY];
+  __shared__ double wsx[NT + NHX], wsy[NT + NHY];
+  // Face corrections (1/J)(f* - F)/w of the tile's face nodes: element s,
+  // slot s 2NP + q for its left (bottom) face node q, s 2NP + NP + q for its
+  // right (top) one.
+  __shared__ double cx[4][2 * T * NP], cy[4][2 * T * NP];
+  __shared__ double sD[NP2];
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
-  // D in LDS (stage_basis), read per lane after the first barrier
-  __shared__ double sD[NP * NP];
   const double gam = a.gamma;
   TileInfo<NP, TX, TY> t(g, blk, nblk);
   const int tid = threadIdx.x;
@@ -461,14 +474,14 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
   const size_t N = (size_t)g.Nex * g.Ney * NP2;
   const size_t gme = valid ? t.gidx(s, n) : 0;
 
-  // Own node and (threads below NH*NP) one halo face node: both sets of
-  // loads are issued, with no branch between them, before either is written
-  // to LDS (an invalid thread or empty halo slot reads node 0 and takes the
+  // Own node and (some threads) one halo face node: both sets of loads are
+  // issued, with no branch between them, before either is written to LDS
+  // (an invalid thread or empty halo slot reads node 0 and takes the
   // default state instead); q^n of stages 2 and 3 comes with them.
-  static_assert(NH * NP <= NT, "one halo face node per thread");
-  const int hidx = tid;
+  const bool hx = tid < NHX, hy = tid >= HY0 && tid < HY0 + NHY;
+  const int hidx = hx ? tid : tid - HY0 + NHX;
   size_t hgi = 0;
-  const bool hok = hidx < NH * NP && t.halo_face_global(hidx / NP, hidx % NP, hgi);
+  const bool hok = (hx || hy) && t.halo_face_global(hidx / NP, hidx % NP, hgi);
   if (!hok) hgi = 0;
   const double* qnp = a.mode >= 2 ? a.qn : a.q;
   double lo[5], ln[4], lh[5];
@@ -488,21 +501,38 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 #pragma unroll
   for (int v = 0; v < 4; ++v) qn[v] = (valid && a.mode >= 2) ? ln[v] : 0.0;
 #pragma unroll
-  for (int v = 0; v < 5; ++v) qs[v][tid] = qo[v];
-  if (hidx < NH * NP) {
+  for (int v = 0; v < 4; ++v) qs[v][tid] = qo[v];
+  {
+    double fxo[4], fyo[4];
+    flux_x2(gam, qo, fxo);
+    flux_y2(gam, qo, fyo);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) hq[v][hidx] = hok ? lh[v] : qdef[v];
+    for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
   }
-  double fxo[4], fyo[4];
-  flux_x2(gam, qo, fxo);
-  flux_y2(gam, qo, fyo);
-#pragma unroll
-  for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
   const bool face = (i == 0 || i == NP - 1 || j == 0 || j == NP - 1);
   if (valid && face) {
     const double c = sound2(gam, qo, a.flag);
     wsx[tid] = fabs(qo[0] / qo[2]) + c;
     wsy[tid] = fabs(qo[1] / qo[2]) + c;
+  }
+  if (hx || hy) {
+    double qh[5], fh[4];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qh[v] = hok ? lh[v] : qdef[v];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qs[v][NT + hidx] = qh[v];
+    const double c = sound2(gam, qh, a.flag);
+    if (hx) {
+      flux_x2(gam, qh, fh);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Fx[v][NT + hidx] = fh[v];
+      wsx[NT + hidx] = fabs(qh[0] / qh[2]) + c;
+    } else {
+      flux_y2(gam, qh, fh);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Fy[v][NT + hidx - NHX] = fh[v];
+      wsy[NT + hidx - NHX] = fabs(qh[1] / qh[2]) + c;
+    }
   }
   stage_basis<NP>(B, sD, nullptr);
   __syncthreads();
@@ -517,96 +547,106 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * sD[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
     }
-    // face fluxes, one evaluation per face inside the tile
-    if (i == NP - 1) {
-      double qR[5], wR;
-      const int sR = t.nb_right(s);
-      if (sR < T) {
+  }
+
+  // Face terms, one LLF flux per face (hyperbolic.jl:75-116).  The tile's
+  // faces are spread over the workgroup face-major: the right face of every
+  // element (T NP items), the top face of every element (T NP), the left
+  // face of the first column (TY NP) and the bottom face of the first row
+  // (TX NP), each range starting on a wave boundary where the workgroup has
+  // room, so a wave evaluates faces of one direction and one kind (a
+  // per-node mix of the four faces left three quarters of every wave idle
+  // through the fluxes and divisions).  An item writes the correction
+  // (1/J)(f* - F)/w of each of its two nodes that lies in the tile; the
+  // node's owner applies it after the barrier.
+  {
+    constexpr int NXR = T * NP, NYT = T * NP, NXL = TY * NP, NYB = TX * NP;
+    constexpr int RU1 = (NXR + WAVE - 1) / WAVE * WAVE, RU2 = (NYT + WAVE - 1) / WAVE * WAVE;
+    constexpr int RU3 = (NXL + WAVE - 1) / WAVE * WAVE;
+    constexpr bool AL = RU1 + RU2 + RU3 + NYB <= NT;
+    constexpr int O1 = AL ? RU1 : NXR, O2 = O1 + (AL ? RU2 : NYT), O3 = O2 + (AL ? RU3 : NXL);
+    constexpr int OE = O3 + NYB;
+    // one face: L / R are qs slots, fL / fR the flux and wave-speed slots,
+    // cL / cR the correction slots (-1: the node is a halo node)
+    auto face_item = [&](auto Dc, const int L, const int R, const int fL_, const int fR_, const int cL,
+                         const int cR) {
+      constexpr int D = decltype(Dc)::value;
+      double qL[4], qR[4], fL[4], fR[4], wL, wR;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qR[v] = qs[v][sR * NP2 + NP * j];
-        wR = wsx[sR * NP2 + NP * j];
+      for (int v = 0; v < 4; ++v) { qL[v] = qs[v][L]; qR[v] = qs[v][R]; }
+      if constexpr (D == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { fL[v] = Fx[v][fL_]; fR[v] = Fx[v][fR_]; }
+        wL = wsx[fL_]; wR = wsx[fR_];
       } else {
-        const int h = sR - T;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qR[v] = hq[v][h * NP + j];
-        wR = fabs(qR[0] / qR[2]) + sound2(gam, qR, a.flag);
+        for (int v = 0; v < 4; ++v) { fL[v] = Fy[v][fL_]; fR[v] = Fy[v][fR_]; }
+        wL = wsy[fL_]; wR = wsy[fR_];
       }
-      double fR[4], f[4];
-      flux_x2(gam, qR, fR);
-      llf_combine(fxo, fR, qo, qR, fmax_first(wsx[tid], wR), f);
+      double f[4];
+      llf_combine(fL, fR, qL, qR, fmax_first(wL, wR), f);
+      const double inv = D == 0 ? g.invJx : g.invJy;
+      double (*cc)[2 * T * NP] = D == 0 ? cx : cy;
+      if (cL >= 0) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) fxr[v][s * NP + j] = f[v];
-    }
-    if (i == 0 && lx == 0) {
-      double qL[5];
-      const int h = t.nb_left(s) - T;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) qL[v] = hq[v][h * NP + j];
-      const double wL = fabs(qL[0] / qL[2]) + sound2(gam, qL, a.flag);
-      double fL[4], f[4];
-      flux_x2(gam, qL, fL);
-      llf_combine(fL, fxo, qL, qo, fmax_first(wL, wsx[tid]), f);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) fxl[v][s * NP + j] = f[v];
-    }
-    if (j == NP - 1) {
-      double qR[5], wR;
-      const int sT = t.nb_top(s);
-      if (sT < T) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) qR[v] = qs[v][sT * NP2 + i];
-        wR = wsy[sT * NP2 + i];
-      } else {
-        const int h = sT - T;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) qR[v] = hq[v][h * NP + i];
-        wR = fabs(qR[1] / qR[2]) + sound2(gam, qR, a.flag);
+        for (int v = 0; v < 4; ++v) cc[v][cL] = inv * (f[v] - fL[v]) / B.w[NP - 1];
       }
-      double fR[4], f[4];
-      flux_y2(gam, qR, fR);
-      llf_combine(fyo, fR, qo, qR, fmax_first(wsy[tid], wR), f);
+      if (cR >= 0) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) fyt[v][s * NP + i] = f[v];
-    }
-    if (j == 0 && ly == 0) {
-      double qL[5];
-      const int h = t.nb_bottom(s) - T;
+        for (int v = 0; v < 4; ++v) cc[v][cR] = inv * (f[v] - fR[v]) / B.w[0];
+      }
+    };
 #pragma unroll
-      for (int v = 0; v < 5; ++v) qL[v] = hq[v][h * NP + i];
-      const double wL = fabs(qL[1] / qL[2]) + sound2(gam, qL, a.flag);
-      double fL[4], f[4];
-      flux_y2(gam, qL, fL);
-      llf_combine(fL, fyo, qL, qo, fmax_first(wL, wsy[tid]), f);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) fyb[v][s * NP + i] = f[v];
+    for (int w0 = 0; w0 < OE; w0 += NT) {
+      const int w = w0 + tid;
+      if (w < NXR) {                                   // right face of element e, node q along it
+        const int e = w / NP, q = w % NP;
+        if (e % TX < t.nvx && e / TX < t.nvy) {
+          const int L = e * NP2 + (NP - 1) + NP * q;
+          const int eR = t.nb_right(e);
+          const int R = eR < T ? eR * NP2 + NP * q : NT + (eR - T) * NP + q;
+          face_item(std::integral_constant<int, 0>{}, L, R, L, R, e * 2 * NP + NP + q,
+                    eR < T ? eR * 2 * NP + q : -1);
+        }
+      } else if (w >= O1 && w < O1 + NYT) {            // top face
+        const int e = (w - O1) / NP, q = (w - O1) % NP;
+        if (e % TX < t.nvx && e / TX < t.nvy) {
+          const int L = e * NP2 + q + NP * (NP - 1);
+          const int eT = t.nb_top(e);
+          const int h = (eT - T) * NP + q;              // halo slot when eT >= T
+          face_item(std::integral_constant<int, 1>{}, L, eT < T ? eT * NP2 + q : NT + h, L,
+                    eT < T ? eT * NP2 + q : NT + h - NHX, e * 2 * NP + NP + q, eT < T ? eT * 2 * NP + q : -1);
+        }
+      } else if (w >= O2 && w < O2 + NXL) {            // left face of the first column
+        const int r = (w - O2) / NP, q = (w - O2) % NP;
+        if (r < t.nvy) {
+          const int e = r * TX;
+          const int R = e * NP2 + NP * q;
+          const int L = NT + r * NP + q;                  // left strip, slot r
+          face_item(std::integral_constant<int, 0>{}, L, R, L, R, -1, e * 2 * NP + q);
+        }
+      } else if (w >= O3 && w < OE) {                  // bottom face of the first row
+        const int e = (w - O3) / NP, q = (w - O3) % NP;
+        if (e < t.nvx) {
+          const int R = e * NP2 + q;
+          const int h = (2 * TY + e) * NP + q;            // bottom strip, slot e
+          face_item(std::integral_constant<int, 1>{}, NT + h, R, NT + h - NHX, R, -1, e * 2 * NP + q);
+        }
+      }
     }
   }
   __syncthreads();
   if (!valid) return;
   // face corrections: x faces, then y faces (hyperbolic.jl:75-116)
-  if (i == NP - 1) {
+  const int xsl = s * 2 * NP + (i == 0 ? 0 : NP) + j;
+  const int ysl = s * 2 * NP + (j == 0 ? 0 : NP) + i;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * (fxr[v][s * NP + j] - fxo[v]) / B.w[NP - 1];
-  }
-  if (i == 0) {
-    const int sL = t.nb_left(s);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const double f = (sL < T) ? fxr[v][sL * NP + j] : fxl[v][s * NP + j];
-      acc[v] += g.invJx * (f - fxo[v]) / B.w[0];
-    }
-  }
-  if (j == NP - 1) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * (fyt[v][s * NP + i] - fyo[v]) / B.w[NP - 1];
-  }
-  if (j == 0) {
-    const int sB = t.nb_bottom(s);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const double f = (sB < T) ? fyt[v][sB * NP + i] : fyb[v][s * NP + i];
-      acc[v] += g.invJy * (f - fyo[v]) / B.w[0];
-    }
+  for (int v = 0; v < 4; ++v) {
+    const double tx = cx[v][xsl], ty = cy[v][ysl];
+    double r = acc[v];
+    r = i == NP - 1 ? r - tx : (i == 0 ? r + tx : r);
+    r = j == NP - 1 ? r - ty : (j == 0 ? r + ty : r);
+    acc[v] = r;
   }
   // SSP-RK3 stage combination (IGR_adjoints_2d/forward.jl:53-68)
 #pragma unroll

@@ -61,6 +61,22 @@ def test_hyperbolic_rhs_2d_bitwise(orc, nx, ny):
     np.testing.assert_array_equal(c.hyperbolic_rhs(1.4, q, S), g.hyperbolic_rhs(1.4, q, S))
 
 
+@pytest.mark.parametrize("p,htile", [(1, 0), (2, 0), (3, 404), (3, 804), (3, 202), (4, 0)])
+def test_hyperbolic_rhs_2d_bitwise_degrees_and_tiles(orc, p, htile):
+    """Bit-equal hyperbolic RHS for every degree and p = 3 tile shape (the
+    face items of a tile fill several waves, or loop when they outnumber
+    the threads at p = 1), on meshes the tiles divide unevenly and on a
+    single element (every face periodic onto itself)."""
+    for nx, ny in ((20, 12), (9, 17), (1, 1)):
+        c = ctx2(p, nx, ny, 1.0, 0.7)
+        if htile:
+            c.set_option(igr.core.OPT_HTILE, htile)
+        g = orc.Grid2D(p, nx, ny, 1.0, 0.7)
+        q = configs.smooth_random2(p, nx, ny, seed=7 * nx + ny + p, amp=0.3)
+        S = 0.01 * np.random.default_rng(p).standard_normal(g.shape)
+        np.testing.assert_array_equal(c.hyperbolic_rhs(1.4, q, S), g.hyperbolic_rhs(1.4, q, S)), (p, htile, nx, ny)
+
+
 def test_igr_rhs_and_diag_2d(orc):
     c = ctx2(3, 6, 5)
     g = orc.Grid2D(3, 6, 5)
