@@ -165,11 +165,14 @@ struct GradWork {
   }
 };
 
-// Doubles of one StepSet for n solver iterations on N nodes.
+// Doubles of one StepSet for n solver iterations on N nodes, as alloc_sets
+// carves it: every piece rounded up to 32 doubles (256 B).
+static size_t carve_round(size_t cnt) { return (cnt + 31) / 32 * 32; }
 static size_t step_set_doubles(size_t N, int n) {
-  const size_t per_stage = (size_t)std::max(n, 1) * N * 2 + (size_t)(n + 1) * N + 3 * N +
-                           (4 * (size_t)std::max(n, 1) + 1) + (5 * (size_t)std::max(n, 1) + 1);
-  return 3 * per_stage + 3 * 4 * N + 4 * N;
+  const size_t n1 = (size_t)std::max(n, 1);
+  const size_t per_stage = 2 * carve_round(n1 * N) + carve_round((size_t)(n + 1) * N) + 3 * carve_round(N) +
+                           carve_round(4 * n1 + 1) + carve_round(5 * n1 + 1);
+  return 3 * per_stage + 3 * carve_round(4 * N) + 4 * carve_round(N);
 }
 
 // Carve K step sets out of one allocation (kept until the context goes).
@@ -177,14 +180,15 @@ static int alloc_sets(Ctx& c, GradWork& W, int K) {
   if ((int)W.sets.size() >= K) return IGR_OK;
   const size_t N = c.N;
   const int n = W.n_iter;
-  const size_t per = step_set_doubles(N, n) + 64;
+  const size_t per = step_set_doubles(N, n);
   if (W.set_block) { pool_give(W.set_block, W.set_block_pooled); W.set_block = nullptr; W.sets.clear(); }
   const size_t bytes = per * sizeof(double) * (size_t)K;
   W.set_block = pool_take(bytes, W.set_block_pooled);
   if (!W.set_block) return IGR_ENOMEM;
   W.set_block_bytes = bytes;
   double* p = (double*)W.set_block;
-  auto take = [&](size_t cnt) { double* r = p; p += (cnt + 31) / 32 * 32; return r; };
+  double* const end = p + per * (size_t)K;
+  auto take = [&](size_t cnt) { double* r = p; p += carve_round(cnt); return r; };
   W.sets.resize(K);
   for (int k = 0; k < K; ++k) {
     StepSet& S = W.sets[k];
@@ -196,6 +200,12 @@ static int alloc_sets(Ctx& c, GradWork& W, int K) {
     }
     for (int q = 0; q < 3; ++q) S.Q[q] = take(4 * N);
     for (int q = 0; q < 4; ++q) S.Sg[q] = take(N);
+  }
+  if (p != end) {   // the carve must fill exactly what step_set_doubles sized
+    pool_give(W.set_block, W.set_block_pooled);
+    W.set_block = nullptr;
+    W.sets.clear();
+    return IGR_EINVAL;
   }
   return IGR_OK;
 }
@@ -788,7 +798,7 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
     // steps as 85% of the free HBM holds, so their reverse needs no recompute
     size_t fr = 0, tot = 0;
     HIPCHK(*c, hipMemGetInfo(&fr, &tot));
-    const size_t per = (step_set_doubles(N, n_iter) + 64) * sizeof(double);
+    const size_t per = step_set_doubles(N, n_iter) * sizeof(double);
     const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes + pool_idle_bytes()));
     int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
     // the block grows only when it would at least double: growing it frees
