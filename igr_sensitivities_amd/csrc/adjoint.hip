@@ -565,19 +565,28 @@ static void step_rev(Ctx& c, GradWork& W, StepView V, double dt, double gamma, d
 // ---- binomial checkpointing -------------------------------------------------
 // cost(n, s): forward steps (without storage) spent reversing n steps whose
 // first state is checkpointed, with s free slots; split(n, s): steps to
-// advance before placing the next checkpoint.
+// advance before placing the next checkpoint.  With an iterate cache of K
+// whole steps (2D), a segment of at most L = K + 1 steps is a window: one
+// forward pass from its first state stores the iterates of its first K steps
+// in the cache, its last step is reversed as a base step, then the cached
+// ones, with no recomputation; every step is stored once either way, so a
+// window costs no forward steps.  A segment with no free slot advances to
+// its last window, reverses it, and does the same for the rest from its own
+// first state.  L = 1 is the plain binomial (Revolve) schedule.
 struct Schedule {
   std::vector<std::vector<long long>> cost;
   std::vector<std::vector<int>> split;
   bool store_all = false;
-  void build(int n, int s) {
+  int win_len = 1;
+  void build(int n, int s, int L) {
     if (s >= n - 1) { store_all = true; return; }   // every step gets a slot: advance one step at a time
+    win_len = std::max(L, 1);
     cost.assign(n + 1, std::vector<long long>(s + 1, 0));
     split.assign(n + 1, std::vector<int>(s + 1, 1));
     for (int m = 1; m <= n; ++m)
       for (int k = 0; k <= s; ++k) {
-        if (m == 1) { cost[m][k] = 0; continue; }
-        if (k == 0) { cost[m][k] = (long long)m * (m - 1) / 2; continue; }
+        if (m <= win_len) { cost[m][k] = 0; continue; }
+        if (k == 0) { cost[m][k] = (m - win_len) + cost[m - win_len][0]; continue; }
         long long best = -1; int bj = 1;
         for (int j = 1; j < m; ++j) {
           const long long v = j + cost[m - j][k - 1] + cost[j][k];
@@ -688,6 +697,26 @@ struct GradRun {
     }
     std::swap(W->QBn, W->QBnext);
   }
+  // reverse the window [a, b) (b - a <= cache + 1) from its first state, in
+  // (c.q, c.S) already or restored from slot k: steps a .. b-2 run once,
+  // storing their iterates in the cache, step b-1 is a base step, then the
+  // cached steps are reversed
+  void window(int a, int b, int k, bool restore_first) {
+    if (restore_first) restore(k);
+    for (int m = a; m < b - 1; ++m) {
+      StepSet& S = W->sets[m - a];
+      launch_copy(c, S.Q[0], c.q, 4 * c.N);
+      launch_copy(c, S.Sg[0], c.S, c.N);
+      step_store(c, S.view(), dt, gamma, alpha, n_iter, solver, c.q);
+      launch_copy(c, c.S, S.Sg[3], c.N);
+    }
+    base_step(b - 1);
+    for (int m = b - 2; m >= a; --m) {
+      step_rev(c, *W, W->sets[m - a].view(), dt, gamma, alpha, n_iter, solver, !first_base);
+      first_base = false;
+      std::swap(W->QBn, W->QBnext);
+    }
+  }
   // reverse steps [a, b): the state at a is in `slot`; the slots after it are free
   void reverse_range(int a, int b, int k, int free_slots) {
     const int n = b - a;
@@ -696,8 +725,20 @@ struct GradRun {
       base_step(a);
       return;
     }
+    const int L = sch.win_len;
+    if (!sch.store_all && L > 1 && n <= L) {
+      window(a, b, k, true);
+      return;
+    }
     const int fs = std::min(free_slots, n - 1);
     if (fs == 0) {
+      if (!sch.store_all && L > 1) {
+        restore(k);
+        advance(a, n - L);
+        window(b - L, b, k, false);
+        reverse_range(a, b - L, k, 0);
+        return;
+      }
       for (int m = b - 1; m >= a; --m) { restore(k); advance(a, m - a); base_step(m); }
       return;
     }
@@ -792,23 +833,29 @@ int igr_gradient(igr_ctx* c, int objective, const double* prm, int n_steps, doub
   run.n_total = n_steps;
   const double* pp = prm ? prm : KE_WINDOW_DEFAULT;
   for (int k = 0; k < 4; ++k) run.prm[k] = pp[k];
-  run.sch.build(n_steps, slots - 1);
-  if (c->dim == 2 && run.sch.store_all && n_steps >= 2) {
-    // every step is checkpointed: keep the iterates of as many of the last
-    // steps as 85% of the free HBM holds, so their reverse needs no recompute
+  // The iterate cache: whole steps whose solver iterates are kept in HBM, as
+  // many as 85% of the free memory holds (capped by IGR_OPT_KEEP).  With
+  // every step checkpointed it keeps the last steps from the forward sweep;
+  // otherwise it sets the schedule's window length.
+  const bool store_all = slots - 1 >= n_steps - 1;
+  int K = 0;
+  if (c->dim == 2 && n_steps >= 2 && c->keep_steps != 0) {
     size_t fr = 0, tot = 0;
     HIPCHK(*c, hipMemGetInfo(&fr, &tot));
     const size_t per = step_set_doubles(N, n_iter) * sizeof(double);
     const size_t budget = (size_t)(0.85 * (double)(fr + run.W->set_block_bytes + pool_idle_bytes()));
-    int K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
+    K = (int)std::min<size_t>((size_t)(n_steps - 1), budget / per);
+    if (c->keep_steps > 0) K = std::min(K, c->keep_steps);
     // the block grows only when it would at least double: growing it frees
     // and obtains it again (about 30 ms per GB of a just-released block),
     // while a few more kept steps save a few step recomputes (the free
     // memory seen here can differ by a set between calls)
     const int K_have = (int)run.W->sets.size();
     if (K_have > 0 && K > K_have && K < 2 * K_have) K = K_have;
-    if (K >= 1 && alloc_sets(*c, *run.W, K) == IGR_OK) run.kfirst = n_steps - 1 - K;
+    if (K < 1 || alloc_sets(*c, *run.W, K) != IGR_OK) K = 0;
   }
+  run.sch.build(n_steps, slots - 1, store_all ? 1 : K + 1);
+  if (store_all && K >= 1) run.kfirst = n_steps - 1 - K;
   run.save(0);
   run.reverse_range(0, n_steps, 0, slots - 1);
   // the last reversed step left dJ/dq0 in QBnext

@@ -435,6 +435,11 @@ int igr_set_option(igr_ctx* c, int option, int value) {
     }
     return IGR_OK;
   }
+  if (option == IGR_OPT_KEEP) {
+    if (value < -1) return seterr(*c, IGR_EINVAL, "kept steps must be >= -1");
+    c->keep_steps = value;
+    return IGR_OK;
+  }
   if (option == IGR_OPT_AUTOTUNE) {
     if (c->dim != 2 || c->np != 4) return IGR_OK;   // one tile shape exists for the other cases
     int rc = begin_call(*c);
@@ -454,6 +459,7 @@ int igr_get_option(const igr_ctx* c, int option, int* value) {
   else if (option == IGR_OPT_RTILE) *value = 100 * c->rtile_x + c->rtile_y;
   else if (option == IGR_OPT_GROUPS) *value = c->bgroups;
   else if (option == IGR_OPT_WAVE1D) *value = c->k1_wave;
+  else if (option == IGR_OPT_KEEP) *value = c->keep_steps;
   else return IGR_EINVAL;
   return IGR_OK;
 }

@@ -79,6 +79,9 @@ struct Ctx {
   // workgroups of the PCG x/r update (k2_pcg_b): 0 = one per tile; n > 0 =
   // at most n, each running every n-th tile
   int bgroups = 0;
+  // 2D gradient: the most whole steps whose solver iterates are kept in HBM
+  // (-1: as many as 85% of the free memory holds; 0: none)
+  int keep_steps = -1;
 
   // run_forward_store snapshots: records of (state, Sigma), (nv + 1) N
   // doubles each, for the continuous adjoint (adjpde.hip)

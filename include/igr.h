@@ -72,12 +72,20 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
  *  IGR_OPT_WAVE1D   1D: 1 (default) = a plain run of N_e = 64 elements, or
  *                   128 up to p = 3, runs in one wavefront with no barrier
  *                   (elements in registers, faces by lane rotates), 0 = the
- *                   one-workgroup kernel; results agree to rounding.
+ *                   one-workgroup kernel; results agree to rounding;
+ *  IGR_OPT_KEEP     2D gradient: the most whole steps whose solver iterates
+ *                   are kept in HBM (-1, the default: as many as 85% of the
+ *                   free memory holds; 0: none, the plain binomial schedule).
+ *                   With every step checkpointed they are the last steps of
+ *                   the forward sweep; with fewer checkpoint slots a segment
+ *                   of up to that many steps + 1 is reversed after one
+ *                   storing pass, without recomputation.  Gradients are
+ *                   bit-identical for every value.
  * The defaults come from a size table measured on MI355X.  The shapes change
  * results only through the summation order of the solvers' dot products. */
 enum igr_option {
   IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4,
-  IGR_OPT_GROUPS = 5, IGR_OPT_WAVE1D = 6
+  IGR_OPT_GROUPS = 5, IGR_OPT_WAVE1D = 6, IGR_OPT_KEEP = 7
 };
 int igr_set_option(igr_ctx* ctx, int option, int value);
 int igr_get_option(const igr_ctx* ctx, int option, int* value);

@@ -198,10 +198,19 @@ def test_checkpoint_schedules_bitwise_equal(solver):
     q0 = configs.smooth_random2(3, nx, ny, seed=70, amp=0.3)
     dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
     res = []
+    # slots 2, 3, 5: binomial schedules, with the iterate cache off (keep 0),
+    # small (windows of 2 or 4 steps, more windows than free slots) or as
+    # large as the memory holds (one window); 10: every step checkpointed
     for slots in (2, 3, 5, 10):
-        c = ctx2(3, nx, ny)
-        c.set_state(q0)
-        res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 15, solver, prm=(0.55, 0.5, 0.15, 0.0), ckpt_slots=slots))
+        for keep in (0, 1, 3, -1):
+            c = ctx2(3, nx, ny)
+            c.set_option(igr.core.OPT_KEEP, keep)
+            assert c.get_option(igr.core.OPT_KEEP) == keep
+            c.set_state(q0)
+            res.append(c.gradient(3, ns, dt, 1.4, 3e-3, 15, solver, prm=(0.55, 0.5, 0.15, 0.0), ckpt_slots=slots))
+            c.close()
+    with pytest.raises(ValueError):
+        ctx2(3, nx, ny).set_option(igr.core.OPT_KEEP, -2)
     for r in res[1:]:
         assert r[0] == res[0][0]
         np.testing.assert_array_equal(r[1], res[0][1])
