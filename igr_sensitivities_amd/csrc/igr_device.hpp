@@ -166,6 +166,41 @@ __device__ __forceinline__ void block_partials_store(double (&v)[NV], double* /*
   }
 }
 
+// The same partial sums as block_partials_store without its barrier: each
+// wave writes its sums to LDS and counts itself in *arrive (an LDS counter
+// the caller sets to 0 before an earlier barrier); the wave that counts
+// last adds the per-wave sums in wave order, as block_partials_store does,
+// and stores the first NSTORE of them.  The other waves go on at once (to
+// their stores, which a barrier would have waited for).  Returns true in
+// the last wave, whose lanes then hold all NV workgroup totals in v.
+template <int NV, int NSTORE = NV>
+__device__ __forceinline__ bool block_partials_store_last(double (&v)[NV], double* part, int nb, int bid,
+                                                          unsigned* arrive) {
+  __shared__ double sp[NV * MAX_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  wave_sums_last<NV>(v);
+  const int sl = wave_sum_lane();
+  unsigned prev = 0;
+  if (lane == sl) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sp[k * nw + wid] = v[k];
+    prev = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  prev = (unsigned)__builtin_amdgcn_readlane((int)prev, sl);
+  const bool last = prev == (unsigned)(nw - 1);
+  if (last) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += sp[k * nw + w];
+      v[k] = s;
+      if (k < NSTORE && lane == sl) part[(size_t)k * nb + bid] = s;
+    }
+  }
+  return last;
+}
+
 // Every workgroup sums the nb partials of the previous kernel in the same
 // fixed order (per thread, then per wave, then over the waves), so all
 // workgroups hold bit-identical totals.  grid_prefetch issues the loads of

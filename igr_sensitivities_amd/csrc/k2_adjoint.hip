@@ -206,6 +206,8 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const int k = a.k;
   // basis in LDS (stage_basis), read per lane below
   __shared__ double sD[NP * NP], sW[NP];
+  __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
+  if (tid == 0) arrive = 0;     // published by the staging barrier
 
   // reverse scalars of this iteration, from the dot products the previous
   // kernel left as partial sums (every workgroup forms the same totals)
@@ -511,11 +513,11 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
       }
     }
     // the four dot products for the next iteration and this block's alpha-bar
-    // share one reduction (the alpha-bar total is not stored as a partial);
-    // its barrier comes before the stores, which a barrier would wait for
+    // share one reduction (the alpha-bar total is not stored as a partial)
     double dv5[5] = {dv[0], dv[1], dv[2], dv[3], ab};
-    block_partials_store<5, 4>(dv5, red, a.part_out, gridDim.x, blockIdx.x);
-    if (tid == 0) a.abar_blk[blockIdx.x] = f_ab + dv5[4];
+    if (block_partials_store_last<5, 4>(dv5, a.part_out, gridDim.x, blockIdx.x, &arrive) &&
+        (tid & (WAVE - 1)) == wave_sum_lane())
+      a.abar_blk[blockIdx.x] = f_ab + dv5[4];
     if (valid) {
       a.irbar[gme] = f_irb + irb;
       a.imbar[gme] = f_imb + o_zb * o_R;

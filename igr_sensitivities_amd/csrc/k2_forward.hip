@@ -57,7 +57,6 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   __shared__ double fP[T * 4 * NP], fQ[T * 4 * NP], fS[T * 4 * NP];   // face-node terms
   __shared__ double fG[INIT ? T * 4 * NP : 1];                        // penalty coefficients
   __shared__ double us[INIT ? T * NP2 : 1], ws_[INIT ? T * NP2 : 1];
-  __shared__ double red[4 * (NT / WAVE + 1)];
 
   const Geo2 g = a.g;
   const BasisK& B = a.B;
@@ -70,6 +69,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
 
   // basis in LDS (stage_basis), read per lane below
   __shared__ double sD[NP * NP], sW[NP];
+  __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
+  if (tid == 0) arrive = 0;
 
   double own_d = 0.0;
   double beta = 0.0, sc_tot = 0.0;
@@ -300,8 +301,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         o_r = bval - y;
       }
     }
-    // the reduction's barrier first: stores ahead of a barrier are waited for
-    if constexpr (MODE == M_INIT_PCG) block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+    if constexpr (MODE == M_INIT_PCG) block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
     if (valid) {
       a.invR_out[gme] = ir;
       a.invM_out[gme] = o_iM;
@@ -315,9 +315,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       }
     }
   } else if constexpr (MODE == M_PCG_A) {
-    // the reduction's barrier first: stores ahead of a barrier are waited for
     double red_v[1] = {valid ? vown * y : 0.0};
-    block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+    block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
     if (valid) {
       a.pnew[gme] = vown;
       a.y[gme] = y;
@@ -358,10 +357,11 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs*
 // come before the grid total of the previous kernel's partial sums.
 template <int NP, int TX, int TY, int GU = 8>
 __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
-  constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
-  __shared__ double red[NT / WAVE + 1];
+  constexpr int NP2 = NP * NP;
+  __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
   const int ntiles = tiles_of<TX, TY>(a.g);
   const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
+  if (tid == 0) arrive = 0;     // published by grid_finish's barrier
   const int lx = s % TX, ly = s / TX;
   double* sc = a.sc;
   const int nit = a.n_iter, it = a.it;
@@ -398,7 +398,7 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
     }
   }
   double red_v[1] = {acc};
-  block_partials_store<1>(red_v, red, a.part_out, nblk, blk);
+  block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
   if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
 }
 template <int NP, int TX, int TY>
