@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 
 SOLVERS = {"pcg": 0, "jacobi": 1, "chebyshev": 2}
-OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE, OPT_RTILE, OPT_GROUPS, OPT_WAVE1D, OPT_KEEP = 0, 1, 2, 3, 4, 5, 6, 7
+OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE, OPT_RTILE, OPT_GROUPS, OPT_WAVE1D, OPT_KEEP, OPT_RGROUPS = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
 
 class DomainError(ArithmeticError):

@@ -79,6 +79,10 @@ struct Ctx {
   // workgroups of the PCG x/r update (k2_pcg_b): 0 = one per tile; n > 0 =
   // at most n, each running every n-th tile
   int bgroups = 0;
+  // workgroups of the reverse PCG iteration (k2_sip_rev PCG_ITER) and its
+  // start kernel: 0 = one per reverse tile; n > 0 = at most n, each running
+  // every n-th tile and storing one partial sum per dot product
+  int rgroups = 0;
   // 2D gradient: the most whole steps whose solver iterates are kept in HBM
   // (-1: as many as 85% of the free memory holds; 0: none)
   int keep_steps = -1;

@@ -32,6 +32,15 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// A wave-uniform double moved to scalar registers (the first active lane's
+// value); for values every lane holds alike, such as grid totals.
+__device__ __forceinline__ double uniform_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Wavefront sums of NV values, in a fixed order, valid in the last lane
 // (WAVE - 1).  A workgroup whose size is a multiple of 64 (every lane of
 // every wave active) sums by DPP lane moves, the NV values step by step

@@ -178,8 +178,19 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
 // ------------------------------------------------------------------------
 // Solve reverse (PCG) and the standalone SIP VJP
 // ------------------------------------------------------------------------
+// State a reverse PCG iteration workgroup carries from tile to tile: the
+// reverse scalars (formed once, from the previous kernel's partial sums) and
+// its running sums of the four dot products and of alpha-bar.
+struct RevLoop {
+  double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
+  double r_bb = 0.0, r_abk = 0.0, r_carry = 0.0;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+};
+
+// One tile of the reverse SIP family: tile `tile` of `ntiles` (a single-tile
+// workgroup passes its block index and the grid size).
 template <int NP, int TX, int TY, int MODE>
-__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
+__device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile, const int ntiles, RevLoop& L) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool FIN = (MODE == RV_PCG_FIN || MODE == RV_JAC_FIN || MODE == RV_CHEB_FIN);
   __shared__ double ws[NS * NP2];   // adjoint vector w (A is applied to it)
@@ -195,7 +206,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const Geo2 g = a.g;
   const BasisK& B = a.B;
   const double al = a.alpha;
-  TileInfo<NP, TX, TY> t(g);
+  TileInfo<NP, TX, TY> t(g, tile, ntiles);
   const int tid = threadIdx.x;
   const int s = tid / NP2, n = tid % NP2, i = n % NP, j = n / NP;
   const int lx = s % TX, ly = s / TX;
@@ -206,22 +217,18 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   const int k = a.k;
   // basis in LDS (stage_basis), read per lane below
   __shared__ double sD[NP * NP], sW[NP];
-  __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
-  if (tid == 0) arrive = 0;     // published by the staging barrier
 
   // reverse scalars of this iteration, from the dot products the previous
   // kernel left as partial sums (every workgroup forms the same totals)
   double ak = 0.0, bk = 0.0, rzb = 0.0, papb = 0.0;
-  double r_bb = 0.0, r_abk = 0.0, r_carry = 0.0;   // reverse scalars block 0 records at the end
   // PCG_ITER: the previous kernel's partial sums are requested first; then
   // the tile + halo fields and this thread's own-node inputs, every address
   // first and the loads with no branch between them (an empty slot reads
   // node 0 and is zeroed afterwards), so all are in flight together
-  double pre4[4] = {0.0, 0.0, 0.0, 0.0};
-  if constexpr (MODE == RV_PCG_ITER) grid_prefetch<4>(a.part_in, gridDim.x, pre4);
+
   constexpr int NL = (NS * NP2 + NT - 1) / NT;
   double l_ir[NL], l_iM[NL], l_rc[NL], l_pb[NL], l_R[NL], l_p[NL];
-  double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0, f_ab = 0.0;
+  double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0;
   if constexpr (MODE == RV_PCG_ITER) {
     size_t gis[NL];
     bool oks[NL];
@@ -250,7 +257,6 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     }
     f_irb = a.irbar[gme]; f_imb = a.imbar[gme]; f_xb = a.xbar[gme]; f_apk = APk[gme];
     f_pm = Pm[gme]; f_apm = APm[gme]; f_Rk = Rk[gme];
-    f_ab = a.abar_blk[blockIdx.x];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       if (!oks[l]) { l_ir[l] = 0.0; l_iM[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0; }
@@ -260,21 +266,7 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
     if (!valid || k < 1) { f_pm = 0.0; f_apm = 0.0; f_Rk = 0.0; }
   }
   if constexpr (MODE == RV_PCG_ITER) {
-    // the stored scalars are loaded with the partials, not after them
-    ak = a.sc[fs.a(k)];
-    bk = a.sc[fs.beta(k)];
-    const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
-    const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
-    double d[4];
-    grid_finish<4, 4>(a.part_in, gridDim.x, pre4, d);
-    const double eps = 1e-30;
-    const double bb = d[0];
-    rzb = carry_next + bb / (rzk + eps);
-    const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
-    papb = -abk * rzk / ((papk + eps) * (papk + eps));
-    // stored after the last barrier (a barrier waits for earlier stores)
-    r_bb = bb; r_abk = abk;
-    r_carry = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
+    ak = L.ak; bk = L.bk; rzb = L.rzb; papb = L.papb;
   } else if constexpr (FIN) {
     rzb = a.n_iter > 0 ? a.rs[RS::carry(0)] : 0.0;
   }
@@ -512,24 +504,16 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
         dv[3] = (f_Rk * iM) * apm;
       }
     }
-    // the four dot products for the next iteration and this block's alpha-bar
-    // share one reduction (the alpha-bar total is not stored as a partial)
-    double dv5[5] = {dv[0], dv[1], dv[2], dv[3], ab};
-    if (block_partials_store_last<5, 4>(dv5, a.part_out, gridDim.x, blockIdx.x, &arrive) &&
-        (tid & (WAVE - 1)) == wave_sum_lane())
-      a.abar_blk[blockIdx.x] = f_ab + dv5[4];
+    // the four dot products for the next iteration and the alpha-bar terms,
+    // summed per thread over the workgroup's tiles (the kernel reduces them)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) L.acc[q] += dv[q];
+    L.acc[4] += ab;
     if (valid) {
       a.irbar[gme] = f_irb + irb;
       a.imbar[gme] = f_imb + o_zb * o_R;
       a.rbar_out[gme] = o_rb;
       a.pbar_out[gme] = pbk;
-    }
-    if (blockIdx.x == 0 && tid == 0) {
-      a.rs[RS::bbar(k)] = r_bb;
-      a.rs[RS::abar(k)] = r_abk;
-      a.rs[RS::papbar(k)] = papb;
-      a.rs[RS::rzbar_next(k)] = rzb;
-      a.rs[RS::carry(k)] = r_carry;
     }
   } else if constexpr (MODE == RV_JAC_ITER) {
     if (valid) {
@@ -622,20 +606,95 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
   }
 }
 
-// First reverse PCG step: a-bar_{n-1} = <x-bar, p_{n-1}> (the later adjoints are 0)
+// The reverse SIP family.  The reverse PCG iteration may run on a bounded
+// grid (the launcher's workgroup count, at most one per tile): a workgroup forms
+// the iteration's reverse scalars once from the previous kernel's partial
+// sums, then runs tiles blockIdx.x, blockIdx.x + gridDim.x, ..., and stores
+// one partial sum per dot product for the whole workgroup, so the next
+// iteration reduces gridDim.x partials instead of one per tile.  The other
+// modes run one tile per workgroup.
+template <int NP, int TX, int TY, int MODE, bool LOOP>
+__device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
+  RevLoop L;
+  if constexpr (MODE == RV_PCG_ITER) {
+    __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
+    const int ntiles = tiles_of<TX, TY>(a.g);
+    if (threadIdx.x == 0) arrive = 0;   // published by grid_finish's barrier
+    // the iteration's reverse scalars from the previous kernel's partial
+    // sums (the stored forward scalars are loaded with them)
+    const int k = a.k;
+    const FS fs(a.n_iter);
+    double pre4[4];
+    grid_prefetch<4>(a.part_in, a.np_in, pre4);
+    const double f_ab = a.abar_blk[blockIdx.x];
+    L.ak = a.sc[fs.a(k)];
+    L.bk = a.sc[fs.beta(k)];
+    const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
+    const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
+    double d[4];
+    grid_finish<4, 4>(a.part_in, a.np_in, pre4, d);
+    const double eps = 1e-30;
+    const double bb = d[0];
+    const double rzb = carry_next + bb / (rzk + eps);
+    const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
+    const double papb = -abk * rzk / ((papk + eps) * (papk + eps));
+    const double carry = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
+    // every lane holds the same totals: keep them in scalar registers
+    L.rzb = uniform_d(rzb); L.papb = uniform_d(papb);
+    L.r_bb = uniform_d(bb); L.r_abk = uniform_d(abk); L.r_carry = uniform_d(carry);
+    if constexpr (LOOP) {
+      for (int tt = blockIdx.x; tt < ntiles; tt += gridDim.x) k2_sip_rev_body<NP, TX, TY, MODE>(a, tt, ntiles, L);
+    } else {
+      k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L);
+    }
+    double dv5[5] = {L.acc[0], L.acc[1], L.acc[2], L.acc[3], L.acc[4]};
+    if (block_partials_store_last<5, 4>(dv5, a.part_out, gridDim.x, blockIdx.x, &arrive) &&
+        (threadIdx.x & (WAVE - 1)) == wave_sum_lane())
+      a.abar_blk[blockIdx.x] = f_ab + dv5[4];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const int k = a.k;
+      a.rs[RS::bbar(k)] = L.r_bb;
+      a.rs[RS::abar(k)] = L.r_abk;
+      a.rs[RS::papbar(k)] = L.papb;
+      a.rs[RS::rzbar_next(k)] = L.rzb;
+      a.rs[RS::carry(k)] = L.r_carry;
+    }
+  } else {
+    k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L);
+  }
+}
+// one tile per workgroup (every mode)
+template <int NP, int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_rev(RevArgs a) {
+  k2_sip_rev_kernel<NP, TX, TY, MODE, false>(a);
+}
+// the reverse PCG iteration on fewer workgroups than tiles; the tile loop
+// needs the compiler held to four waves per SIMD (two 8-wave workgroups per
+// CU), which it otherwise spends on registers
+template <int NP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY * NP * NP, 4) void k2_sip_rev_loop(RevArgs a) {
+  k2_sip_rev_kernel<NP, TX, TY, RV_PCG_ITER, true>(a);
+}
+
+// First reverse PCG step: a-bar_{n-1} = <x-bar, p_{n-1}> (the later adjoints
+// are 0), on the reverse PCG iteration's grid: tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ..., one partial sum per workgroup.
 template <int NP, int TX, int TY>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_rev_start(RevArgs a) {
   constexpr int NP2 = NP * NP, NT = TX * TY * NP2;
   __shared__ double red[4 * (NT / WAVE + 1)];
-  TileInfo<NP, TX, TY> t(a.g);
+  const int ntiles = tiles_of<TX, TY>(a.g);
   const int tid = threadIdx.x, s = tid / NP2, n = tid % NP2;
   const int lx = s % TX, ly = s / TX;
   const size_t N = (size_t)a.g.Nex * a.g.Ney * NP2;
   const int nit = a.n_iter;
   double dv[1] = {0.0};
-  if (lx < t.nvx && ly < t.nvy) {
-    const size_t gi = t.gidx(s, n);
-    dv[0] = a.xbar[gi] * a.P[(size_t)(nit - 1) * N + gi];
+  for (int tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+    const TileInfo<NP, TX, TY> t(a.g, tt, ntiles);
+    if (lx < t.nvx && ly < t.nvy) {
+      const size_t gi = t.gidx(s, n);
+      dv[0] += a.xbar[gi] * a.P[(size_t)(nit - 1) * N + gi];
+    }
   }
   double d4[4] = {0.0, dv[0], 0.0, 0.0};
   block_partials_store<4>(d4, red, a.part_out, gridDim.x, blockIdx.x);
@@ -744,11 +803,14 @@ struct LaunchRev {
     hipLaunchKernelGGL((k2_hyp_rev<NP, TX, TY>), grid(a.g), block(), 0, st, a);
   }
   static void rev(int mode, const RevArgs& a, hipStream_t st) {
-    dim3 G = grid(a.g), Bk = block();
+    dim3 G = grid(a.g), Bk = block(), GP(a.ngroups);
     switch (mode) {
       case RV_VJP: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_VJP>), G, Bk, 0, st, a); break;
-      case RV_PCG_START: hipLaunchKernelGGL((k2_pcg_rev_start<NP, TX, TY>), G, Bk, 0, st, a); break;
-      case RV_PCG_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_ITER>), G, Bk, 0, st, a); break;
+      case RV_PCG_START: hipLaunchKernelGGL((k2_pcg_rev_start<NP, TX, TY>), GP, Bk, 0, st, a); break;
+      case RV_PCG_ITER:
+        if ((int)GP.x < (int)G.x) hipLaunchKernelGGL((k2_sip_rev_loop<NP, TX, TY>), GP, Bk, 0, st, a);
+        else hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_ITER>), G, Bk, 0, st, a);
+        break;
       case RV_PCG_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_PCG_FIN>), G, Bk, 0, st, a); break;
       case RV_JAC_ITER: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_JAC_ITER>), G, Bk, 0, st, a); break;
       case RV_JAC_FIN: hipLaunchKernelGGL((k2_sip_rev<NP, TX, TY, RV_JAC_FIN>), G, Bk, 0, st, a); break;
@@ -789,7 +851,13 @@ struct LaunchRev {
   }
 
 void launch2_hyp_rev(const Ctx& c, const HypRevArgs& a) { IGR_HREV_DISPATCH(c.np, c.htile_x, c.htile_y, hyp(a, c.stream)); }
-void launch2_rev(const Ctx& c, int mode, const RevArgs& a) {
+// workgroups of the reverse PCG iteration (and of its start kernel): at
+// most c.rgroups (0: one per reverse tile)
+int grid2_rpgroups(const Ctx& c);
+void launch2_rev(const Ctx& c, int mode, const RevArgs& a0) {
+  RevArgs a = a0;
+  a.ngroups = grid2_rpgroups(c);
+  a.np_in = a.ngroups;   // the start kernel and every iteration write that many partial sums
   IGR_REV_DISPATCH(c.np, c.rtile_x, c.rtile_y, rev(mode, a, c.stream));
 }
 void launch2_obj(const Ctx& c, const ObjArgs& a, bool grad) {

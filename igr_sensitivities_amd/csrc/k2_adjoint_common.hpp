@@ -73,6 +73,8 @@ struct RevArgs {
   unsigned* counter = nullptr;
   const double* part_in = nullptr;
   double* part_out = nullptr;
+  int ngroups = 0;   // workgroups of the reverse PCG start / iteration kernels (set by the launcher)
+  int np_in = 0;     // partial sums the previous reverse PCG kernel wrote (set by the launcher)
 };
 
 // reverse-scalar layout per solve: for iteration k

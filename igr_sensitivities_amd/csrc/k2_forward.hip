@@ -858,6 +858,12 @@ int grid2_rblocks(const Ctx& c) {
   else if (c.np == 4) { tx = c.rtile_x; ty = c.rtile_y; }
   return ((c.Nex + tx - 1) / tx) * ((c.Ney + ty - 1) / ty);
 }
+// workgroups of the reverse PCG iteration and its start kernel: at most
+// c.rgroups (0: one per reverse tile), each running every n-th tile
+int grid2_rpgroups(const Ctx& c) {
+  const int nt = grid2_rblocks(c);
+  return c.rgroups > 0 ? std::min(nt, c.rgroups) : nt;
+}
 // the partial sums are the last PCG_B's (n_iter > 0) or the INIT kernel's
 void launch_pcg_final(const Ctx& c, const double* part, double* sc, int nit) {
   const int nb = nit > 0 ? grid2_bblocks(c) : grid2_blocks(c);

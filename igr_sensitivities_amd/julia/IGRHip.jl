@@ -12,7 +12,7 @@ const SOLVERS = Dict(:pcg => 0, :jacobi => 1, :chebyshev => 2)
 const OBJECTIVES = Dict(:l2_density => 0, :l2_pressure => 1, :kinetic_energy => 2,
                         :windowed_kinetic_energy => 3, :weighted_momentum => 4,
                         :windowed_density => 5, :disc_kinetic_energy => 6)
-const OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE, OPT_RTILE, OPT_GROUPS, OPT_WAVE1D, OPT_KEEP = 0, 1, 2, 3, 4, 5, 6, 7
+const OPT_GRAPHS, OPT_TILE, OPT_HTILE, OPT_AUTOTUNE, OPT_RTILE, OPT_GROUPS, OPT_WAVE1D, OPT_KEEP, OPT_RGROUPS = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
 mutable struct Context
     h::Ptr{Cvoid}

@@ -80,12 +80,17 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
  *                   the forward sweep; with fewer checkpoint slots a segment
  *                   of up to that many steps + 1 is reversed after one
  *                   storing pass, without recomputation.  Gradients are
- *                   bit-identical for every value.
+ *                   bit-identical for every value;
+ *  IGR_OPT_RGROUPS  workgroups of the 2D gradient's reverse PCG iteration:
+ *                   0 = one per reverse tile, n > 0 = at most n, each
+ *                   running every n-th tile and storing one partial sum per
+ *                   dot product (the default comes from the size table;
+ *                   results change only in the dot products' order).
  * The defaults come from a size table measured on MI355X.  The shapes change
  * results only through the summation order of the solvers' dot products. */
 enum igr_option {
   IGR_OPT_GRAPHS = 0, IGR_OPT_TILE = 1, IGR_OPT_HTILE = 2, IGR_OPT_AUTOTUNE = 3, IGR_OPT_RTILE = 4,
-  IGR_OPT_GROUPS = 5, IGR_OPT_WAVE1D = 6, IGR_OPT_KEEP = 7
+  IGR_OPT_GROUPS = 5, IGR_OPT_WAVE1D = 6, IGR_OPT_KEEP = 7, IGR_OPT_RGROUPS = 8
 };
 int igr_set_option(igr_ctx* ctx, int option, int value);
 int igr_get_option(const igr_ctx* ctx, int option, int* value);

@@ -192,6 +192,57 @@ def test_gradient_reverse_tiles_agree(solver):
         close(r[3], res[0][3], 1e-10)
 
 
+def test_gradient_reverse_groups(orc):
+    """The reverse PCG iteration on fewer workgroups than reverse tiles
+    (IGR_OPT_RGROUPS), each looping over every n-th tile and storing one
+    partial sum per dot product: 210 tiles of 2x2 on 64, 13 and 1 workgroups
+    (uneven shares, one workgroup running them all) agree with one workgroup
+    per tile to rounding, and the same group count gives the same bits on a
+    second call; on 12x10 elements (30 tiles of 2x2 on 7 and 4 workgroups)
+    the gradient satisfies the dot-product identity with the oracle's exact
+    JVP."""
+    nx, ny, ns = 30, 28, 3
+    q0 = configs.smooth_random2(3, nx, ny, seed=131, amp=0.3)
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, 1.4))
+    prm = (0.55, 0.5, 0.15, 0.0)
+    res = {}
+    for rg in (0, 64, 13, 1):
+        c = ctx2(3, nx, ny)
+        c.set_option(igr.core.OPT_RTILE, 202)
+        c.set_option(igr.core.OPT_RGROUPS, rg)
+        assert c.get_option(igr.core.OPT_RGROUPS) == rg
+        c.set_state(q0)
+        res[rg] = c.gradient(3, ns, dt, 1.4, 3e-3, 12, "pcg", prm=prm)
+        if rg == 13:
+            c.set_state(q0)
+            again = c.gradient(3, ns, dt, 1.4, 3e-3, 12, "pcg", prm=prm)
+            np.testing.assert_array_equal(again[1], res[rg][1])
+        c.close()
+    for rg in (64, 13, 1):
+        assert res[rg][0] == res[0][0]
+        np.testing.assert_allclose(res[rg][1], res[0][1], rtol=0, atol=1e-11 * np.abs(res[0][1]).max())
+        close(res[rg][2], res[0][2], 1e-10)
+        close(res[rg][3], res[0][3], 1e-10)
+    nx, ny, ns = 12, 10, 2
+    q0 = configs.smooth_random2(3, nx, ny, seed=120, amp=0.3)
+    gam, a = 1.4, 4e-3
+    dt = 0.3 * (1 / nx) / (7 * configs.max_wavespeed2(q0, gam))
+    p4 = (0.5, 0.5, 0.2, 0.0)
+    u = np.random.default_rng(121).standard_normal(q0.shape)
+    Jr, dJ, _, _ = orc.Grid2D(3, nx, ny).jvp_objective_run(gam, 0.4, a, 0.3 * a, 15, orc.PCG, ns, dt, 3, q0, u, p4)
+    for rg in (7, 4):
+        c = ctx2(3, nx, ny)
+        c.set_option(igr.core.OPT_RTILE, 202)
+        c.set_option(igr.core.OPT_RGROUPS, rg)
+        c.set_state(q0)
+        J, qb, gb, ab = c.gradient(3, ns, dt, gam, a, 15, "pcg", prm=p4)
+        c.close()
+        close(J, Jr, 1e-12)
+        close(dJ, np.vdot(qb, u) + 0.4 * gb + 0.3 * a * ab, 1e-10)
+    with pytest.raises(ValueError):
+        ctx2(3, 8, 8).set_option(igr.core.OPT_RGROUPS, -1)
+
+
 @pytest.mark.parametrize("solver", ["pcg", "chebyshev"])
 def test_checkpoint_schedules_bitwise_equal(solver):
     nx, ny, ns = 6, 6, 9
