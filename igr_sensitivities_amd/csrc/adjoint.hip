@@ -352,14 +352,14 @@ static void solve2_store_pcg(Ctx& c, const double* qs, double* x, double alpha, 
     k.sc = st.sc; k.n_iter = n_iter; k.it = it; k.first = (it == 0);
     k.r = st.R + (size_t)it * N; k.invM = st.invM; k.invR = st.invR;
     k.pold = st.P + (size_t)(it > 0 ? it - 1 : 0) * N; k.pnew = st.P + (size_t)it * N; k.y = st.AP + (size_t)it * N;
-    k.part_in = R0; k.part_out = R1;
+    k.xout = x; k.part_in = R0; k.part_out = R1;
     launch2_sip(c, M_PCG_A, k);
     SipArgs m = sip_base(c, alpha);
     m.sc = st.sc; m.n_iter = n_iter; m.it = it;
     m.xout = x; m.pold = st.P + (size_t)it * N; m.y = st.AP + (size_t)it * N;
     m.r = st.R + (size_t)it * N; m.rout = st.R + (size_t)(it + 1) * N; m.invM = st.invM;
     m.part_in = R1; m.part_out = R0;
-    launch2_sip(c, M_PCG_B, m);
+    launch2_sip(c, it + 1 < n_iter ? M_PCG_B : M_PCG_BX, m);
   }
   launch_pcg_final(c, R0, st.sc, n_iter);
 }

@@ -82,7 +82,8 @@ struct Program {
     for (const L& l : ls) {
       switch (l.kind) {
         case R_SIP:
-          if (l.mode == M_PCG_B) launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, bwork, bnblocks, st);
+          if (l.mode == M_PCG_B || l.mode == M_PCG_BX)
+            launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, bwork, bnblocks, st);
           else launch2_sip_batch(np, tx, ty, l.mode, sip + l.off, work, nblocks, st);
           break;
         case R_HYP: launch2_hyp_batch(np, htx, hty, hyp + l.off, hwork, hnblocks, st); break;

@@ -129,14 +129,14 @@ void solve2_fields(Ctx& c, const double* mx, const double* my, const double* rho
       SipArgs k = sip_base(c, alpha);
       k.sc = sc; k.n_iter = n_iter; k.it = it; k.first = (it == 0);
       k.r = c.r; k.invM = c.invM; k.invR = c.invR;
-      k.pold = P[(it + 1) % 2]; k.pnew = P[it % 2]; k.y = c.Ap;
+      k.pold = P[(it + 1) % 2]; k.pnew = P[it % 2]; k.y = c.Ap; k.xout = x;
       k.part_in = R0; k.part_out = R1;
       launch2_sip(c, M_PCG_A, k);
       SipArgs m = sip_base(c, alpha);
       m.sc = sc; m.n_iter = n_iter; m.it = it;
       m.xout = x; m.pold = P[it % 2]; m.y = c.Ap; m.rout = c.r; m.invM = c.invM;
       m.part_in = R1; m.part_out = R0;
-      launch2_sip(c, M_PCG_B, m);
+      launch2_sip(c, it + 1 < n_iter ? M_PCG_B : M_PCG_BX, m);
     }
     launch_pcg_final(c, R0, sc, n_iter);
   } else if (solver == IGR_SOLVER_JACOBI) {
@@ -739,7 +739,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
       SipArgs k = sip_base(*c, alpha);
       k.sc = c->scal; k.n_iter = nit; k.it = it; k.first = (it == 0);
       k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = pbuf[(it + 1) % 2]; k.pnew = pbuf[it % 2];
-      k.y = c->Ap; k.part_in = R0; k.part_out = R1;
+      k.y = c->Ap; k.xout = c->xA; k.part_in = R0; k.part_out = R1;
       if (it == 1 && kernel == 3) {
         rc = batch_ms([&]() { launch2_sip(*c, M_PCG_A, k); });
         if (rc) return rc;
@@ -756,7 +756,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
         if (rc) return rc;
         return end_call(*c);
       }
-      launch2_sip(*c, M_PCG_B, m);
+      launch2_sip(*c, it + 1 < nit ? M_PCG_B : M_PCG_BX, m);
     }
   }
   std::vector<hipEvent_t> ev(2 * (size_t)reps);
@@ -765,7 +765,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
     SipArgs k = sip_base(*c, alpha);
     k.sc = c->scal; k.n_iter = reps; k.it = it; k.first = (it == 0);
     k.r = c->r; k.invM = c->invM; k.invR = c->invR; k.pold = pbuf[(it + 1) % 2]; k.pnew = pbuf[it % 2]; k.y = c->Ap;
-    k.part_in = R0; k.part_out = R1;
+    k.xout = c->xA; k.part_in = R0; k.part_out = R1;
     SipArgs m = sip_base(*c, alpha);
     m.sc = c->scal; m.n_iter = reps; m.it = it;
     m.xout = c->xA; m.pold = pbuf[it % 2]; m.y = c->Ap; m.rout = c->r; m.r = nullptr; m.invM = c->invM;
@@ -773,7 +773,7 @@ int igr_time_kernel(igr_ctx* c, int kernel, double gamma, double alpha, int reps
     if (kernel == 0) set_kernel_timer(ev[2 * it], ev[2 * it + 1]);
     launch2_sip(*c, M_PCG_A, k);
     if (kernel == 1) set_kernel_timer(ev[2 * it], ev[2 * it + 1]);
-    launch2_sip(*c, M_PCG_B, m);
+    launch2_sip(*c, it + 1 < reps ? M_PCG_B : M_PCG_BX, m);
   }
   HIPCHK(*c, hipEventSynchronize(ev.back()));
   double tot = 0.0;

@@ -13,7 +13,8 @@ struct Geo2 {
 
 enum SipMode {
   M_APPLY = 0, M_INIT_PCG = 1, M_INIT_CHEB = 2, M_INIT_JAC = 3,
-  M_PCG_A = 4, M_JACOBI = 5, M_CHEB = 6, M_PCG_B = 7
+  M_PCG_A = 4, M_JACOBI = 5, M_CHEB = 6, M_PCG_B = 7,
+  M_PCG_BX = 8   // PCG_B of the last iteration, which also updates x (the others leave it to the next PCG_A)
 };
 
 struct SipArgs {

@@ -96,6 +96,10 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   // iteration; r stands in for them there, so every load has a valid address.
   const double* pold = a.first ? a.r : a.pold;
   const double* dold = a.first ? a.r : a.d;
+  // PCG_A after the first iteration: x += a_{it-1} p_{it-1} at the own node
+  // (the previous PCG_B left x alone), with x read beside the tile loads
+  double x_own = 0.0;
+  if constexpr (MODE == M_PCG_A) x_own = (a.first ? a.r : a.xout)[gme];
   double L0[NL], L1[NL], L2[NL], L3[NL], L4[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
@@ -320,6 +324,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     if (valid) {
       a.pnew[gme] = vown;
       a.y[gme] = y;
+      if (!a.first) a.xout[gme] = fma(a.sc[2 * a.n_iter + 1 + a.it - 1], L2[0], x_own);
     }
     if (blk == 0 && tid == 0) {
       a.sc[a.it] = sc_tot;
@@ -354,8 +359,11 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_sip_batch(const SipArgs*
 // x += a p ; r -= a Ap ; z = r/M ; <r, z>   (pcg_fixed!, elliptic.jl:339-343)
 // Workgroup blk of nblk processes tiles blk, blk + nblk, ...; the next
 // tile's loads are issued before this tile's stores.  The first tile's loads
-// come before the grid total of the previous kernel's partial sums.
-template <int NP, int TX, int TY, int GU = 8>
+// come before the grid total of the previous kernel's partial sums.  Only the
+// last iteration's update (XU) touches x: x += a_k p_k of the earlier
+// iterations is done by the next iteration's PCG_A, which holds p_k at its
+// own nodes anyway (same arithmetic; this kernel then streams 4 fields, not 7).
+template <int NP, int TX, int TY, bool XU, int GU = 8>
 __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP;
   __shared__ unsigned arrive;   // waves done with the partial sums (block_partials_store_last)
@@ -372,12 +380,13 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
   const double* rin = a.r ? a.r : a.rout;
   bool valid;
   size_t gi;
-  double x, p, r, ap, im;
+  double x = 0.0, p = 0.0, r, ap, im;
   auto issue = [&](int tt) {
     const TileInfo<NP, TX, TY> t(a.g, tt, ntiles);
     valid = lx < t.nvx && ly < t.nvy;
     gi = valid ? t.gidx(s, n) : 0;
-    x = a.xout[gi]; p = a.pold[gi]; r = rin[gi]; ap = a.y[gi]; im = a.invM[gi];
+    if constexpr (XU) { x = a.xout[gi]; p = a.pold[gi]; }
+    r = rin[gi]; ap = a.y[gi]; im = a.invM[gi];
   };
   issue(blk);
   const double rz_it = sc[it];
@@ -393,7 +402,7 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
     acc += v ? rn * (rn * im) : 0.0;
     if (tt + nblk < ntiles) issue(tt + nblk);      // uniform branch: the workgroup's next tile
     if (v) {
-      a.xout[g0] = xn;
+      if constexpr (XU) a.xout[g0] = xn;
       a.rout[g0] = rn;
     }
   }
@@ -401,16 +410,16 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
   block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
   if (blk == 0 && tid == 0) { sc[nit + 1 + it] = tot[0]; sc[2 * nit + 1 + it] = ak; }
 }
-template <int NP, int TX, int TY>
+template <int NP, int TX, int TY, bool XU>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b(SipArgs a) {
-  k2_pcg_b_body<NP, TX, TY>(a, blockIdx.x, gridDim.x);
+  k2_pcg_b_body<NP, TX, TY, XU>(a, blockIdx.x, gridDim.x);
 }
-template <int NP, int TX, int TY>
+template <int NP, int TX, int TY, bool XU>
 __global__ __launch_bounds__(TX * TY * NP * NP) void k2_pcg_b_batch(const SipArgs* __restrict__ tab,
                                                                     const int2* __restrict__ work) {
   const int2 w = work[blockIdx.x];
   const SipArgs& a = tab[w.x];
-  k2_pcg_b_body<NP, TX, TY, 1>(a, w.y, a.ngroups);
+  k2_pcg_b_body<NP, TX, TY, XU, 1>(a, w.y, a.ngroups);
 }
 
 
@@ -735,7 +744,8 @@ struct Launch2 {
       case M_PCG_A: launch_k(k2_sip<NP, TX, TY, M_PCG_A>, G, Bk, st, a); break;
       case M_JACOBI: launch_k(k2_sip<NP, TX, TY, M_JACOBI>, G, Bk, st, a); break;
       case M_CHEB: launch_k(k2_sip<NP, TX, TY, M_CHEB>, G, Bk, st, a); break;
-      case M_PCG_B: launch_k(k2_pcg_b<NP, TX, TY>, G, Bk, st, a); break;
+      case M_PCG_B: launch_k(k2_pcg_b<NP, TX, TY, false>, G, Bk, st, a); break;
+      case M_PCG_BX: launch_k(k2_pcg_b<NP, TX, TY, true>, G, Bk, st, a); break;
     }
   }
   static void hyp(const HypArgs& a, hipStream_t st) {
@@ -751,7 +761,8 @@ struct Launch2 {
       case M_PCG_A: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_PCG_A>), G, Bk, 0, st, tab, work); break;
       case M_JACOBI: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_JACOBI>), G, Bk, 0, st, tab, work); break;
       case M_CHEB: hipLaunchKernelGGL((k2_sip_batch<NP, TX, TY, M_CHEB>), G, Bk, 0, st, tab, work); break;
-      case M_PCG_B: hipLaunchKernelGGL((k2_pcg_b_batch<NP, TX, TY>), G, Bk, 0, st, tab, work); break;
+      case M_PCG_B: hipLaunchKernelGGL((k2_pcg_b_batch<NP, TX, TY, false>), G, Bk, 0, st, tab, work); break;
+      case M_PCG_BX: hipLaunchKernelGGL((k2_pcg_b_batch<NP, TX, TY, true>), G, Bk, 0, st, tab, work); break;
     }
   }
   static void hyp_batch(const HypArgs* tab, const int2* work, int nb, hipStream_t st) {
@@ -818,8 +829,9 @@ void launch_copy_batch(const CopyArgs* tab, int n, size_t maxn, hipStream_t st) 
 void launch2_sip(const Ctx& c, int mode, const SipArgs& a0) {
   SipArgs a = a0;
   const int nt = grid2_blocks(c), nbb = grid2_bblocks(c);
-  a.ngroups = mode == M_PCG_B ? nbb : nt;
-  a.np_in = mode == M_PCG_B ? nt : (mode == M_PCG_A ? (a.first ? nt : nbb) : 0);
+  const bool pb = mode == M_PCG_B || mode == M_PCG_BX;
+  a.ngroups = pb ? nbb : nt;
+  a.np_in = pb ? nt : (mode == M_PCG_A ? (a.first ? nt : nbb) : 0);
   if (g_rec) { g_rec->recs.push_back({R_SIP, mode, g_rec->sip.size()}); g_rec->sip.push_back(a); return; }
   IGR_SIP_TILE_DISPATCH(Launch2, c.np, c.tile_x, c.tile_y, sip(mode, a, c.stream));
 }
