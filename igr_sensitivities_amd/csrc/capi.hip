@@ -379,9 +379,14 @@ int igr_create(igr_ctx** out, int dim, int p, int nex, int ney, double lx, doubl
   if (ok && dev_malloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->counter = (unsigned*)pp; } else ok = false;
   if (ok && dev_malloc(&pp, 64) == hipSuccess) { c->owned.push_back(pp); c->flag = (int*)pp; } else ok = false;
   if (ok) ok = ensure_scal(*c, 64) == IGR_OK;
-  if (ok) ok = hipMemset(c->counter, 0, 64) == hipSuccess && hipMemset(c->flag, 0, 64) == hipSuccess &&
-               hipMemset(c->S, 0, N * sizeof(double)) == hipSuccess &&
-               hipMemset(c->q, 0, nvN * sizeof(double)) == hipSuccess;
+  // on the context's own stream and waited for: the stream does not wait for
+  // the null stream, so a null-stream memset could still be pending (behind
+  // other work queued there) when the first kernel reads Sigma or the flags
+  if (ok) ok = hipMemsetAsync(c->counter, 0, 64, c->stream) == hipSuccess &&
+               hipMemsetAsync(c->flag, 0, 64, c->stream) == hipSuccess &&
+               hipMemsetAsync(c->S, 0, N * sizeof(double), c->stream) == hipSuccess &&
+               hipMemsetAsync(c->q, 0, nvN * sizeof(double), c->stream) == hipSuccess &&
+               hipStreamSynchronize(c->stream) == hipSuccess;
   if (!ok) { igr_destroy(c); return IGR_ENOMEM; }
   *out = c;
   return IGR_OK;
