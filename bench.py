@@ -29,6 +29,24 @@ HBM_PEAK_GBS = 8000.0
 GRAPHS = True      # replay captured hipGraphs (off: direct launches, for profilers)
 
 
+def step_roofline(n_nodes, n_iter, seconds_per_step):
+    """Whole-step HBM fraction of a 2D PCG forward step: algorithmic bytes
+    per SSP-RK3 step (each kernel's distinct fields once, 8 B per node) over
+    the step time.  Per stage: hyperbolic + RK3 9 / 13 / 13 fields, PCG init
+    7, PCG iterations 12 n + 1 (the search-direction kernel 6 fields in the
+    first iteration and 8 after, the r update 4 and 7 in the last); the
+    13-fields-per-iteration count is the two-pass schedule that leaves x to
+    the r update."""
+    def frac(fields):
+        b = 8.0 * n_nodes * fields
+        return {"alg_bytes_per_step": b, "gbs": b / seconds_per_step / 1e9,
+                "frac": b / seconds_per_step / 1e9 / HBM_PEAK_GBS}
+    out = frac(35 + 3 * 7 + 3 * (12 * n_iter + 1))
+    out["schedule"] = "as run: 12 n + 1 fields per solve of n PCG iterations"
+    out["two_pass_13_fields"] = frac(35 + 3 * 7 + 3 * 13 * n_iter)
+    return out
+
+
 def new_ctx(igr, mesh):
     c = igr.Context(mesh)
     if not GRAPHS:
@@ -128,6 +146,7 @@ def c5_leg(igr, configs, budget_slots=40):
         ctx.run_steps(ns, dt, gam, al, n_iter, "pcg")
         out["forward_s"] = time.perf_counter() - t0
         out["dof_stage_per_s"] = 4 * 16 * N1 * N1 * 3 * ns / out["forward_s"]
+        out["step_roofline"] = step_roofline(16 * N1 * N1, n_iter, out["forward_s"] / ns)
         # warm-up: the gradient's workspace (checkpoints, kept iterates) is
         # allocated once per context; the timed calls below reuse it
         ctx.set_state(cfg["q"])
@@ -461,6 +480,7 @@ def main():
             "hyp_rk3": {"us": ms_h * 1e3, "alg_bytes": 13 * 8 * n_nodes,
                         "frac": 13 * 8 * n_nodes / (ms_h * 1e-3) / 1e9 / HBM_PEAK_GBS},
         },
+        "step_roofline": step_roofline(n_nodes, n_iter, elapsed / args.steps),
         "roofline_256x256": big,
         "gradient_over_forward": t_grad / elapsed,
         "gradient": {"objective": "int rho(T) exp(-|x-c|^2/0.1^2)", "steps": args.steps, "seconds": t_grad,

@@ -239,31 +239,22 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
       gis[l] = gi;
     }
     // r-bar and p-bar of the next iteration do not exist in a first reverse
-    // iteration, nor P_{k-1}, AP_{k-1}, R_k at k = 0: valid stand-ins are read
-    // and the values dropped
+    // iteration: valid stand-ins are read and the values dropped
     const double* rcin = a.first ? a.invR : a.rbar_in;
     const double* pbin = a.first ? a.invR : a.pbar_in;
     const double* Rk1 = a.R + (size_t)(k + 1) * N;
     const double* Pk = a.P + (size_t)k * N;
-    const double* APk = a.AP + (size_t)k * N;
-    const double* Pm = k >= 1 ? a.P + (size_t)(k - 1) * N : a.P;
-    const double* APm = k >= 1 ? a.AP + (size_t)(k - 1) * N : a.AP;
-    const double* Rk = a.R + (size_t)k * N;
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       const size_t gi = gis[l];
       l_ir[l] = a.invR[gi]; l_iM[l] = a.invM[gi]; l_rc[l] = rcin[gi]; l_pb[l] = pbin[gi];
       l_R[l] = Rk1[gi]; l_p[l] = Pk[gi];
     }
-    f_irb = a.irbar[gme]; f_imb = a.imbar[gme]; f_xb = a.xbar[gme]; f_apk = APk[gme];
-    f_pm = Pm[gme]; f_apm = APm[gme]; f_Rk = Rk[gme];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       if (!oks[l]) { l_ir[l] = 0.0; l_iM[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0; }
       if (!oks[l] || a.first) { l_rc[l] = 0.0; l_pb[l] = 0.0; }
     }
-    if (!valid) { f_irb = 0.0; f_imb = 0.0; f_xb = 0.0; f_apk = 0.0; }
-    if (!valid || k < 1) { f_pm = 0.0; f_apm = 0.0; f_Rk = 0.0; }
   }
   if constexpr (MODE == RV_PCG_ITER) {
     ak = L.ak; bk = L.bk; rzb = L.rzb; papb = L.papb;
@@ -346,6 +337,20 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
   }
   stage_basis<NP>(B, sD, sW);
   __syncthreads();
+  if constexpr (MODE == RV_PCG_ITER) {
+    // own-node inputs used only with the outputs: requested here, out of the
+    // entry burst, so that they arrive under the derivative and face phases
+    // (P_{k-1}, AP_{k-1}, R_k do not exist at k = 0: valid stand-ins are read
+    // and the values dropped)
+    const double* APk = a.AP + (size_t)k * N;
+    const double* Pm = k >= 1 ? a.P + (size_t)(k - 1) * N : a.P;
+    const double* APm = k >= 1 ? a.AP + (size_t)(k - 1) * N : a.AP;
+    const double* Rk = a.R + (size_t)k * N;
+    f_irb = a.irbar[gme]; f_imb = a.imbar[gme]; f_xb = a.xbar[gme]; f_apk = APk[gme];
+    f_pm = Pm[gme]; f_apm = APm[gme]; f_Rk = Rk[gme];
+    if (!valid) { f_irb = 0.0; f_imb = 0.0; f_xb = 0.0; f_apk = 0.0; }
+    if (!valid || k < 1) { f_pm = 0.0; f_apm = 0.0; f_Rk = 0.0; }
+  }
 
   const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
   const double wown = ws[s * NP2 + n], vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];

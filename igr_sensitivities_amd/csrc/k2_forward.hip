@@ -96,10 +96,6 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   // iteration; r stands in for them there, so every load has a valid address.
   const double* pold = a.first ? a.r : a.pold;
   const double* dold = a.first ? a.r : a.d;
-  // PCG_A after the first iteration: x += a_{it-1} p_{it-1} at the own node
-  // (the previous PCG_B left x alone), with x read beside the tile loads
-  double x_own = 0.0;
-  if constexpr (MODE == M_PCG_A) x_own = (a.first ? a.r : a.xout)[gme];
   double L0[NL], L1[NL], L2[NL], L3[NL], L4[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
@@ -161,6 +157,17 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   }
   stage_basis<NP>(B, sD, sW);
   __syncthreads();
+  // PCG_A after the first iteration: x += a_{it-1} p_{it-1} at the own node
+  // (the previous PCG_B left x alone); x is needed only with the outputs, so
+  // it is requested here, out of the entry burst, and arrives under the
+  // derivative and face phases
+  // (so is a_{it-1}: read where it is used, the compiler made it a per-lane
+  // load after the last phase, whose latency the kernel then waited out)
+  double x_own = 0.0, a_prev = 0.0;
+  if constexpr (MODE == M_PCG_A) {
+    x_own = (a.first ? a.r : a.xout)[gme];
+    a_prev = a.first ? 0.0 : a.sc[2 * a.n_iter + 1 + a.it - 1];
+  }
 
   const double al = a.alpha;
   const double vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
@@ -324,7 +331,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     if (valid) {
       a.pnew[gme] = vown;
       a.y[gme] = y;
-      if (!a.first) a.xout[gme] = fma(a.sc[2 * a.n_iter + 1 + a.it - 1], L2[0], x_own);
+      if (!a.first) a.xout[gme] = fma(a_prev, L2[0], x_own);
     }
     if (blk == 0 && tid == 0) {
       a.sc[a.it] = sc_tot;
@@ -486,7 +493,10 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
   // Own node and (some threads) one halo face node: both sets of loads are
   // issued, with no branch between them, before either is written to LDS
   // (an invalid thread or empty halo slot reads node 0 and takes the
-  // default state instead); q^n of stages 2 and 3 comes with them.
+  // default state instead); q^n of stages 2 and 3 comes with them (issued
+  // after the staging barrier instead, where it enters, it took 256^2 from
+  // 23.8 to 26.0 us: with four rounds of workgroups the entry loads already
+  // overlap other workgroups' phases).
   const bool hx = tid < NHX, hy = tid >= HY0 && tid < HY0 + NHY;
   const int hidx = hx ? tid : tid - HY0 + NHX;
   size_t hgi = 0;
