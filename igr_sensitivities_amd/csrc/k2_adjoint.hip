@@ -164,11 +164,10 @@ __global__ __launch_bounds__(TX * TY * NP * NP) void k2_hyp_rev(HypRevArgs a) {
     for (int v = 0; v < 4; ++v) {
       const size_t gi = v * N + gme;
       const double val = a.c_prev * qbo[v] + qb[v];
-      if (a.accumulate) a.qbar_stage[gi] = pf_stage[v] + val;
-      else a.qbar_stage[gi] = val;
-      if (a.qbar_n) a.qbar_n[gi] = pf_n[v] + a.c_n * qbo[v];
+      store_wt(&a.qbar_stage[gi], a.accumulate ? pf_stage[v] + val : val);
+      if (a.qbar_n) store_wt(&a.qbar_n[gi], pf_n[v] + a.c_n * qbo[v]);
     }
-    a.sbar_out[gme] = pf_sb + sb;
+    store_wt(&a.sbar_out[gme], pf_sb + sb);
   }
   double rv[1] = {gb};
   block_sum<1>(rv, red);
@@ -515,10 +514,10 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
     for (int q = 0; q < 4; ++q) L.acc[q] += dv[q];
     L.acc[4] += ab;
     if (valid) {
-      a.irbar[gme] = f_irb + irb;
-      a.imbar[gme] = f_imb + o_zb * o_R;
-      a.rbar_out[gme] = o_rb;
-      a.pbar_out[gme] = pbk;
+      store_wt(&a.irbar[gme], f_irb + irb);
+      store_wt(&a.imbar[gme], f_imb + o_zb * o_R);
+      store_wt(&a.rbar_out[gme], o_rb);
+      store_wt(&a.pbar_out[gme], pbk);
     }
   } else if constexpr (MODE == RV_JAC_ITER) {
     if (valid) {

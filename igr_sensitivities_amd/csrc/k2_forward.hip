@@ -280,7 +280,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   }
 
   if constexpr (MODE == M_APPLY) {
-    if (valid) a.y[gme] = y;
+    if (valid) store_wt(&a.y[gme], y);
   } else if constexpr (INIT) {
     double red_v[1] = {0.0};
     double o_iM = 0.0, o_r = 0.0;
@@ -314,24 +314,24 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     }
     if constexpr (MODE == M_INIT_PCG) block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
     if (valid) {
-      a.invR_out[gme] = ir;
-      a.invM_out[gme] = o_iM;
+      store_wt(&a.invR_out[gme], ir);
+      store_wt(&a.invM_out[gme], o_iM);
       if constexpr (MODE == M_INIT_PCG) {
-        a.rout[gme] = o_r;
+        store_wt(&a.rout[gme], o_r);
       } else if constexpr (MODE == M_INIT_CHEB) {
-        a.rout[gme] = o_r;
-        a.b_out[gme] = bval;
+        store_wt(&a.rout[gme], o_r);
+        store_wt(&a.b_out[gme], bval);
       } else {
-        a.b_out[gme] = bval;
+        store_wt(&a.b_out[gme], bval);
       }
     }
   } else if constexpr (MODE == M_PCG_A) {
     double red_v[1] = {valid ? vown * y : 0.0};
     block_partials_store_last<1>(red_v, a.part_out, nblk, blk, &arrive);
     if (valid) {
-      a.pnew[gme] = vown;
-      a.y[gme] = y;
-      if (!a.first) a.xout[gme] = fma(a_prev, L2[0], x_own);
+      store_wt(&a.pnew[gme], vown);
+      store_wt(&a.y[gme], y);
+      if (!a.first) store_wt(&a.xout[gme], fma(a_prev, L2[0], x_own));
     }
     if (blk == 0 && tid == 0) {
       a.sc[a.it] = sc_tot;
@@ -339,14 +339,14 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     }
   } else if constexpr (MODE == M_JACOBI) {
     if (valid) {
-      if (a.y) a.y[gme] = y;     // A x_k, kept by the adjoint's recompute
-      a.xout[gme] = fma(a.omega * (a.b[gme] - y), a.invM[gme], vown);
+      if (a.y) store_wt(&a.y[gme], y);     // A x_k, kept by the adjoint's recompute
+      store_wt(&a.xout[gme], fma(a.omega * (a.b[gme] - y), a.invM[gme], vown));
     }
   } else if constexpr (MODE == M_CHEB) {
     if (valid) {
-      a.xout[gme] = vown;
-      a.dout[gme] = own_d;
-      a.rout[gme] = a.b[gme] - y;
+      store_wt(&a.xout[gme], vown);
+      store_wt(&a.dout[gme], own_d);
+      store_wt(&a.rout[gme], a.b[gme] - y);
     }
   }
 }
@@ -409,8 +409,8 @@ __device__ __forceinline__ void k2_pcg_b_body(const SipArgs& a, const int blk, c
     acc += v ? rn * (rn * im) : 0.0;
     if (tt + nblk < ntiles) issue(tt + nblk);      // uniform branch: the workgroup's next tile
     if (v) {
-      if constexpr (XU) a.xout[g0] = xn;
-      a.rout[g0] = rn;
+      if constexpr (XU) store_wt(&a.xout[g0], xn);
+      store_wt(&a.rout[g0], rn);
     }
   }
   double red_v[1] = {acc};
@@ -676,7 +676,7 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
     else if (a.mode == 1) o = qo[v] + a.cdt * acc[v];
     else if (a.mode == 2) o = 3.0 / 4.0 * qn[v] + 1.0 / 4.0 * qo[v] + a.cdt * acc[v];
     else o = 1.0 / 3.0 * qn[v] + 2.0 / 3.0 * qo[v] + a.cdt * acc[v];
-    a.out[gi] = o;
+    store_wt(&a.out[gi], o);
   }
 }
 template <int NP, int TX, int TY>
