@@ -51,11 +51,21 @@ template <int NP, int TX, int TY, int MODE, int GU = 8>
 __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, const int nblk) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool INIT = (MODE == M_INIT_PCG || MODE == M_INIT_CHEB || MODE == M_INIT_JAC);
-  __shared__ double vs[NS * NP2];
-  __shared__ double irs[NS * NP2];
+  // LDS bank layout.  A 64-bit LDS read is served in two passes of 32 lanes
+  // over a bank space of 32 doubles (same-address lanes broadcast), a 64-bit
+  // write in four passes of 16 lanes over 16 doubles.  A face pass reads the
+  // same node of 32 different element slots, which with a slot stride of 16
+  // doubles all fall on one bank; an odd slot stride SS spreads them over 32.
+  // The face terms are stored face-major, one row of FR doubles per face
+  // node (F, q) with the tile elements along the row, so the face pass
+  // writes consecutive doubles and the owner's reads of its 4 NP terms fall
+  // on distinct banks.
+  constexpr int SS = NP2 % 2 == 0 ? NP2 + 1 : NP2, FR = T + 4;
+  __shared__ double vs[NS * SS];
+  __shared__ double irs[NS * SS];
   __shared__ double gx[T * NP2], gy[T * NP2];
-  __shared__ double fP[T * 4 * NP], fQ[T * 4 * NP], fS[T * 4 * NP];   // face-node terms
-  __shared__ double fG[INIT ? T * 4 * NP : 1];                        // penalty coefficients
+  __shared__ double fP[4 * NP * FR], fQ[4 * NP * FR], fS[4 * NP * FR];   // face-node terms
+  __shared__ double fG[INIT ? 4 * NP * FR : 1];                         // penalty coefficients
   __shared__ double us[INIT ? T * NP2 : 1], ws_[INIT ? T * NP2 : 1];
 
   const Geo2 g = a.g;
@@ -152,8 +162,9 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
         if (idx == tid) own_d = dn;
       }
     }
-    vs[idx] = v;
-    irs[idx] = ir;
+    const int o = sl * SS + (idx - sl * NP2);
+    vs[o] = v;
+    irs[o] = ir;
   }
   stage_basis<NP>(B, sD, sW);
   __syncthreads();
@@ -170,14 +181,14 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   }
 
   const double al = a.alpha;
-  const double vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
+  const double vown = vs[s * SS + n], ir = irs[s * SS + n];
   double bval = 0.0;
   if (valid) {
     double dxi = 0.0, deta = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      dxi = fma(sD[i * NP + k], vs[s * NP2 + k + NP * j], dxi);
-      deta = fma(sD[j * NP + k], vs[s * NP2 + i + NP * k], deta);
+      dxi = fma(sD[i * NP + k], vs[s * SS + k + NP * j], dxi);
+      deta = fma(sD[j * NP + k], vs[s * SS + i + NP * k], deta);
     }
     gx[s * NP2 + n] = al * sW[i] * sW[j] * g.Jy * g.invJx * ir * dxi;
     gy[s * NP2 + n] = al * sW[i] * sW[j] * g.Jx * g.invJy * ir * deta;
@@ -199,12 +210,12 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   }
   // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
   // faces, NP nodes each); the tile's T 4 NP face nodes are spread over the
-  // workgroup face-major (item w: face F = w / (T NP), then element, then
-  // node q along the face), so a wave works on one face direction and its
+  // workgroup face-major (item w: face F = w / (T NP), then node q along
+  // the face, then element, fastest), so a wave works on one face direction and its
   // D rows, strides and left/right roles are compile-time constants (a
   // per-lane mix of the four faces spent most of this phase on addressing).
-  // Each face node f = F NP + q gets its consistency term P, penalty term Q
-  // and symmetry factor S in LDS; the owner node adds its own after the
+  // Each face node (F, q) of element se gets its consistency term P, penalty
+  // term Q and symmetry factor S in LDS at (F NP + q) FR + se; the owner node adds its own after the
   // barrier.  Faces F = 0, 2 (left, bottom) have the owner on their right
   // side, F = 1, 3 on their left.
   constexpr int NF = 4 * NP, NFI = T * NF, NFR = (NFI + NT - 1) / NT;
@@ -217,7 +228,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     const int lxe = se % TX, lye = se / TX;
     if (lxe < t.nvx && lye < t.nvy) {
       const int sN = F == 0 ? t.nb_left(se) : F == 1 ? t.nb_right(se) : F == 2 ? t.nb_bottom(se) : t.nb_top(se);
-      const int bo = se * NP2 + (xf ? NP * q : q), bn = sN * NP2 + (xf ? NP * q : q);
+      const int bo = se * SS + (xf ? NP * q : q), bn = sN * SS + (xf ? NP * q : q);
       double dno = 0.0, dnn = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
@@ -237,7 +248,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       Q = al * G * jump * wf;
       S = -al * (iro / 2.0) * invJ * jump * wf;
     }
-    const int f = se * NF + F * NP + q;
+    const int f = (F * NP + q) * FR + se;
     fP[f] = P;
     fQ[f] = Q;
     fS[f] = S;
@@ -248,7 +259,7 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
     const int w = tid + r * NT;
     if (NFI % NT != 0 && w >= NFI) continue;
     const int F = w / (T * NP), rem = w - F * (T * NP);
-    const int se = rem / NP, q = rem - se * NP;
+    const int q = rem / T, se = rem - q * T;
     switch (F) {
       case 0: face_node(std::integral_constant<int, 0>{}, se, q); break;
       case 1: face_node(std::integral_constant<int, 1>{}, se, q); break;
@@ -258,14 +269,16 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
   }
   __syncthreads();
 
-  const int fb = s * NF;
+  // this element's term of face node (F, q): row F NP + q, column s
+  const int fb = s;
+  constexpr int R1 = NP * FR, R2 = 2 * NP * FR, R3 = 3 * NP * FR;
   double y = 0.0;
   if (valid) {
     double fc = 0.0;        // this node's own face terms (consistency + penalty)
-    if (i == 0) { fc += fP[fb + j]; fc -= fQ[fb + j]; }
-    if (i == NP - 1) { fc -= fP[fb + NP + j]; fc += fQ[fb + NP + j]; }
-    if (j == 0) { fc += fP[fb + 2 * NP + i]; fc -= fQ[fb + 2 * NP + i]; }
-    if (j == NP - 1) { fc -= fP[fb + 3 * NP + i]; fc += fQ[fb + 3 * NP + i]; }
+    if (i == 0) { fc += fP[fb + j * FR]; fc -= fQ[fb + j * FR]; }
+    if (i == NP - 1) { fc -= fP[fb + R1 + j * FR]; fc += fQ[fb + R1 + j * FR]; }
+    if (j == 0) { fc += fP[fb + R2 + i * FR]; fc -= fQ[fb + R2 + i * FR]; }
+    if (j == NP - 1) { fc -= fP[fb + R3 + i * FR]; fc += fQ[fb + R3 + i * FR]; }
     y = sW[i] * sW[j] * g.Jx * g.Jy * vown * ir;
     double st = 0.0;
 #pragma unroll
@@ -274,8 +287,8 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
       st = fma(sD[k * NP + j], gy[s * NP2 + i + NP * k], st);
     }
     y += st;
-    y += fma(fS[fb + j], sD[i], fS[fb + NP + j] * sD[(NP - 1) * NP + i]);
-    y += fma(fS[fb + 2 * NP + i], sD[j], fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j]);
+    y += fma(fS[fb + j * FR], sD[i], fS[fb + R1 + j * FR] * sD[(NP - 1) * NP + i]);
+    y += fma(fS[fb + R2 + i * FR], sD[j], fS[fb + R3 + i * FR] * sD[(NP - 1) * NP + j]);
     y += fc;
   }
 
@@ -290,17 +303,17 @@ __device__ __forceinline__ void k2_sip_body(const SipArgs& a, const int blk, con
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const double D2 = sD[k * NP + i] * sD[k * NP + i];
-        Md = fma(al * sW[k] * sW[j] * g.Jy * g.invJx * D2, irs[s * NP2 + k + NP * j], Md);
+        Md = fma(al * sW[k] * sW[j] * g.Jy * g.invJx * D2, irs[s * SS + k + NP * j], Md);
       }
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const double D2 = sD[k * NP + j] * sD[k * NP + j];
-        Md = fma(al * sW[i] * sW[k] * g.Jx * g.invJy * D2, irs[s * NP2 + i + NP * k], Md);
+        Md = fma(al * sW[i] * sW[k] * g.Jx * g.invJy * D2, irs[s * SS + i + NP * k], Md);
       }
-      if (i == NP - 1) Md = fma(fma(al, fG[fb + NP + j], -al * g.invJx * B.D[(NP - 1) * NP + NP - 1] * ir), sW[j] * g.Jy, Md);
-      if (i == 0) Md = fma(fma(al, fG[fb + j], al * g.invJx * B.D[0] * ir), sW[j] * g.Jy, Md);
-      if (j == NP - 1) Md = fma(fma(al, fG[fb + 3 * NP + i], -al * g.invJy * B.D[(NP - 1) * NP + NP - 1] * ir), sW[i] * g.Jx, Md);
-      if (j == 0) Md = fma(fma(al, fG[fb + 2 * NP + i], al * g.invJy * B.D[0] * ir), sW[i] * g.Jx, Md);
+      if (i == NP - 1) Md = fma(fma(al, fG[fb + R1 + j * FR], -al * g.invJx * B.D[(NP - 1) * NP + NP - 1] * ir), sW[j] * g.Jy, Md);
+      if (i == 0) Md = fma(fma(al, fG[fb + j * FR], al * g.invJx * B.D[0] * ir), sW[j] * g.Jy, Md);
+      if (j == NP - 1) Md = fma(fma(al, fG[fb + R3 + i * FR], -al * g.invJy * B.D[(NP - 1) * NP + NP - 1] * ir), sW[i] * g.Jx, Md);
+      if (j == 0) Md = fma(fma(al, fG[fb + R2 + i * FR], al * g.invJy * B.D[0] * ir), sW[i] * g.Jx, Md);
       const double iM = 1.0 / Md;
       o_iM = iM;
       if (!(ir > 0.0) || !isfinite(ir)) flag_domain(a.flag);
