@@ -192,12 +192,15 @@ template <int NP, int TX, int TY, int MODE>
 __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile, const int ntiles, RevLoop& L) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool FIN = (MODE == RV_PCG_FIN || MODE == RV_JAC_FIN || MODE == RV_CHEB_FIN);
-  __shared__ double ws[NS * NP2];   // adjoint vector w (A is applied to it)
-  __shared__ double vs[NS * NP2];   // vector v of the forward product A v
-  __shared__ double irs[NS * NP2];
+  // LDS bank layout as in the forward SIP family: element slots SS doubles
+  // apart (odd), face terms face-major in rows of FR doubles
+  constexpr int SS = NP2 % 2 == 0 ? NP2 + 1 : NP2, FR = T + 4;
+  __shared__ double ws[NS * SS];   // adjoint vector w (A is applied to it)
+  __shared__ double vs[NS * SS];   // vector v of the forward product A v
+  __shared__ double irs[NS * SS];
   __shared__ double gx[T * NP2], gy[T * NP2];
-  __shared__ double fC[T * 4 * NP], fS[T * 4 * NP], fR[T * 4 * NP];   // face-node terms
-  __shared__ double mbs[FIN ? NS * NP2 : 1];                 // M-bar
+  __shared__ double fC[4 * NP * FR], fS[4 * NP * FR], fR[4 * NP * FR];   // face-node terms
+  __shared__ double mbs[FIN ? NS * SS : 1];                 // M-bar
   __shared__ double us[FIN ? T * NP2 : 1], vvs[FIN ? T * NP2 : 1];
   __shared__ double Gxx[FIN ? T * NP2 : 1], Gxy[FIN ? T * NP2 : 1], Gyx[FIN ? T * NP2 : 1], Gyy[FIN ? T * NP2 : 1];
   __shared__ double red[4 * (NT / WAVE + 1)];
@@ -272,9 +275,10 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
         const double z = l_R[l] * l_iM[l];
         const double zb = l_pb[l] + rzb * l_R[l];
         const double rb = l_rc[l] + rzb * z + zb * l_iM[l];
-        ws[idx] = -ak * rb + papb * l_p[l];
-        vs[idx] = l_p[l];
-        irs[idx] = l_ir[l];
+        const int o = (idx / NP2) * SS + idx % NP2;
+        ws[o] = -ak * rb + papb * l_p[l];
+        vs[o] = l_p[l];
+        irs[o] = l_ir[l];
         if (l == 0) { o_rb = rb; o_zb = zb; o_R = l_R[l]; o_pb = l_pb[l]; }
       }
     }
@@ -331,8 +335,9 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
         if (idx == tid) { o_rb = rb; o_zb = zb; o_R = Rk; o_pb = pb; }
       }
     }
-    ws[idx] = w; vs[idx] = v; irs[idx] = ir;
-    if constexpr (FIN) mbs[idx] = mb;
+    const int o = sl * SS + nn;
+    ws[o] = w; vs[o] = v; irs[o] = ir;
+    if constexpr (FIN) mbs[o] = mb;
   }
   stage_basis<NP>(B, sD, sW);
   __syncthreads();
@@ -352,17 +357,17 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
   }
 
   const int sL = t.nb_left(s), sR = t.nb_right(s), sB = t.nb_bottom(s), sT = t.nb_top(s);
-  const double wown = ws[s * NP2 + n], vown = vs[s * NP2 + n], ir = irs[s * NP2 + n];
+  const double wown = ws[s * SS + n], vown = vs[s * SS + n], ir = irs[s * SS + n];
   double fc = 0.0, irb = 0.0;
   double bown = 0.0;
   if (valid) {
     double dxw = 0.0, dyw = 0.0, dxv = 0.0, dyv = 0.0;
 #pragma unroll
     for (int kk = 0; kk < NP; ++kk) {
-      dxw += sD[i * NP + kk] * ws[s * NP2 + kk + NP * j];
-      dyw += sD[j * NP + kk] * ws[s * NP2 + i + NP * kk];
-      dxv += sD[i * NP + kk] * vs[s * NP2 + kk + NP * j];
-      dyv += sD[j * NP + kk] * vs[s * NP2 + i + NP * kk];
+      dxw += sD[i * NP + kk] * ws[s * SS + kk + NP * j];
+      dyw += sD[j * NP + kk] * ws[s * SS + i + NP * kk];
+      dxv += sD[i * NP + kk] * vs[s * SS + kk + NP * j];
+      dyv += sD[j * NP + kk] * vs[s * SS + i + NP * kk];
     }
     const double kx = al * sW[i] * sW[j] * g.Jy * g.invJx;
     const double ky = al * sW[i] * sW[j] * g.Jx * g.invJy;
@@ -393,10 +398,10 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
   }
   // Face terms.  An element has 4 NP face nodes (left, right, bottom, top
   // faces, NP nodes each); the tile's T 4 NP face nodes are spread over the
-  // workgroup face-major (item w: face F = w / (T NP), then element, then
-  // node q along the face), so a wave works on one face direction with
-  // compile-time D rows, strides and left/right roles.  Face node
-  // f = F NP + q leaves in LDS its part of the owner's A w (consistency +
+  // workgroup face-major (item w: face F = w / (T NP), then node q along
+  // the face, then element, fastest), so a wave works on one face direction with
+  // compile-time D rows, strides and left/right roles.  Face node (F, q) of
+  // element se leaves in LDS, at (F NP + q) FR + se, its part of the owner's A w (consistency +
   // penalty), its symmetry factor, and its part of d/d(1/rho_owner) of the
   // face's bilinear form in (w, v); the owner adds its own after the
   // barrier.  Faces F = 0, 2 (left, bottom) have the owner on their right
@@ -411,7 +416,7 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
     const int lxe = se % TX, lye = se / TX;
     if (lxe < t.nvx && lye < t.nvy) {
       const int sN = F == 0 ? t.nb_left(se) : F == 1 ? t.nb_right(se) : F == 2 ? t.nb_bottom(se) : t.nb_top(se);
-      const int bo = se * NP2 + (xf ? NP * q : q), bn = sN * NP2 + (xf ? NP * q : q);
+      const int bo = se * SS + (xf ? NP * q : q), bn = sN * SS + (xf ? NP * q : q);
       double dow = 0.0, dov = 0.0, dnw = 0.0, dnv = 0.0;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
@@ -442,7 +447,7 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
       Rb = -al * wf * jw * invJ * dv_own / 2.0 - al / 2.0 * invJ * jv * wf * dw_own;
       if (own_L == L_chosen) Rb += al * g.eta / h * jv * wf * jw;
     }
-    const int f = se * NF + F * NP + q;
+    const int f = (F * NP + q) * FR + se;
     fC[f] = C;
     fS[f] = Sy;
     fR[f] = Rb;
@@ -452,7 +457,7 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
     const int w = tid + r * NT;
     if (NFI % NT != 0 && w >= NFI) continue;
     const int F = w / (T * NP), rem = w - F * (T * NP);
-    const int se = rem / NP, q = rem - se * NP;
+    const int q = rem / T, se = rem - q * T;
     switch (F) {
       case 0: face_node(std::integral_constant<int, 0>{}, se, q); break;
       case 1: face_node(std::integral_constant<int, 1>{}, se, q); break;
@@ -462,13 +467,15 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
   }
   __syncthreads();
 
-  const int fb = s * NF;
+  // this element's term of face node (F, q): row F NP + q, column s
+  const int fb = s;
+  constexpr int R1 = NP * FR, R2 = 2 * NP * FR, R3 = 3 * NP * FR;
   double y = 0.0, ab = 0.0;
   if (valid) {
-    if (i == 0) { fc += fC[fb + j]; irb += fR[fb + j]; }
-    if (i == NP - 1) { fc += fC[fb + NP + j]; irb += fR[fb + NP + j]; }
-    if (j == 0) { fc += fC[fb + 2 * NP + i]; irb += fR[fb + 2 * NP + i]; }
-    if (j == NP - 1) { fc += fC[fb + 3 * NP + i]; irb += fR[fb + 3 * NP + i]; }
+    if (i == 0) { fc += fC[fb + j * FR]; irb += fR[fb + j * FR]; }
+    if (i == NP - 1) { fc += fC[fb + R1 + j * FR]; irb += fR[fb + R1 + j * FR]; }
+    if (j == 0) { fc += fC[fb + R2 + i * FR]; irb += fR[fb + R2 + i * FR]; }
+    if (j == NP - 1) { fc += fC[fb + R3 + i * FR]; irb += fR[fb + R3 + i * FR]; }
     const double cm = sW[i] * sW[j] * g.Jx * g.Jy;
     y = cm * wown * ir;
     double st = 0.0;
@@ -478,8 +485,8 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
       st += sD[kk * NP + j] * gy[s * NP2 + i + NP * kk];
     }
     y += st;
-    y += fS[fb + j] * sD[i] + fS[fb + NP + j] * sD[(NP - 1) * NP + i];
-    y += fS[fb + 2 * NP + i] * sD[j] + fS[fb + 3 * NP + i] * sD[(NP - 1) * NP + j];
+    y += fS[fb + j * FR] * sD[i] + fS[fb + R1 + j * FR] * sD[(NP - 1) * NP + i];
+    y += fS[fb + R2 + i * FR] * sD[j] + fS[fb + R3 + i * FR] * sD[(NP - 1) * NP + j];
     y += fc;
     // d/d alpha of <w, A v> = <v, K w>, K = (A - mass)/alpha
     if (al != 0.0) ab = vown * (y - cm * wown * ir) / al;
@@ -554,19 +561,19 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
       // diag(A) reverse: M = cm/rho + x/y stiffness self-coupling + face terms
       const double iM = a.invM[gme];
       const double Mown = 1.0 / iM;
-      const double mbo = mbs[s * NP2 + n];
+      const double mbo = mbs[s * SS + n];
       ab_extra += (al != 0.0) ? mbo * (Mown - cm * ir) / al : 0.0;
       double irbd = cm * mbo;
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk) {
         // x-stiffness of node (kk, j) couples to this node (i, j): alpha w_i w_j Jy/Jx D[i][kk]^2
         const double Dx = sD[i * NP + kk];
-        irbd += al * sW[i] * sW[j] * g.Jy * g.invJx * Dx * Dx * mbs[s * NP2 + kk + NP * j];
+        irbd += al * sW[i] * sW[j] * g.Jy * g.invJx * Dx * Dx * mbs[s * SS + kk + NP * j];
         const double Dy = sD[j * NP + kk];
-        irbd += al * sW[i] * sW[j] * g.Jx * g.invJy * Dy * Dy * mbs[s * NP2 + i + NP * kk];
+        irbd += al * sW[i] * sW[j] * g.Jx * g.invJy * Dy * Dy * mbs[s * SS + i + NP * kk];
       }
       auto dface = [&](bool own_L, int sn, int nnode, double invJ, double h, double wf, double Dself) {
-        const double irn = irs[sn * NP2 + nnode], mbn = mbs[sn * NP2 + nnode];
+        const double irn = irs[sn * SS + nnode], mbn = mbs[sn * SS + nnode];
         const double irL = own_L ? ir : irn, irR = own_L ? irn : ir;
         irbd += (own_L ? -1.0 : 1.0) * al * invJ * Dself * wf * mbo;
         const bool L_chosen = !(irR > irL);
