@@ -202,7 +202,7 @@ def p4_leg(igr, configs, fwd_steps=60):
         out["forward_ms_per_step"] = t / fwd_steps * 1e3
         out["dof_stage_per_s"] = 4 * n_nodes * 3 * fwd_steps / t
         for name, kern, fields in KERNEL_FIELDS:
-            ms = ctx.time_kernel(kern, gam, al, 200)
+            ms = kernel_ms(ctx, kern, gam, al)[0]
             alg = fields * 8 * n_nodes
             out[name] = {"us": ms * 1e3, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         ctx.set_state(cfg["q"])
@@ -241,6 +241,17 @@ def c1_leg(igr, configs, cpu_steps=200):
     return out
 
 
+def kernel_ms(ctx, kern, gam, al, reps=400, rounds=5):
+    """Mean time per launch (ms) of `reps` back-to-back launches between two
+    HIP events on the context stream: the median of `rounds` such timings,
+    after one discarded round that brings the GPU clocks up from the idle
+    state the previous leg left (a first timing from idle reads up to 10%
+    long).  Returns (median, min, max)."""
+    ctx.time_kernel(kern, gam, al, reps)
+    ts = sorted(ctx.time_kernel(kern, gam, al, reps) for _ in range(rounds))
+    return ts[len(ts) // 2], ts[0], ts[-1]
+
+
 # Algorithmic bytes per node of the three forward kernels as timed (an
 # iteration after the first, which is not the last; each distinct input and
 # output field once, 8 B per node, halo re-reads excluded): PCG_A reads r,
@@ -250,10 +261,11 @@ def c1_leg(igr, configs, cpu_steps=200):
 KERNEL_FIELDS = (("pcg_a", 3, 8), ("pcg_b", 4, 4), ("hyp", 5, 13))
 
 
-def kernel_rows(igr, configs, N, reps=200):
+def kernel_rows(igr, configs, N, reps=400):
     """Mean in-stream time per launch of the three forward kernels on the
     headline workload's data at N x N (back-to-back launches between two HIP
-    events on the context's stream) and their fraction of the HBM peak."""
+    events on the context's stream, median of five timings: kernel_ms) and
+    their fraction of the HBM peak."""
     cfg = configs.config2(N=N)
     ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(3)))
     ctx.set_state(cfg["q"])
@@ -262,10 +274,10 @@ def kernel_rows(igr, configs, N, reps=200):
            "tiles": {"sip": ctx.get_option(igr.core.OPT_TILE), "hyperbolic": ctx.get_option(igr.core.OPT_HTILE),
                      "pcg_b_workgroups": ctx.get_option(igr.core.OPT_GROUPS)}}
     for name, kern, fields in KERNEL_FIELDS:
-        ms = ctx.time_kernel(kern, cfg["gamma"], cfg["alpha"], reps)
+        ms, lo, hi = kernel_ms(ctx, kern, cfg["gamma"], cfg["alpha"], reps)
         alg = fields * 8 * n_nodes
-        row[name] = {"us": ms * 1e3, "alg_bytes": alg, "gbs": alg / (ms * 1e-3) / 1e9,
-                     "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        row[name] = {"us": ms * 1e3, "us_min_max": [lo * 1e3, hi * 1e3], "alg_bytes": alg,
+                     "gbs": alg / (ms * 1e-3) / 1e9, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     ctx.close()
     return row
 
@@ -275,7 +287,7 @@ def size_sweep_leg(igr, configs, sizes=(128, 256, 512, 1024)):
     moves about one latency-bandwidth product of data (12.6 MB), so the time
     is mostly latency; larger meshes show the streaming rate the same
     kernels reach."""
-    return [kernel_rows(igr, configs, N, reps=200 if N <= 512 else 50) for N in sizes]
+    return [kernel_rows(igr, configs, N, reps=400 if N <= 512 else 50) for N in sizes]
 
 
 def adjoint_pde_leg(igr, configs, cfg, steps, prm, qbar, t_forward):
@@ -360,11 +372,12 @@ def main():
     # algorithmic bytes per launch: read r, 1/M, p_old, 1/rho, x; write p,
     # Ap, x (the previous iteration's x += a p)
     # duration per launch: 400 back-to-back launches of one PCG_A (same
-    # inputs each time) between two events on the context stream; the
-    # per-launch kernel-stamped figure is reported beside it
+    # inputs each time) between two events on the context stream, median of
+    # five such timings after a discarded one (kernel_ms); the per-launch
+    # kernel-stamped figure is reported beside it
     reps = 400
-    ms = ctx.time_kernel(3, gam, al, reps)
-    ms_stamped = ctx.time_kernel(0, gam, al, reps)
+    ms, ms_lo, ms_hi = kernel_ms(ctx, 3, gam, al, reps)
+    ms_stamped = kernel_ms(ctx, 0, gam, al, reps)[0]
     alg_bytes = 8 * 8 * n_nodes
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -376,8 +389,8 @@ def main():
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    ms_b = ctx.time_kernel(4, gam, al, reps)
-    ms_h = ctx.time_kernel(5, gam, al, reps)
+    ms_b = kernel_ms(ctx, 4, gam, al, reps)[0]
+    ms_h = kernel_ms(ctx, 5, gam, al, reps)[0]
 
     # adjoint gradient of J = int rho(T) w (Gaussian window, sigma 0.1) over the
     # same steps, w.r.t. the initial state, gamma and alpha; checkpoint slots
@@ -469,11 +482,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "launch_us": ms * 1e3,
+            "launch_us_min_max": [ms_lo * 1e3, ms_hi * 1e3],
             "launch_us_kernel_stamped": ms_stamped * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },
         "kernels": {
-            "timing": "back-to-back launches between two HIP events on the context stream, mean per launch",
+            "timing": "back-to-back launches between two HIP events on the context stream, mean per launch; "
+                      "median of five 400-launch timings after a discarded one",
             "pcg_a_sip": {"us": ms * 1e3, "frac": achieved / HBM_PEAK_GBS},
             "pcg_b_update": {"us": ms_b * 1e3, "alg_bytes": 4 * 8 * n_nodes,
                              "frac": 4 * 8 * n_nodes / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS},
