@@ -188,8 +188,9 @@ struct RevLoop {
 
 // One tile of the reverse SIP family: tile `tile` of `ntiles` (a single-tile
 // workgroup passes its block index and the grid size).
-template <int NP, int TX, int TY, int MODE>
-__device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile, const int ntiles, RevLoop& L) {
+template <int NP, int TX, int TY, int MODE, class Pre>
+__device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile, const int ntiles, RevLoop& L,
+                                                Pre&& scalars) {
   constexpr int NP2 = NP * NP, T = TX * TY, NS = T + 2 * (TX + TY), NT = T * NP2;
   constexpr bool FIN = (MODE == RV_PCG_FIN || MODE == RV_JAC_FIN || MODE == RV_CHEB_FIN);
   // LDS bank layout as in the forward SIP family: element slots SS doubles
@@ -230,10 +231,10 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
 
   constexpr int NL = (NS * NP2 + NT - 1) / NT;
   double l_ir[NL], l_iM[NL], l_rc[NL], l_pb[NL], l_R[NL], l_p[NL];
+  bool oks[NL];
   double f_irb = 0.0, f_imb = 0.0, f_xb = 0.0, f_apk = 0.0, f_pm = 0.0, f_apm = 0.0, f_Rk = 0.0;
   if constexpr (MODE == RV_PCG_ITER) {
     size_t gis[NL];
-    bool oks[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       size_t gi = 0;
@@ -252,11 +253,9 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
       l_ir[l] = a.invR[gi]; l_iM[l] = a.invM[gi]; l_rc[l] = rcin[gi]; l_pb[l] = pbin[gi];
       l_R[l] = Rk1[gi]; l_p[l] = Pk[gi];
     }
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      if (!oks[l]) { l_ir[l] = 0.0; l_iM[l] = 0.0; l_R[l] = 0.0; l_p[l] = 0.0; }
-      if (!oks[l] || a.first) { l_rc[l] = 0.0; l_pb[l] = 0.0; }
-    }
+    // the reverse scalars (the previous kernel's grid totals) while the
+    // loads are in flight; an empty slot's values are dropped at staging
+    scalars(L);
   }
   if constexpr (MODE == RV_PCG_ITER) {
     ak = L.ak; bk = L.bk; rzb = L.rzb; papb = L.papb;
@@ -272,14 +271,17 @@ __device__ __forceinline__ void k2_sip_rev_body(const RevArgs& a, const int tile
     for (int l = 0; l < NL; ++l) {
       const int idx = tid + l * NT;
       if (idx < NS * NP2) {
-        const double z = l_R[l] * l_iM[l];
-        const double zb = l_pb[l] + rzb * l_R[l];
-        const double rb = l_rc[l] + rzb * z + zb * l_iM[l];
+        const bool ok = oks[l], okc = ok && !a.first;
+        const double R = ok ? l_R[l] : 0.0, iM = ok ? l_iM[l] : 0.0, p = ok ? l_p[l] : 0.0;
+        const double ir = ok ? l_ir[l] : 0.0, rc = okc ? l_rc[l] : 0.0, pb = okc ? l_pb[l] : 0.0;
+        const double z = R * iM;
+        const double zb = pb + rzb * R;
+        const double rb = rc + rzb * z + zb * iM;
         const int o = (idx / NP2) * SS + idx % NP2;
-        ws[o] = -ak * rb + papb * l_p[l];
-        vs[o] = l_p[l];
-        irs[o] = l_ir[l];
-        if (l == 0) { o_rb = rb; o_zb = zb; o_R = l_R[l]; o_pb = l_pb[l]; }
+        ws[o] = -ak * rb + papb * p;
+        vs[o] = p;
+        irs[o] = ir;
+        if (l == 0) { o_rb = rb; o_zb = zb; o_R = R; o_pb = pb; }
       }
     }
   }
@@ -635,6 +637,9 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
     // sums (the stored forward scalars are loaded with them)
     const int k = a.k;
     const FS fs(a.n_iter);
+    // the partial sums are requested first; the grid total and the reverse
+    // scalars are formed inside the (first) tile, after its field loads are
+    // issued, so that both memory latencies overlap
     double pre4[4];
     grid_prefetch<4>(a.part_in, a.np_in, pre4);
     const double f_ab = a.abar_blk[blockIdx.x];
@@ -642,21 +647,30 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
     L.bk = a.sc[fs.beta(k)];
     const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
     const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
-    double d[4];
-    grid_finish<4, 4>(a.part_in, a.np_in, pre4, d);
-    const double eps = 1e-30;
-    const double bb = d[0];
-    const double rzb = carry_next + bb / (rzk + eps);
-    const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
-    const double papb = -abk * rzk / ((papk + eps) * (papk + eps));
-    const double carry = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
-    // every lane holds the same totals: keep them in scalar registers
-    L.rzb = uniform_d(rzb); L.papb = uniform_d(papb);
-    L.r_bb = uniform_d(bb); L.r_abk = uniform_d(abk); L.r_carry = uniform_d(carry);
+    bool formed = false;
+    auto scalars = [&](RevLoop& S) {
+      if (formed) return;        // a looping workgroup forms them with its first tile
+      double d[4];
+      grid_finish<4, 4>(a.part_in, a.np_in, pre4, d);
+      const double eps = 1e-30;
+      const double bb = d[0];
+      const double rzb = carry_next + bb / (rzk + eps);
+      const double abk = d[1] - d[2] - 2.0 * rzb * d[3];
+      const double papb = -abk * rzk / ((papk + eps) * (papk + eps));
+      const double carry = -bb * rzk1 / ((rzk + eps) * (rzk + eps)) + abk / (papk + eps);
+      // every lane holds the same totals: keep them in scalar registers
+      S.rzb = uniform_d(rzb); S.papb = uniform_d(papb);
+      S.r_bb = uniform_d(bb); S.r_abk = uniform_d(abk); S.r_carry = uniform_d(carry);
+      formed = true;
+    };
     if constexpr (LOOP) {
-      for (int tt = blockIdx.x; tt < ntiles; tt += gridDim.x) k2_sip_rev_body<NP, TX, TY, MODE>(a, tt, ntiles, L);
+      // formed before the tile loop: inside it, the loop's register budget
+      // spills (256^2 20-step gradient 0.245 -> 0.299 s)
+      scalars(L);
+      for (int tt = blockIdx.x; tt < ntiles; tt += gridDim.x)
+        k2_sip_rev_body<NP, TX, TY, MODE>(a, tt, ntiles, L, [](RevLoop&) {});
     } else {
-      k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L);
+      k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L, scalars);
     }
     double dv5[5] = {L.acc[0], L.acc[1], L.acc[2], L.acc[3], L.acc[4]};
     if (block_partials_store_last<5, 4>(dv5, a.part_out, gridDim.x, blockIdx.x, &arrive) &&
@@ -671,7 +685,7 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
       a.rs[RS::carry(k)] = L.r_carry;
     }
   } else {
-    k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L);
+    k2_sip_rev_body<NP, TX, TY, MODE>(a, blockIdx.x, gridDim.x, L, [](RevLoop&) {});
   }
 }
 // one tile per workgroup (every mode)
