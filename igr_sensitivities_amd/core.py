@@ -158,7 +158,7 @@ class Context:
         return v.value
 
     def autotune(self):
-        """Time the tile shapes on the current state and keep the fastest (2D, p = 3);
+        """Time the tile shapes on the current state and keep the fastest (2D, p = 3 and 4);
         returns (SIP tile, hyperbolic tile) as tile_x*100 + tile_y."""
         self.set_option(OPT_AUTOTUNE, 0)
         return self.get_option(OPT_TILE), self.get_option(OPT_HTILE)

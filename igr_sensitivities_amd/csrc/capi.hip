@@ -241,12 +241,13 @@ void set_tiles(Ctx& c, int tx, int ty, int htx, int hty) {
 void default_tiles(Ctx& c) {
   if (c.np == 5) {
     // p = 4: the SIP family's tile by mesh size (the other families keep
-    // 2x4).  At 160^2 (the reference's p = 4 heatmap runs), one tile per x/r
-    // workgroup, a PCG-60 step takes 5.41 ms with 8x5 tiles, 5.81 (4x4),
-    // 5.86 (5x5) and 6.10 (2x4); with the x/r update on 256 workgroups 5x5
-    // tiles take 5.09 ms and 8x5 5.24.
+    // 2x4).  A PCG-60 step of the triple-Sedov data on MI355X, 4x4 / 5x5 /
+    // 8x5 tiles (x/r update on max(256, tiles / 16) workgroups beyond 512
+    // tiles): 64^2 1.96 / 2.08 / 2.31 ms; 96^2 2.96 / 2.69 / 2.45; 128^2
+    // 3.64 / 3.74 / 3.70; 160^2 (the reference's p = 4 heatmap runs) 5.03 /
+    // 5.04 / 4.91; 200^2 7.21 / 7.20 / 6.33; 256^2 10.45 / 10.90 / 10.55.
     const long long E = (long long)c.Nex * c.Ney;
-    if (E >= 96 * 96) { c.tile_x = 5; c.tile_y = 5; }
+    if (E >= 96 * 96) { c.tile_x = 8; c.tile_y = 5; }
     else if (E >= 32 * 32) { c.tile_x = 4; c.tile_y = 4; }
     else { c.tile_x = 2; c.tile_y = 4; }
     const int nt = ((c.Nex + c.tile_x - 1) / c.tile_x) * ((c.Ney + c.tile_y - 1) / c.tile_y);
@@ -286,7 +287,13 @@ void default_tiles(Ctx& c) {
 // hyperbolic-family shape (HIP events, best of 3), and keep the fastest.
 // Results change only in the order of the solver's partial sums.
 int autotune(Ctx& c) {
-  const int sip[3][2] = {{4, 4}, {8, 4}, {8, 8}}, hyp[2][2] = {{4, 4}, {8, 4}};
+  // p = 3: three SIP-family and two hyperbolic-family shapes; p = 4: the
+  // four SIP-family shapes (its hyperbolic family has one)
+  const bool p4 = c.np == 5;
+  const int sip3[3][2] = {{4, 4}, {8, 4}, {8, 8}}, sip4[4][2] = {{2, 4}, {4, 4}, {5, 5}, {8, 5}};
+  const int (*sip)[2] = p4 ? sip4 : sip3;
+  const int nsip = p4 ? 4 : 3, nhyp = p4 ? 1 : 2;
+  const int hyp[2][2] = {{4, 4}, {8, 4}};
   const double h = c.Lx / c.Nex;
   const double alpha = 10.0 * (h / 4.0) * (h / 4.0), gamma = 1.4;
   const int n_it = 30;
@@ -311,7 +318,7 @@ int autotune(Ctx& c) {
   int bt = 0, bh = 0;
   float tb = 1e30f, th = 1e30f;
   const int htx0 = c.htile_x, hty0 = c.htile_y;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < nsip; ++k) {
     set_tiles(c, sip[k][0], sip[k][1], htx0, hty0);
     const float t = timed([&]() {
       launch_copy(c, c.xA, c.S, c.N);
@@ -319,7 +326,7 @@ int autotune(Ctx& c) {
     });
     if (t < tb) { tb = t; bt = k; }
   }
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < nhyp && !p4; ++k) {
     set_tiles(c, sip[bt][0], sip[bt][1], hyp[k][0], hyp[k][1]);
     HypArgs a = hyp_base(c, gamma);
     a.q = c.q; a.mode = 2; a.cdt = 1e-6; a.out = c.q2; a.qn = c.q;
@@ -328,7 +335,8 @@ int autotune(Ctx& c) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  set_tiles(c, sip[bt][0], sip[bt][1], hyp[bh][0], hyp[bh][1]);
+  if (p4) set_tiles(c, sip[bt][0], sip[bt][1], htx0, hty0);
+  else set_tiles(c, sip[bt][0], sip[bt][1], hyp[bh][0], hyp[bh][1]);
   return IGR_OK;
 }
 
@@ -459,7 +467,7 @@ int igr_set_option(igr_ctx* c, int option, int value) {
     return IGR_OK;
   }
   if (option == IGR_OPT_AUTOTUNE) {
-    if (c->dim != 2 || c->np != 4) return IGR_OK;   // one tile shape exists for the other cases
+    if (c->dim != 2 || (c->np != 4 && c->np != 5)) return IGR_OK;   // one tile shape exists for the other cases
     int rc = begin_call(*c);
     if (rc) return rc;
     rc = autotune(*c);

@@ -57,11 +57,13 @@ int igr_sizes(const igr_ctx* ctx, long long* n_nodes, int* n_vars);
  *  IGR_OPT_GRAPHS   1 = capture each SSP-RK3 step in a hipGraph and replay it,
  *                   0 = launch kernels directly;
  *  IGR_OPT_TILE     2D elements per workgroup of the SIP / solver kernels,
- *                   tile_x*100 + tile_y (p = 3: 404, 804, 808, 202);
+ *                   tile_x*100 + tile_y (p = 3: 404, 804, 808, 202;
+ *                   p = 4: 204, 404, 505, 805);
  *  IGR_OPT_HTILE    the same for the hyperbolic, tangent and objective kernels
  *                   (p = 3: 404, 804, 202);
  *  IGR_OPT_AUTOTUNE (value ignored) time the forward shapes on the current
- *                   state and keep the fastest of each family;
+ *                   state and keep the fastest of each family (2D, p = 3
+ *                   and 4);
  *  IGR_OPT_RTILE    the same for the reverse SIP / solver kernels of the
  *                   gradient and the VJPs (p = 3: 404, 804, 808, 202);
  *  IGR_OPT_GROUPS   workgroups of the PCG x/r update kernel (2D): 0 = one per

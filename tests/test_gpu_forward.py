@@ -234,6 +234,22 @@ def test_tile_shapes_agree(orc, nx, ny):
         c.set_option(igr.core.OPT_TILE, 303)
 
 
+def test_autotune_p4_keeps_results(orc):
+    # p = 4: the autotuner times the four SIP-family shapes (2x4, 4x4, 5x5,
+    # 8x5) and keeps the fastest; the run after it still matches the oracle
+    q0 = configs.taylor_green2(4, 12, 10, A=0.3)
+    dt = 0.4 * (1 / 12) / (9 * configs.max_wavespeed2(q0, 1.4))
+    c = ctx2(4, 12, 10)
+    c.set_state(q0)
+    tile, htile = c.autotune()
+    print(f"[autotune] p=4 12x10 -> SIP tile {tile}", flush=True)
+    assert tile in (204, 404, 505, 805)
+    c.set_state(q0)
+    c.run_steps(2, dt, 1.4, 2e-3, 60, "pcg")
+    ref, _ = orc.Grid2D(4, 12, 10).run_steps(1.4, 2e-3, 60, orc.PCG, 2, dt, q0)
+    assert rel(c.get_state()[0], ref) < 1e-11
+
+
 def test_autotune_keeps_results(orc):
     q0 = configs.taylor_green2(3, 32, 32, A=0.3)
     dt = 0.4 * (1 / 32) / (7 * configs.max_wavespeed2(q0, 1.4))
