@@ -177,13 +177,13 @@ def c5_leg(igr, configs, budget_slots=40):
     return out
 
 
-def p4_leg(igr, configs, fwd_steps=60):
+def p4_leg(igr, configs, fwd_steps=60, script=True):
     """The reference's p = 4 production run (the triple-Sedov heatmap of
     IGR_adjoints_2d/paper_scripts/generate_sedov_heatmap_data.jl: 160 x 160,
     PCG-60, T = 0.10): forward throughput over `fwd_steps` steps, the three
     forward kernels' fraction of HBM peak, and the script's whole GPU work,
     run_forward_store over T followed by run_adjoint_conservative of the
-    kinetic energy."""
+    kinetic energy (skipped with script=False, for profiler runs)."""
     cfg = configs.sedov_heatmap_p4()
     N = cfg["N"]
     ctx = new_ctx(igr, igr.CartesianMesh2D(N, N, 1.0, 1.0, igr.DGBasis(4)))
@@ -205,6 +205,9 @@ def p4_leg(igr, configs, fwd_steps=60):
             ms = kernel_ms(ctx, kern, gam, al)[0]
             alg = fields * 8 * n_nodes
             out[name] = {"us": ms * 1e3, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if not script:
+            ctx.close()
+            return out
         ctx.set_state(cfg["q"])
         t0 = time.perf_counter()
         n = ctx.run_forward_store(dt, cfg["T"], gam, al, n_iter, "pcg")
@@ -337,6 +340,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 1D blast run")
     ap.add_argument("--no-adjpde", action="store_true", help="skip the continuous-adjoint leg")
     ap.add_argument("--no-p4", action="store_true", help="skip the p = 4 heatmap-run leg")
+    ap.add_argument("--p4-no-script", action="store_true",
+                    help="p = 4 leg: forward steps and kernel timings only, not the heatmap script's store + adjoint")
     ap.add_argument("--graphs-off", action="store_true",
                     help="launch every kernel directly instead of replaying captured hipGraphs (for profilers)")
     args = ap.parse_args()
@@ -411,7 +416,7 @@ def main():
     ens = None if args.no_ensemble else ensemble_leg(igr, configs)
     c5 = None if args.no_c5 else c5_leg(igr, configs)
     blast1 = None if args.no_c1 else c1_leg(igr, configs)
-    p4 = None if args.no_p4 else p4_leg(igr, configs)
+    p4 = None if args.no_p4 else p4_leg(igr, configs, script=not args.p4_no_script)
     sweep = None if args.no_sweep else size_sweep_leg(igr, configs)
     # the kernels at the 256x256 size of the blast-vortex workload
     big = None
