@@ -637,9 +637,9 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
     // sums (the stored forward scalars are loaded with them)
     const int k = a.k;
     const FS fs(a.n_iter);
-    // the partial sums are requested first; the grid total and the reverse
-    // scalars are formed inside the (first) tile, after its field loads are
-    // issued, so that both memory latencies overlap
+    // the partial sums are requested first; a one-tile workgroup forms the
+    // grid total and the reverse scalars inside its tile, after the tile's
+    // field loads are issued, so that both memory latencies overlap
     double pre4[4];
     grid_prefetch<4>(a.part_in, a.np_in, pre4);
     const double f_ab = a.abar_blk[blockIdx.x];
@@ -647,9 +647,7 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
     L.bk = a.sc[fs.beta(k)];
     const double rzk = a.sc[fs.rz(k)], rzk1 = a.sc[fs.rz(k + 1)], papk = a.sc[fs.pap(k)];
     const double carry_next = a.first ? 0.0 : a.rs[RS::carry(k + 1)];
-    bool formed = false;
     auto scalars = [&](RevLoop& S) {
-      if (formed) return;        // a looping workgroup forms them with its first tile
       double d[4];
       grid_finish<4, 4>(a.part_in, a.np_in, pre4, d);
       const double eps = 1e-30;
@@ -661,7 +659,6 @@ __device__ __forceinline__ void k2_sip_rev_kernel(const RevArgs& a) {
       // every lane holds the same totals: keep them in scalar registers
       S.rzb = uniform_d(rzb); S.papb = uniform_d(papb);
       S.r_bb = uniform_d(bb); S.r_abk = uniform_d(abk); S.r_carry = uniform_d(carry);
-      formed = true;
     };
     if constexpr (LOOP) {
       // formed before the tile loop: inside it, the loop's register budget
