@@ -24,8 +24,10 @@ OBJ_WINDOWED_KE, OBJ_WEIGHTED_MOMENTUM, OBJ_WINDOWED_DENSITY, OBJ_DISC_KE = 3, 4
 KE_WINDOW = (0.55, 0.50, 0.15)   # IGR_adjoints_2d/enzyme_adjoint.jl:26
 
 
-def build():
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(force=False):
+    """Compile the oracle libraries (make); force=True recompiles them even
+    when they are up to date."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
 
 
 _lib = None
