@@ -510,6 +510,21 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
   // after the staging barrier instead, where it enters, it took 256^2 from
   // 23.8 to 26.0 us: with four rounds of workgroups the entry loads already
   // overlap other workgroups' phases).
+  // LDS bank swizzle of the tile's node slots at p = 3: node n of element s
+  // sits at s NP2 + (n ^ 5a), a = (s / 2) mod 4, so the face pass (the same
+  // face column or row of eight consecutive elements per 32 lanes, which
+  // 16-double element slots put on one bank pair) and the volume term's
+  // reads fall on distinct banks; the slot of a node within its element
+  // changes, the values and their order of use do not.
+  auto nslot = [](const int se, const int nn) -> int {
+    if constexpr (NP == 4) {
+      const int a = (se >> 1) & 3;
+      return se * NP2 + (nn ^ (a | (a << 2)));
+    } else {
+      return se * NP2 + nn;
+    }
+  };
+  const int me = nslot(s, n);
   const bool hx = tid < NHX, hy = tid >= HY0 && tid < HY0 + NHY;
   const int hidx = hx ? tid : tid - HY0 + NHX;
   size_t hgi = 0;
@@ -533,19 +548,19 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 #pragma unroll
   for (int v = 0; v < 4; ++v) qn[v] = (valid && a.mode >= 2) ? ln[v] : 0.0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) qs[v][tid] = qo[v];
+  for (int v = 0; v < 4; ++v) qs[v][me] = qo[v];
   {
     double fxo[4], fyo[4];
     flux_x2(gam, qo, fxo);
     flux_y2(gam, qo, fyo);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) { Fx[v][tid] = fxo[v]; Fy[v][tid] = fyo[v]; }
+    for (int v = 0; v < 4; ++v) { Fx[v][me] = fxo[v]; Fy[v][me] = fyo[v]; }
   }
   const bool face = (i == 0 || i == NP - 1 || j == 0 || j == NP - 1);
   if (valid && face) {
     const double c = sound2(gam, qo, a.flag);
-    wsx[tid] = fabs(qo[0] / qo[2]) + c;
-    wsy[tid] = fabs(qo[1] / qo[2]) + c;
+    wsx[me] = fabs(qo[0] / qo[2]) + c;
+    wsy[me] = fabs(qo[1] / qo[2]) + c;
   }
   if (hx || hy) {
     double qh[5], fh[4];
@@ -575,9 +590,9 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * sD[i * NP + k] * Fx[v][s * NP2 + k + NP * j];
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJx * sD[i * NP + k] * Fx[v][nslot(s, k + NP * j)];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * sD[j * NP + k] * Fy[v][s * NP2 + i + NP * k];
+      for (int v = 0; v < 4; ++v) acc[v] -= g.invJy * sD[j * NP + k] * Fy[v][nslot(s, i + NP * k)];
     }
   }
 
@@ -634,33 +649,34 @@ __device__ __forceinline__ void k2_hyp_body(const HypArgs& a, const int blk, con
       if (w < NXR) {                                   // right face of element e, node q along it
         const int e = w / NP, q = w % NP;
         if (e % TX < t.nvx && e / TX < t.nvy) {
-          const int L = e * NP2 + (NP - 1) + NP * q;
+          const int L = nslot(e, (NP - 1) + NP * q);
           const int eR = t.nb_right(e);
-          const int R = eR < T ? eR * NP2 + NP * q : NT + (eR - T) * NP + q;
+          const int R = eR < T ? nslot(eR, NP * q) : NT + (eR - T) * NP + q;
           face_item(std::integral_constant<int, 0>{}, L, R, L, R, e * 2 * NP + NP + q,
                     eR < T ? eR * 2 * NP + q : -1);
         }
       } else if (w >= O1 && w < O1 + NYT) {            // top face
         const int e = (w - O1) / NP, q = (w - O1) % NP;
         if (e % TX < t.nvx && e / TX < t.nvy) {
-          const int L = e * NP2 + q + NP * (NP - 1);
+          const int L = nslot(e, q + NP * (NP - 1));
           const int eT = t.nb_top(e);
           const int h = (eT - T) * NP + q;              // halo slot when eT >= T
-          face_item(std::integral_constant<int, 1>{}, L, eT < T ? eT * NP2 + q : NT + h, L,
-                    eT < T ? eT * NP2 + q : NT + h - NHX, e * 2 * NP + NP + q, eT < T ? eT * 2 * NP + q : -1);
+          const int Rt = eT < T ? nslot(eT, q) : 0;
+          face_item(std::integral_constant<int, 1>{}, L, eT < T ? Rt : NT + h, L,
+                    eT < T ? Rt : NT + h - NHX, e * 2 * NP + NP + q, eT < T ? eT * 2 * NP + q : -1);
         }
       } else if (w >= O2 && w < O2 + NXL) {            // left face of the first column
         const int r = (w - O2) / NP, q = (w - O2) % NP;
         if (r < t.nvy) {
           const int e = r * TX;
-          const int R = e * NP2 + NP * q;
+          const int R = nslot(e, NP * q);
           const int L = NT + r * NP + q;                  // left strip, slot r
           face_item(std::integral_constant<int, 0>{}, L, R, L, R, -1, e * 2 * NP + q);
         }
       } else if (w >= O3 && w < OE) {                  // bottom face of the first row
         const int e = (w - O3) / NP, q = (w - O3) % NP;
         if (e < t.nvx) {
-          const int R = e * NP2 + q;
+          const int R = nslot(e, q);
           const int h = (2 * TY + e) * NP + q;            // bottom strip, slot e
           face_item(std::integral_constant<int, 1>{}, NT + h, R, NT + h - NHX, R, -1, e * 2 * NP + q);
         }
