@@ -48,7 +48,7 @@ _omp_lib = None
 
 def omp_lib():
     """The OpenMP build of the same oracle (identical results; used to time
-    the multi-core CPU baseline, never as the checker)."""
+    the multi-core CPU baseline, never to check results)."""
     global _omp_lib
     if _omp_lib is None:
         if not os.path.exists(_OMP_PATH):
